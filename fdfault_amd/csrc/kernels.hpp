@@ -1,0 +1,983 @@
+// HIP kernels of the MI355X time-stepping core (gfx950, wave64).
+//
+// One RK stage is two launches per interface / block:
+//   fault_kernel : one thread per fault point; gathers both sides of the
+//                  interface from f (read-only this stage), rotates into the
+//                  fault frame, solves the friction law in FP64, writes the
+//                  SAT terms for both sides into a small per-interface buffer
+//                  and advances slip/state with the low-storage RK update.
+//   bulk_kernel  : one thread per grid point; df = A*df + dt*RHS(f) with the
+//                  SBP interior/boundary rows, then the external-boundary SAT
+//                  of the point's faces, then the interface SAT terms written
+//                  by fault_kernel, then f_new = f + B*df into the other f
+//                  buffer (ping-pong), so f and df are each read and written
+//                  once per stage.
+// In FP64 every expression keeps the association of the reference routine it
+// restates (cited at each function) and the code is compiled with
+// -ffp-contract=off, so results track the reference to the last bit.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace fdf {
+
+// SBP tables in constant memory: row 0 interior, rows 1.. boundary rows
+__constant__ double c_fd[7][9];
+__constant__ double c_diss[7][9];
+
+struct FaceDev {  // external boundary face (boundary.cpp:15-240)
+  int active;
+  int n0, n1;
+  double r, alpha, theta, cp, cs, zp, zs, gamma;
+  const double* nrm;  // [ndim][n0][n1] outward unit normal
+  const double* dl;   // [n0][n1]
+};
+
+struct IfFaceDev {    // an interface face touching this block
+  int side;           // 0: this block is block 1 (minus side), 1: block 2
+  int dir;            // interface normal direction
+  int fixed;          // global index along dir of the face row in this block
+  int a, b;           // in-plane axes
+  int org_a, org_b;   // global in-plane origin
+  int n;              // points in the interface
+  int n1;
+  const double* sat;  // [2 sides][2 terms][nf][n]
+};
+
+struct BlockDev {
+  int lo[3], n[3];    // first point and extents (global indices)
+  int mc[3], mrb[3];  // end of left boundary rows, start of right boundary rows
+  int hi[3];          // lo + n
+  double K[3][4];     // per direction: dt/rho/dx, dt(2G+lam)/dx, dt G/dx, dt lam/dx
+  double dx[3];
+  double nu;          // plane-strain factor lam/(2(lam+G)) for plastic mode II
+  double lam, G;
+  double pmu, pc, pbeta, peta;
+  int nif;
+  FaceDev face[6];
+  IfFaceDev ifc[6];
+};
+
+struct GridDev {
+  int ndim, mode, nf, s;
+  int Ny, Nz;
+  long long N;
+  int plastic, hetmat, hetstress;
+  double cdiss;
+  double s0[6];
+  int ns, nv, sidx[6];
+  int nfp, ptensor;
+};
+
+struct StageDev {
+  double A, B, dt;
+  int roundtrip;  // end of an elastic step: the set_stress/remove_stress rounding of problem.cpp:160-164
+};
+
+// rotated boundary fields, boundary.cpp:505-568
+struct BFd {
+  double v1, v2, v3, s11, s12, s13, s22, s23, s33;
+};
+
+__device__ __forceinline__ BFd rot_to_nt(const BFd& b, const double* nn, const double* t1, const double* t2) {
+  BFd o;
+  o.v1 = nn[0] * b.v1 + nn[1] * b.v2 + nn[2] * b.v3;
+  o.v2 = t1[0] * b.v1 + t1[1] * b.v2 + t1[2] * b.v3;
+  o.v3 = t2[0] * b.v1 + t2[1] * b.v2 + t2[2] * b.v3;
+  o.s11 = (nn[0] * (b.s11 * nn[0] + b.s12 * nn[1] + b.s13 * nn[2]) + nn[1] * (b.s12 * nn[0] + b.s22 * nn[1] + b.s23 * nn[2]) +
+           nn[2] * (b.s13 * nn[0] + b.s23 * nn[1] + b.s33 * nn[2]));
+  o.s12 = (nn[0] * (b.s11 * t1[0] + b.s12 * t1[1] + b.s13 * t1[2]) + nn[1] * (b.s12 * t1[0] + b.s22 * t1[1] + b.s23 * t1[2]) +
+           nn[2] * (b.s13 * t1[0] + b.s23 * t1[1] + b.s33 * t1[2]));
+  o.s13 = (nn[0] * (b.s11 * t2[0] + b.s12 * t2[1] + b.s13 * t2[2]) + nn[1] * (b.s12 * t2[0] + b.s22 * t2[1] + b.s23 * t2[2]) +
+           nn[2] * (b.s13 * t2[0] + b.s23 * t2[1] + b.s33 * t2[2]));
+  o.s22 = (t1[0] * (b.s11 * t1[0] + b.s12 * t1[1] + b.s13 * t1[2]) + t1[1] * (b.s12 * t1[0] + b.s22 * t1[1] + b.s23 * t1[2]) +
+           t1[2] * (b.s13 * t1[0] + b.s23 * t1[1] + b.s33 * t1[2]));
+  o.s23 = (t1[0] * (b.s11 * t2[0] + b.s12 * t2[1] + b.s13 * t2[2]) + t1[1] * (b.s12 * t2[0] + b.s22 * t2[1] + b.s23 * t2[2]) +
+           t1[2] * (b.s13 * t2[0] + b.s23 * t2[1] + b.s33 * t2[2]));
+  o.s33 = (t2[0] * (b.s11 * t2[0] + b.s12 * t2[1] + b.s13 * t2[2]) + t2[1] * (b.s12 * t2[0] + b.s22 * t2[1] + b.s23 * t2[2]) +
+           t2[2] * (b.s13 * t2[0] + b.s23 * t2[1] + b.s33 * t2[2]));
+  return o;
+}
+
+__device__ __forceinline__ BFd rot_to_xy(const BFd& b, const double* nn, const double* t1, const double* t2) {
+  BFd o;
+  o.v1 = nn[0] * b.v1 + t1[0] * b.v2 + t2[0] * b.v3;
+  o.v2 = nn[1] * b.v1 + t1[1] * b.v2 + t2[1] * b.v3;
+  o.v3 = nn[2] * b.v1 + t1[2] * b.v2 + t2[2] * b.v3;
+  o.s11 = (nn[0] * (b.s11 * nn[0] + b.s12 * t1[0] + b.s13 * t2[0]) + t1[0] * (b.s12 * nn[0] + b.s22 * t1[0] + b.s23 * t2[0]) +
+           t2[0] * (b.s13 * nn[0] + b.s23 * t1[0] + b.s33 * t2[0]));
+  o.s12 = (nn[0] * (b.s11 * nn[1] + b.s12 * t1[1] + b.s13 * t2[1]) + t1[0] * (b.s12 * nn[1] + b.s22 * t1[1] + b.s23 * t2[1]) +
+           t2[0] * (b.s13 * nn[1] + b.s23 * t1[1] + b.s33 * t2[1]));
+  o.s13 = (nn[0] * (b.s11 * nn[2] + b.s12 * t1[2] + b.s13 * t2[2]) + t1[0] * (b.s12 * nn[2] + b.s22 * t1[2] + b.s23 * t2[2]) +
+           t2[0] * (b.s13 * nn[2] + b.s23 * t1[2] + b.s33 * t2[2]));
+  o.s22 = (nn[1] * (b.s11 * nn[1] + b.s12 * t1[1] + b.s13 * t2[1]) + t1[1] * (b.s12 * nn[1] + b.s22 * t1[1] + b.s23 * t2[1]) +
+           t2[1] * (b.s13 * nn[1] + b.s23 * t1[1] + b.s33 * t2[1]));
+  o.s23 = (nn[1] * (b.s11 * nn[2] + b.s12 * t1[2] + b.s13 * t2[2]) + t1[1] * (b.s12 * nn[2] + b.s22 * t1[2] + b.s23 * t2[2]) +
+           t2[1] * (b.s13 * nn[2] + b.s23 * t1[2] + b.s33 * t2[2]));
+  o.s33 = (nn[2] * (b.s11 * nn[2] + b.s12 * t1[2] + b.s13 * t2[2]) + t1[2] * (b.s12 * nn[2] + b.s22 * t1[2] + b.s23 * t2[2]) +
+           t2[2] * (b.s13 * nn[2] + b.s23 * t1[2] + b.s33 * t2[2]));
+  return o;
+}
+
+template <int M, typename R>
+__device__ __forceinline__ BFd gather_bf(const R* __restrict__ f, long long N, long long id) {
+  BFd b;
+  if constexpr (M == 2) {
+    b.v1 = (double)f[id]; b.v2 = (double)f[N + id]; b.v3 = (double)f[2 * N + id];
+    b.s11 = (double)f[3 * N + id]; b.s12 = (double)f[4 * N + id]; b.s13 = (double)f[5 * N + id];
+    b.s22 = (double)f[6 * N + id]; b.s23 = (double)f[7 * N + id]; b.s33 = (double)f[8 * N + id];
+  } else if constexpr (M == 0) {
+    b.v1 = (double)f[id]; b.v2 = (double)f[N + id]; b.v3 = 0.;
+    b.s11 = (double)f[2 * N + id]; b.s12 = (double)f[3 * N + id]; b.s13 = 0.;
+    b.s22 = (double)f[4 * N + id]; b.s23 = 0.; b.s33 = 0.;
+  } else {
+    b.v1 = 0.; b.v2 = 0.; b.v3 = (double)f[id];
+    b.s11 = 0.; b.s12 = 0.; b.s13 = (double)f[N + id]; b.s22 = 0.; b.s23 = (double)f[2 * N + id]; b.s33 = 0.;
+  }
+  return b;
+}
+
+// ---------------------------------------------------------------------------
+// bulk stage kernel
+
+template <int M>
+struct Nf {
+  static constexpr int v = (M == 0) ? 5 : ((M == 1) ? 3 : 9);
+};
+
+// One stencil tap in direction d: neighbour i2 with coefficient c.  n[] is
+// the metric row d at the point itself (stress terms use it), the velocity
+// terms use the metric row and Jacobian at the neighbour.  Expressions
+// restate block.cpp:1067-1081 (mode II), 1402-1405 (mode III), 1563-1594 (3D).
+template <int M, typename R, typename C>
+__device__ __forceinline__ void tap_terms(C* acc, const R* __restrict__ f, const R* __restrict__ mt,
+                                          const R* __restrict__ jac, long long N, long long i2, C c, bool minus,
+                                          int d, const C* n, C g2lam, C g, C lam, C invjac) {
+  const C J2 = (C)jac[i2];
+  const C w = invjac * c * J2;
+  if constexpr (M == 0) {
+    const C m0 = (C)mt[(2 * d + 0) * N + i2], m1 = (C)mt[(2 * d + 1) * N + i2];
+    const C vx = (C)f[i2], vy = (C)f[N + i2], sxx = (C)f[2 * N + i2], sxy = (C)f[3 * N + i2], syy = (C)f[4 * N + i2];
+    C t[5];
+    t[0] = w * (m0 * sxx + m1 * sxy);
+    t[1] = w * (m0 * sxy + m1 * syy);
+    t[2] = c * (g2lam * n[0] * vx + lam * n[1] * vy);
+    t[3] = g * c * (n[0] * vy + n[1] * vx);
+    t[4] = c * (g2lam * n[1] * vy + lam * n[0] * vx);
+    if (minus) {
+#pragma unroll
+      for (int q = 0; q < 5; q++) acc[q] -= t[q];
+    } else {
+#pragma unroll
+      for (int q = 0; q < 5; q++) acc[q] += t[q];
+    }
+  } else if constexpr (M == 1) {
+    const C m0 = (C)mt[(2 * d + 0) * N + i2], m1 = (C)mt[(2 * d + 1) * N + i2];
+    const C vz = (C)f[i2], sxz = (C)f[N + i2], syz = (C)f[2 * N + i2];
+    C t[3];
+    t[0] = w * (m0 * sxz + m1 * syz);
+    t[1] = g * n[0] * c * vz;
+    t[2] = g * n[1] * c * vz;
+    if (minus) {
+#pragma unroll
+      for (int q = 0; q < 3; q++) acc[q] -= t[q];
+    } else {
+#pragma unroll
+      for (int q = 0; q < 3; q++) acc[q] += t[q];
+    }
+  } else {
+    const C m0 = (C)mt[(3 * d + 0) * N + i2], m1 = (C)mt[(3 * d + 1) * N + i2], m2 = (C)mt[(3 * d + 2) * N + i2];
+    const C vx = (C)f[i2], vy = (C)f[N + i2], vz = (C)f[2 * N + i2];
+    const C sxx = (C)f[3 * N + i2], sxy = (C)f[4 * N + i2], sxz = (C)f[5 * N + i2];
+    const C syy = (C)f[6 * N + i2], syz = (C)f[7 * N + i2], szz = (C)f[8 * N + i2];
+    C t[9];
+    t[0] = w * (m0 * sxx + m1 * sxy + m2 * sxz);
+    t[1] = w * (m0 * sxy + m1 * syy + m2 * syz);
+    t[2] = w * (m0 * sxz + m1 * syz + m2 * szz);
+    t[3] = c * (g2lam * n[0] * vx + lam * (n[1] * vy + n[2] * vz));
+    t[4] = c * g * (n[0] * vy + n[1] * vx);
+    t[5] = c * g * (n[0] * vz + n[2] * vx);
+    t[6] = c * (g2lam * n[1] * vy + lam * (n[0] * vx + n[2] * vz));
+    t[7] = c * g * (n[1] * vz + n[2] * vy);
+    t[8] = c * (g2lam * n[2] * vz + lam * (n[0] * vx + n[1] * vy));
+    if (minus) {
+#pragma unroll
+      for (int q = 0; q < 9; q++) acc[q] -= t[q];
+    } else {
+#pragma unroll
+      for (int q = 0; q < 9; q++) acc[q] += t[q];
+    }
+  }
+}
+
+// external boundary SAT of one face at one point, boundary.cpp:265-492.
+// Fault-frame algebra in FP64 whatever the bulk precision.
+template <int M, typename R, typename C>
+__device__ void ext_sat(C* acc, const FaceDev& F, const R* __restrict__ f, const R* __restrict__ mat,
+                        const GridDev& g, long long id, int u, int v, double dt) {
+  const long long N = g.N;
+  const long long nface = (long long)F.n0 * F.n1;
+  const long long o = (long long)u * F.n1 + v;
+  const int ndim = (M == 2) ? 3 : 2;
+  double nn[3] = {0., 0., 0.}, t1[3], t2[3];
+  for (int k = 0; k < ndim; k++) nn[k] = F.nrm[k * nface + o];
+  const double h = dt * F.dl[o];
+  double cp = F.cp, cs = F.cs, zp = F.zp, zs = F.zs, gamma = F.gamma;
+  if (g.hetmat) {
+    const double rho = (double)mat[id];
+    if constexpr (M == 1) {
+      cs = sqrt((double)mat[N + id] / rho);
+      zs = rho * cs;
+    } else {
+      const double lam = (double)mat[N + id], G = (double)mat[2 * N + id];
+      cp = sqrt((lam + 2. * G) / rho);
+      cs = sqrt(G / rho);
+      zp = rho * cp;
+      zs = rho * cs;
+      const double q = cs / cp;
+      gamma = 1. - 2. * (q * q);
+    }
+  }
+  if (fabs(nn[0]) > fabs(nn[1]) && fabs(nn[0]) > fabs(nn[2])) {  // boundary.cpp:316-331
+    t1[2] = 0.;
+    t1[1] = nn[0] / sqrt(nn[0] * nn[0] + nn[1] * nn[1]);
+    t1[0] = -nn[1] / sqrt(nn[0] * nn[0] + nn[1] * nn[1]);
+  } else if (fabs(nn[1]) > fabs(nn[2])) {
+    t1[2] = 0.;
+    t1[0] = nn[1] / sqrt(nn[0] * nn[0] + nn[1] * nn[1]);
+    t1[1] = -nn[0] / sqrt(nn[0] * nn[0] + nn[1] * nn[1]);
+  } else {
+    t1[1] = 0.;
+    t1[0] = nn[2] / sqrt(nn[0] * nn[0] + nn[2] * nn[2]);
+    t1[2] = -nn[0] / sqrt(nn[0] * nn[0] + nn[2] * nn[2]);
+  }
+  t2[0] = nn[1] * t1[2] - nn[2] * t1[1];
+  t2[1] = nn[2] * t1[0] - nn[0] * t1[2];
+  t2[2] = nn[0] * t1[1] - nn[1] * t1[0];
+  BFd br = rot_to_nt(gather_bf<M, R>(f, N, id), nn, t1, t2);
+  BFd brs = br;
+  // characteristic targets (calc_hat, boundary.cpp:494-503)
+  const double ps = (F.r + 1.) * 0.5 * (br.s11 - zp * br.v1), pv = (F.r - 1.) * 0.5 * (br.s11 / zp - br.v1);
+  const double s1s = (F.r + 1.) * 0.5 * (br.s12 - zs * br.v2), s1v = (F.r - 1.) * 0.5 * (br.s12 / zs - br.v2);
+  const double s2s = (F.r + 1.) * 0.5 * (br.s13 - zs * br.v3), s2v = (F.r - 1.) * 0.5 * (br.s13 / zs - br.v3);
+  br.v1 -= pv;
+  br.v2 = 0.;
+  br.v3 = 0.;
+  br.s22 = gamma * (br.s11 - ps);
+  br.s33 = gamma * (br.s11 - ps);
+  br.s11 -= ps;
+  br.s12 = 0.;
+  br.s13 = 0.;
+  br.s23 = 0.;
+  BFd o1 = rot_to_xy(br, nn, t1, t2);
+  if constexpr (M == 2) {
+    const double* a = &o1.v1;
+#pragma unroll
+    for (int q = 0; q < 3; q++) acc[q] = (C)((double)acc[q] - F.alpha * cp * h * a[q]);
+#pragma unroll
+    for (int q = 3; q < 9; q++) acc[q] = (C)((double)acc[q] - F.theta * cp * h * a[q]);
+  } else if constexpr (M == 0) {
+    acc[0] = (C)((double)acc[0] - cp * h * o1.v1);
+    acc[1] = (C)((double)acc[1] - cp * h * o1.v2);
+    acc[2] = (C)((double)acc[2] - cp * h * o1.s11);
+    acc[3] = (C)((double)acc[3] - cp * h * o1.s12);
+    acc[4] = (C)((double)acc[4] - cp * h * o1.s22);
+    if (g.plastic) acc[5] = (C)((double)acc[5] - cp * h * o1.s33);
+  }
+  brs.v1 = 0.;
+  brs.v2 -= s1v;
+  brs.v3 -= s2v;
+  brs.s11 = 0.;
+  brs.s12 -= s1s;
+  brs.s13 -= s2s;
+  brs.s22 = 0.;
+  brs.s23 = 0.;
+  brs.s33 = 0.;
+  BFd o2 = rot_to_xy(brs, nn, t1, t2);
+  if constexpr (M == 2) {
+    const double* a = &o2.v1;
+#pragma unroll
+    for (int q = 0; q < 3; q++) acc[q] = (C)((double)acc[q] - F.alpha * cs * h * a[q]);
+#pragma unroll
+    for (int q = 3; q < 9; q++) acc[q] = (C)((double)acc[q] - F.theta * cs * h * a[q]);
+  } else if constexpr (M == 0) {
+    acc[0] = (C)((double)acc[0] - cs * h * o2.v1);
+    acc[1] = (C)((double)acc[1] - cs * h * o2.v2);
+    acc[2] = (C)((double)acc[2] - cs * h * o2.s11);
+    acc[3] = (C)((double)acc[3] - cs * h * o2.s12);
+    acc[4] = (C)((double)acc[4] - cs * h * o2.s22);
+  } else {
+    acc[0] = (C)((double)acc[0] - cs * h * o2.v3);
+    acc[1] = (C)((double)acc[1] - cs * h * o2.s13);
+    acc[2] = (C)((double)acc[2] - cs * h * o2.s23);
+  }
+}
+
+// Fused stage update of one block: block::calc_df + set_boundaries
+// (block.cpp:405-440), the interface SAT additions (interface.cpp:826-962,
+// precomputed by fault_kernel), and fields::update (fields.cpp:302-309).
+// Per point the accumulation order is the reference's: all x taps, then y,
+// then z; each tap adds every field term then the dissipation terms.
+template <int M, typename R, typename C, bool DISS, bool HET>
+__global__ void __launch_bounds__(256) bulk_kernel(BlockDev bd, GridDev g, StageDev st, const R* __restrict__ f,
+                                                   R* __restrict__ fn, R* __restrict__ df, const R* __restrict__ mt,
+                                                   const R* __restrict__ jac, const R* __restrict__ mat,
+                                                   const double* __restrict__ hs) {
+  constexpr int NFE = Nf<M>::v;
+  constexpr int ND = (M == 2) ? 3 : 2;
+  const long long npts = (long long)bd.n[0] * bd.n[1] * bd.n[2];
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= npts) return;
+  const int lk = (int)(t % bd.n[2]);
+  const long long r = t / bd.n[2];
+  const int lj = (int)(r % bd.n[1]);
+  const int li = (int)(r / bd.n[1]);
+  const int c3[3] = {bd.lo[0] + li, bd.lo[1] + lj, bd.lo[2] + lk};
+  const long long N = g.N;
+  const long long stride[3] = {(long long)g.Ny * g.Nz, (long long)g.Nz, 1};
+  const long long id = c3[0] * stride[0] + c3[1] * stride[1] + c3[2];
+  const int s = g.s;
+  const int NF = g.nf;  // may include szz (plastic mode II)
+
+  double K[3][4];
+  if (HET) {
+    const double rho = (double)mat[id];
+    for (int d = 0; d < ND; d++) {
+      if constexpr (M == 1) {
+        const double G = (double)mat[N + id];
+        K[d][0] = st.dt / rho / bd.dx[d];
+        K[d][2] = st.dt * G / bd.dx[d];
+        K[d][1] = 0.;
+        K[d][3] = 0.;
+      } else {
+        const double lam = (double)mat[N + id], G = (double)mat[2 * N + id];
+        K[d][0] = st.dt / rho / bd.dx[d];
+        K[d][1] = st.dt * (2. * G + lam) / bd.dx[d];
+        K[d][2] = st.dt * G / bd.dx[d];
+        K[d][3] = st.dt * lam / bd.dx[d];
+      }
+    }
+  } else {
+    for (int d = 0; d < ND; d++)
+      for (int q = 0; q < 4; q++) K[d][q] = bd.K[d][q];
+  }
+
+  C acc[NFE + 1];
+#pragma unroll
+  for (int q = 0; q < NFE; q++) acc[q] = (C)df[q * N + id] * (C)st.A;
+  acc[NFE] = (NF > NFE) ? (C)df[NFE * N + id] * (C)st.A : (C)0;
+  const C J1 = (C)jac[id];
+
+#pragma unroll
+  for (int d = 0; d < ND; d++) {
+    const C invjac = (C)K[d][0] / J1;
+    C nrow[3];
+#pragma unroll
+    for (int m = 0; m < ND; m++) nrow[m] = (C)mt[(ND * d + m) * N + id];
+    const int pos = c3[d];
+    int row, ntap, start, step;
+    bool minus = false;
+    if (pos < bd.mc[d]) {
+      row = pos - bd.lo[d] + 1;
+      ntap = 3 * (s - 1);
+      start = bd.lo[d];
+      step = 1;
+    } else if (pos < bd.mrb[d]) {
+      row = 0;
+      ntap = 2 * s - 1;
+      start = pos - s + 1;
+      step = 1;
+    } else {
+      row = bd.hi[d] - pos;
+      ntap = 3 * (s - 1);
+      start = bd.hi[d] - 1;
+      step = -1;
+      minus = true;
+    }
+    const long long base = id - (long long)pos * stride[d];
+    for (int nt = 0; nt < ntap; nt++) {
+      const long long i2 = base + (long long)(start + step * nt) * stride[d];
+      tap_terms<M, R, C>(acc, f, mt, jac, N, i2, (C)c_fd[row][nt], minus, d, nrow, (C)K[d][1], (C)K[d][2], (C)K[d][3],
+                         invjac);
+      if (DISS) {
+        const C dc = (C)c_diss[row][nt];
+        const C J2 = (C)jac[i2];
+#pragma unroll
+        for (int q = 0; q < NFE; q++) acc[q] += (C)g.cdiss * (dc * J2 * (C)f[q * N + i2]) / J1;
+      }
+    }
+  }
+  if constexpr (M == 0) if (NF > NFE) {  // block::calc_df_szz, block.cpp:1280-1295 (assignment, not increment)
+    double nu = bd.nu;
+    if (HET) nu = 0.5 * (double)mat[N + id] / ((double)mat[N + id] + (double)mat[stride[0] + id]);
+    acc[NFE] = (C)nu * (acc[2] + acc[4]);
+  }
+  // external boundaries of this block, in face order (block.cpp:429-440)
+#pragma unroll
+  for (int fc = 0; fc < 2 * ND; fc++) {
+    const FaceDev& F = bd.face[fc];
+    if (!F.active) continue;
+    const int d = fc / 2;
+    const int edge = (fc & 1) ? bd.hi[d] - 1 : bd.lo[d];
+    if (c3[d] != edge) continue;
+    const int a = (d == 0) ? 1 : 0, b = (d == 2) ? 1 : 2;
+    ext_sat<M, R, C>(acc, F, f, mat, g, id, c3[a] - bd.lo[a], c3[b] - bd.lo[b], st.dt);
+  }
+  // interface SAT terms in interface order (domain.cpp:251-256)
+  for (int q = 0; q < bd.nif; q++) {
+    const IfFaceDev& I = bd.ifc[q];
+    if (c3[I.dir] != I.fixed) continue;
+    const long long o = (long long)(c3[I.a] - I.org_a) * I.n1 + (c3[I.b] - I.org_b);
+    const double* sn = I.sat + (long long)(I.side * 2 + 0) * NF * I.n;
+    const double* sv = I.sat + (long long)(I.side * 2 + 1) * NF * I.n;
+    for (int p = 0; p < NF; p++) {
+      C v = (p < NFE) ? acc[p] : acc[NFE];
+      v = (C)((double)v - sn[(long long)p * I.n + o]);
+      v = (C)((double)v - sv[(long long)p * I.n + o]);
+      if (p < NFE) acc[p] = v; else acc[NFE] = v;
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < NFE; q++) df[q * N + id] = (R)acc[q];
+  if (NF > NFE) df[NFE * N + id] = (R)acc[NFE];
+  // RK update into the other f buffer, plus the per-step stress round trip
+  C upd[NFE + 1];
+#pragma unroll
+  for (int q = 0; q < NFE; q++) upd[q] = (C)f[q * N + id] + (C)st.B * acc[q];
+  if (NF > NFE) upd[NFE] = (C)f[NFE * N + id] + (C)st.B * acc[NFE];
+  if (st.roundtrip) {
+    for (int i = 0; i < g.ns; i++) {
+      const int q = g.nv + i;
+      const double s0 = g.s0[g.sidx[i]];
+      double v = (double)((q < NFE) ? upd[q] : upd[NFE]);
+      v += s0;
+      if (g.hetstress) v += hs[i * N + id];
+      v -= s0;
+      if (g.hetstress) v -= hs[i * N + id];
+      if (q < NFE) upd[q] = (C)v; else upd[NFE] = (C)v;
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < NFE; q++) fn[q * N + id] = (R)upd[q];
+  if (NF > NFE) fn[NFE * N + id] = (R)upd[NFE];
+}
+
+// ---------------------------------------------------------------------------
+// fault kernel
+
+struct FaultDev {
+  int law, dir, a, b;
+  int n0, n1, n;
+  int org[3];
+  double cp1, cs1, zp1, zs1, cp2, cs2, zp2, zs2, gamma1, gamma2;
+  const double* nrm;
+  const double* dl1;
+  const double* dl2;
+  int nloads, load_file, nperts, param_file;
+  const double* load_xy;   // [nloads][n]
+  const double* load_par;  // [nloads][10]: t0, amplitudes
+  const double* lf;        // [3][n]
+  const double* pert_xy;   // [nperts][n]
+  const double* pert_par;  // [nperts][10]
+  const double* pf;        // [9][n]
+  double *u, *du, *v, *s, *sn, *ux, *dux, *vx, *sx, *state, *dstate;
+  double* sat;             // [2][2][nf][n]
+  double* front;           // [n] or null
+  double front_value;
+  int front_field;
+};
+
+struct FaultStageDev {
+  double A, B, dt, t;
+  int no_sat;
+  double front_t;  // < 0: no front update
+};
+
+__device__ __forceinline__ double pert_val(const double* xy, const double* par, int k, int q, int n, int p, double t) {
+  const double t0 = par[k * 10];
+  const double tf = (t >= t0) ? 1. : t / t0;
+  return par[k * 10 + 1 + q] * xy[(long long)k * n + p] * tf;
+}
+
+// STZ slip rate and its derivative, stz.cpp:436-484
+__device__ __forceinline__ double stz_vpl(double mu, const double* p) {
+  if (mu <= p[7]) return 0.;
+  return p[4] * exp(-p[5] + mu / p[6] - 1. / p[3]) * (1. - p[7] / mu);
+}
+__device__ __forceinline__ double stz_f(double mu, const double* p) { return p[1] * stz_vpl(mu, p) - p[2] * mu - p[0]; }
+__device__ __forceinline__ double stz_df(double mu, const double* p) {
+  double d = (mu <= p[7]) ? 0. : stz_vpl(mu, p) * (1. / p[6] - p[7] / mu / (mu - p[7]));
+  return p[1] * d - p[2];
+}
+
+// bracketed Newton with bisection fallback, utilities.cpp:89-180; err codes:
+// 1 zero derivative, 2 no convergence, 4 overflow repair did not terminate
+__device__ double stz_newton(double xmin, double xmax, const double* p, int* err, int* iters) {
+  int i = 0;
+  const int nmax = 100;
+  const double tol = 2.e-16;
+  double x = 0.5 * (xmin + xmax), xh = xmax, xl = xmin;
+  double func = stz_f(x, p);
+  const double fh = stz_f(xh, p);
+  if (isinf(fh)) {
+    int guard = 0;
+    while (true) {
+      if (!isinf(func)) {
+        if (func > 0.) { xh = x; break; }
+        xl = x;
+      } else {
+        xh = x;
+      }
+      x = 0.5 * (xh + xl);
+      func = stz_f(x, p);
+      if (++guard > 4000) { *err |= 4; break; }
+    }
+  }
+  x = 0.5 * (xh + xl);
+  func = stz_f(x, p);
+  double der = stz_df(x, p);
+  double dx = fabs(xh - xl), dxold = dx;
+  while (i < nmax) {
+    if (der == 0.) { *err |= 1; break; }
+    if ((((x - xh) * der - func) * ((x - xl) * der - func) > 0.) || fabs(2. * func) > fabs(dxold * der)) {
+      dxold = dx;
+      dx = 0.5 * (xh - xl);
+      x = xl + dx;
+      if (xl == x) break;
+    } else {
+      dxold = dx;
+      dx = func / der;
+      const double tmp = x;
+      x = x - dx;
+      if (tmp == x) break;
+    }
+    if (fabs(dx) < tol) break;
+    func = stz_f(x, p);
+    der = stz_df(x, p);
+    if (func < 0.) xl = x; else xh = x;
+    i++;
+  }
+  if (i == nmax) *err |= 2;
+  *iters = i;
+  return x;
+}
+
+// interface::apply_bcs for one fault point (interface.cpp:573-970) with the
+// friction laws of friction.cpp:259-435, slipweak.cpp:206-308 and
+// stz.cpp:263-356, then friction::calc_df/update (friction.cpp:465-505).
+template <int M, typename R>
+__global__ void __launch_bounds__(256) fault_kernel(FaultDev F, GridDev g, FaultStageDev fs, const R* __restrict__ f,
+                                                    const R* __restrict__ mat, const double* __restrict__ hs,
+                                                    int* __restrict__ err, int* __restrict__ newton_iters) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= F.n) return;
+  const int ii = p / F.n1, jj = p % F.n1;
+  const long long N = g.N;
+  const long long stride[3] = {(long long)g.Ny * g.Nz, (long long)g.Nz, 1};
+  int c[3];
+  c[F.dir] = F.org[F.dir];
+  c[F.a] = F.org[F.a] + ii;
+  c[F.b] = F.org[F.b] + jj;
+  const long long id1 = c[0] * stride[0] + c[1] * stride[1] + c[2];
+  const long long id2 = id1 + stride[F.dir];
+  const double h1 = fs.dt * F.dl1[p], h2 = fs.dt * F.dl2[p];
+  double cp1 = F.cp1, cs1 = F.cs1, zp1 = F.zp1, zs1 = F.zs1, cp2 = F.cp2, cs2 = F.cs2, zp2 = F.zp2, zs2 = F.zs2;
+  double gamma1 = F.gamma1, gamma2 = F.gamma2;
+  if (g.hetmat) {
+    const double r1 = (double)mat[id1], r2 = (double)mat[id2];
+    if constexpr (M == 1) {
+      cs1 = sqrt((double)mat[N + id1] / r1); zs1 = r1 * cs1;
+      cs2 = sqrt((double)mat[N + id2] / r2); zs2 = r2 * cs2;
+    } else {
+      cp1 = sqrt(((double)mat[N + id1] + 2. * (double)mat[2 * N + id1]) / r1);
+      cs1 = sqrt((double)mat[2 * N + id1] / r1);
+      zp1 = r1 * cp1; zs1 = r1 * cs1;
+      double q = cs1 / cp1; gamma1 = 1. - 2. * (q * q);
+      cp2 = sqrt(((double)mat[N + id2] + 2. * (double)mat[2 * N + id2]) / r2);
+      cs2 = sqrt((double)mat[2 * N + id2] / r2);
+      zp2 = r2 * cp2; zs2 = r2 * cs2;
+      q = cs2 / cp2; gamma2 = 1. - 2. * (q * q);
+    }
+  }
+  double nn[3] = {0., 0., 0.}, t1[3], t2[3];
+  const int ndim = (M == 2) ? 3 : 2;
+  for (int k = 0; k < ndim; k++) nn[k] = F.nrm[(long long)k * F.n + p];
+  if (F.dir == 0) {
+    t1[2] = 0.;
+    t1[1] = nn[0] / sqrt(nn[0] * nn[0] + nn[1] * nn[1]);
+    t1[0] = -nn[1] / sqrt(nn[0] * nn[0] + nn[1] * nn[1]);
+  } else if (F.dir == 1) {
+    t1[2] = 0.;
+    t1[0] = nn[1] / sqrt(nn[0] * nn[0] + nn[1] * nn[1]);
+    t1[1] = -nn[0] / sqrt(nn[0] * nn[0] + nn[1] * nn[1]);
+  } else {
+    t1[1] = 0.;
+    t1[0] = nn[2] / sqrt(nn[0] * nn[0] + nn[2] * nn[2]);
+    t1[2] = -nn[0] / sqrt(nn[0] * nn[0] + nn[2] * nn[2]);
+  }
+  t2[0] = nn[1] * t1[2] - nn[2] * t1[1];
+  t2[1] = nn[2] * t1[0] - nn[0] * t1[2];
+  t2[2] = nn[0] * t1[1] - nn[1] * t1[0];
+  if (F.dir == 1) { t2[0] = -t2[0]; t2[1] = -t2[1]; t2[2] = -t2[2]; }
+  BFd b1 = gather_bf<M, R>(f, N, id1), b2 = gather_bf<M, R>(f, N, id2);
+  const double* s0 = g.s0;
+  if constexpr (M == 2) {
+    b1.s11 += s0[0]; b1.s12 += s0[1]; b1.s13 += s0[2]; b1.s22 += s0[3]; b1.s23 += s0[4]; b1.s33 += s0[5];
+    b2.s11 += s0[0]; b2.s12 += s0[1]; b2.s13 += s0[2]; b2.s22 += s0[3]; b2.s23 += s0[4]; b2.s33 += s0[5];
+    if (g.hetstress) {
+      b1.s11 += hs[id1]; b1.s12 += hs[N + id1]; b1.s13 += hs[2 * N + id1];
+      b1.s22 += hs[3 * N + id1]; b1.s23 += hs[4 * N + id1]; b1.s33 += hs[5 * N + id1];
+      b2.s11 += hs[id2]; b2.s12 += hs[N + id2]; b2.s13 += hs[2 * N + id2];
+      b2.s22 += hs[3 * N + id2]; b2.s23 += hs[4 * N + id2]; b2.s33 += hs[5 * N + id2];
+    }
+  } else if constexpr (M == 0) {
+    b1.s11 += s0[0]; b1.s12 += s0[1]; b1.s22 += s0[3];
+    b2.s11 += s0[0]; b2.s12 += s0[1]; b2.s22 += s0[3];
+    if (g.hetstress) {
+      b1.s11 += hs[id1]; b1.s12 += hs[N + id1]; b1.s22 += hs[2 * N + id1];
+      b2.s11 += hs[id2]; b2.s12 += hs[N + id2]; b2.s22 += hs[2 * N + id2];
+    }
+  } else {
+    b1.s11 = s0[0]; b1.s13 += s0[2]; b1.s22 = s0[3]; b1.s23 += s0[4];
+    b2.s11 = s0[0]; b2.s13 += s0[2]; b2.s22 = s0[3]; b2.s23 += s0[4];
+    if (g.hetstress) {
+      b1.s13 += hs[id1]; b1.s23 += hs[N + id1];
+      b2.s13 += hs[id2]; b2.s23 += hs[N + id2];
+    }
+  }
+  BFd r1 = rot_to_nt(b1, nn, t1, t2), r2 = rot_to_nt(b2, nn, t1, t2);
+  BFd rs1 = r1, rs2 = r2;
+  // locked normal characteristic (interface.cpp:1017-1028)
+  auto locked = [](double v1, double v2, double sa, double sb, double z1, double z2, double& vh, double& sh) {
+    sh = (z2 * (sa - z1 * v1) + z1 * (sb + z2 * v2)) / (z1 + z2);
+    vh = (sh - sa) / z1 + v1;
+  };
+  double hv11, hs11, hv12, hs12, hv13, hs13, hv22, hs22, hv23, hs23;
+  locked(r1.v1, r2.v1, r1.s11, r2.s11, zp1, zp2, hv11, hs11);
+  const double hv21 = hv11, hs21 = hs11;
+  if (F.law == 0) {
+    locked(r1.v2, r2.v2, r1.s12, r2.s12, zs1, zs2, hv12, hs12);
+    locked(r1.v3, r2.v3, r1.s13, r2.s13, zs1, zs2, hv13, hs13);
+    hv22 = hv12; hs22 = hs12; hv23 = hv13; hs23 = hs13;
+  } else {
+    double snc = hs11;
+    double s12 = r1.s12, s22 = r2.s12, s13 = r1.s13, s23 = r2.s13;
+    const double v12 = r1.v2, v22 = r2.v2, v13 = r1.v3, v23 = r2.v3;
+    const double z1 = zs1, z2 = zs2;
+    const double eta = z1 * z2 / (z1 + z2);
+    const double t = fs.t;
+    const int n = F.n;
+    if (F.load_file) {
+      snc += F.lf[p];
+      s12 += F.lf[n + p]; s22 += F.lf[n + p];
+      s13 += F.lf[2 * n + p]; s23 += F.lf[2 * n + p];
+    }
+    for (int k = 0; k < F.nloads; k++) {
+      snc += pert_val(F.load_xy, F.load_par, k, 0, n, p, t);
+      const double l2 = pert_val(F.load_xy, F.load_par, k, 1, n, p, t);
+      const double l3 = pert_val(F.load_xy, F.load_par, k, 2, n, p, t);
+      s12 += l2; s22 += l2; s13 += l3; s23 += l3;
+    }
+    const double phi2 = eta * (s12 / z1 - v12 + s22 / z2 + v22);
+    const double phi3 = eta * (s13 / z1 - v13 + s23 / z2 + v23);
+    const double phi = sqrt(phi2 * phi2 + phi3 * phi3);
+    double V, S, dstatedt = 0.;
+    if (F.law == 2) {  // slipweak::solve_fs / calc_mu
+      double c0t = 0.;
+      for (int k = 0; k < F.nperts; k++) c0t += pert_val(F.pert_xy, F.pert_par, k, 3, n, p, t);
+      if (F.param_file) c0t += F.pf[3 * n + p];
+      if (snc < 0.) {
+        double dct = 0., must = 0., mudt = 0., trupt = 0., tct = 0.;
+        for (int k = 0; k < F.nperts; k++) {
+          dct += pert_val(F.pert_xy, F.pert_par, k, 0, n, p, t);
+          must += pert_val(F.pert_xy, F.pert_par, k, 1, n, p, t);
+          mudt += pert_val(F.pert_xy, F.pert_par, k, 2, n, p, t);
+          trupt += pert_val(F.pert_xy, F.pert_par, k, 4, n, p, t);
+          tct += pert_val(F.pert_xy, F.pert_par, k, 5, n, p, t);
+        }
+        if (F.param_file) {
+          dct += F.pf[p]; must += F.pf[n + p]; mudt += F.pf[2 * n + p];
+          trupt += F.pf[4 * n + p]; tct += F.pf[5 * n + p];
+        }
+        const double u = F.u[p];
+        const double f1 = (dct == 0. || u >= dct) ? 1. : u / dct;
+        double f2;
+        if (trupt <= 0. || t < trupt) f2 = 0.;
+        else if (t >= trupt && t < trupt + tct) f2 = (tct == 0.) ? 1. : (t - trupt) / tct;
+        else f2 = 1.;
+        const double mu = (f1 >= f2) ? must + (mudt - must) * f1 : must + (mudt - must) * f2;
+        if (c0t + mu * fabs(snc) > phi) { V = 0.; S = phi; }
+        else { S = c0t + mu * fabs(snc); V = (phi - S) / eta; }
+      } else {
+        if (phi > c0t) { S = c0t; V = (phi - S) / eta; }
+        else { V = 0.; S = phi; }
+      }
+    } else {  // friction::solve_fs with frictionless (mu = 0) or stz calc_mu
+      if (snc < 0.) {
+        double mu = 0.;
+        if (F.law == 3) {
+          double pr[8] = {phi, eta, snc, F.state[p], 0., 0., 0., 0.};
+          for (int k = 0; k < F.nperts; k++) {
+            pr[4] += pert_val(F.pert_xy, F.pert_par, k, 0, n, p, t);
+            pr[5] += pert_val(F.pert_xy, F.pert_par, k, 1, n, p, t);
+            pr[6] += pert_val(F.pert_xy, F.pert_par, k, 2, n, p, t);
+            pr[7] += pert_val(F.pert_xy, F.pert_par, k, 3, n, p, t);
+          }
+          if (F.param_file) { pr[4] += F.pf[p]; pr[5] += F.pf[n + p]; pr[6] += F.pf[2 * n + p]; pr[7] += F.pf[3 * n + p]; }
+          if (!(pr[3] > 0.)) atomicOr(err, 8);
+          if (pr[7] >= -phi / snc) {
+            mu = -phi / snc;
+          } else {
+            int e = 0, it = 0;
+            mu = stz_newton(pr[7], -phi / snc, pr, &e, &it);
+            if (e) atomicOr(err, e);
+            if (newton_iters) atomicAdd(newton_iters, it);
+          }
+        }
+        if (mu * fabs(snc) > phi) { V = 0.; S = phi; }
+        else { S = mu * fabs(snc); V = (phi - S) / eta; }
+      } else {
+        S = 0.;
+        V = phi / eta;
+      }
+      if (F.law == 3) {  // stz::calc_dstatedt / chihat
+        double c0t = 0., Rt = 0., betat = 0., chiwt = 0., v1t = 0.;
+        for (int k = 0; k < F.nperts; k++) {
+          c0t += pert_val(F.pert_xy, F.pert_par, k, 4, n, p, t);
+          Rt += pert_val(F.pert_xy, F.pert_par, k, 5, n, p, t);
+          betat += pert_val(F.pert_xy, F.pert_par, k, 6, n, p, t);
+          chiwt += pert_val(F.pert_xy, F.pert_par, k, 7, n, p, t);
+          v1t += pert_val(F.pert_xy, F.pert_par, k, 8, n, p, t);
+        }
+        if (F.param_file) {
+          c0t += F.pf[4 * n + p]; Rt += F.pf[5 * n + p]; betat += F.pf[6 * n + p];
+          chiwt += F.pf[7 * n + p]; v1t += F.pf[8 * n + p];
+        }
+        if (V > v1t) atomicOr(err, 16);
+        const double chihat = (V == 0.) ? chiwt : chiwt / log(v1t / V);
+        dstatedt = V * S / c0t * (1. - F.state[p] / chihat) - Rt * exp(-betat / F.state[p]);
+      }
+    }
+    double vv2, vv3;
+    if (V == 0.) { vv2 = 0.; vv3 = 0.; }
+    else { vv2 = V * phi2 / (eta * V + S); vv3 = V * phi3 / (eta * V + S); }
+    hs12 = phi2 - eta * vv2;
+    hs22 = hs12;
+    hs13 = phi3 - eta * vv3;
+    hs23 = hs13;
+    hv12 = (hs12 - s12) / z1 + v12;
+    hv22 = (-hs22 + s22) / z2 + v22;
+    hv13 = (hs13 - s13) / z1 + v13;
+    hv23 = (-hs23 + s23) / z2 + v23;
+    F.v[p] = V;
+    F.s[p] = S;
+    F.sn[p] = snc;
+    if constexpr (M == 2) {
+      F.vx[p] = vv2; F.vx[n + p] = vv3;
+      F.sx[p] = hs12; F.sx[n + p] = hs13;
+    } else if constexpr (M == 0) {
+      F.vx[p] = vv2; F.sx[p] = hs12;
+    } else {
+      F.vx[p] = vv3; F.sx[p] = hs13;
+    }
+    if (F.load_file) {
+      hs12 -= F.lf[n + p]; hs22 -= F.lf[n + p];
+      hs13 -= F.lf[2 * n + p]; hs23 -= F.lf[2 * n + p];
+    }
+    for (int k = 0; k < F.nloads; k++) {
+      const double l2 = pert_val(F.load_xy, F.load_par, k, 1, n, p, t);
+      const double l3 = pert_val(F.load_xy, F.load_par, k, 2, n, p, t);
+      hs12 -= l2; hs22 -= l2; hs13 -= l3; hs23 -= l3;
+    }
+    if (!fs.no_sat) {  // friction::scale_df + calc_df + update for this point
+      F.du[p] = F.du[p] * fs.A;
+      F.du[p] += fs.dt * V;
+      F.u[p] += fs.B * F.du[p];
+      const int nc = (M == 2) ? 2 : 1;
+      for (int q = 0; q < nc; q++) {
+        const long long o = (long long)q * n + p;
+        F.dux[o] = F.dux[o] * fs.A;
+        F.dux[o] += fs.dt * F.vx[o];
+        F.ux[o] += fs.B * F.dux[o];
+      }
+      if (F.law == 3) {
+        F.dstate[p] = F.dstate[p] * fs.A;
+        F.dstate[p] += fs.dt * dstatedt;
+        F.state[p] += fs.B * F.dstate[p];
+      }
+    }
+    if (F.front && fs.front_t >= 0. && F.front_field == 0 && F.front[p] < 0. && V >= F.front_value)
+      F.front[p] = fs.front_t;
+  }
+  if (fs.no_sat) return;
+  // SAT terms for both sides (interface.cpp:802-962)
+  const int NF = g.nf;
+  const long long n = F.n;
+  double* sn1 = F.sat + (0 * 2 + 0) * NF * n;
+  double* sv1 = F.sat + (0 * 2 + 1) * NF * n;
+  double* sn2 = F.sat + (1 * 2 + 0) * NF * n;
+  double* sv2 = F.sat + (1 * 2 + 1) * NF * n;
+  r1.v1 -= hv11; r1.v2 = 0.; r1.v3 = 0.;
+  r1.s22 = gamma1 * (r1.s11 - hs11);
+  r1.s33 = gamma1 * (r1.s11 - hs11);
+  r1.s11 -= hs11; r1.s12 = 0.; r1.s13 = 0.; r1.s23 = 0.;
+  r2.v1 -= hv21; r2.v2 = 0.; r2.v3 = 0.;
+  r2.s22 = gamma2 * (r2.s11 - hs21);
+  r2.s33 = gamma2 * (r2.s11 - hs21);
+  r2.s11 -= hs21; r2.s12 = 0.; r2.s13 = 0.; r2.s23 = 0.;
+  BFd o1 = rot_to_xy(r1, nn, t1, t2), o2 = rot_to_xy(r2, nn, t1, t2);
+  if constexpr (M == 2) {
+    const double* a1 = &o1.v1;
+    const double* a2 = &o2.v1;
+    for (int q = 0; q < 9; q++) { sn1[q * n + p] = cp1 * h1 * a1[q]; sn2[q * n + p] = cp2 * h2 * a2[q]; }
+  } else if constexpr (M == 0) {
+    sn1[p] = cp1 * h1 * o1.v1; sn1[n + p] = cp1 * h1 * o1.v2; sn1[2 * n + p] = cp1 * h1 * o1.s11;
+    sn1[3 * n + p] = cp1 * h1 * o1.s12; sn1[4 * n + p] = cp1 * h1 * o1.s22;
+    sn2[p] = cp2 * h2 * o2.v1; sn2[n + p] = cp2 * h2 * o2.v2; sn2[2 * n + p] = cp2 * h2 * o2.s11;
+    sn2[3 * n + p] = cp2 * h2 * o2.s12; sn2[4 * n + p] = cp2 * h2 * o2.s22;
+    if (NF > 5) { sn1[5 * n + p] = cp1 * h1 * o1.s33; sn2[5 * n + p] = cp2 * h2 * o2.s33; }
+  } else {
+    for (int q = 0; q < 3; q++) { sn1[q * n + p] = 0.; sn2[q * n + p] = 0.; }
+  }
+  rs1.v1 = 0.; rs1.v2 -= hv12; rs1.v3 -= hv13;
+  rs1.s11 = 0.; rs1.s12 -= hs12; rs1.s13 -= hs13; rs1.s22 = 0.; rs1.s23 = 0.; rs1.s33 = 0.;
+  rs2.v1 = 0.; rs2.v2 -= hv22; rs2.v3 -= hv23;
+  rs2.s11 = 0.; rs2.s12 -= hs22; rs2.s13 -= hs23; rs2.s22 = 0.; rs2.s23 = 0.; rs2.s33 = 0.;
+  o1 = rot_to_xy(rs1, nn, t1, t2);
+  o2 = rot_to_xy(rs2, nn, t1, t2);
+  if constexpr (M == 2) {
+    const double* a1 = &o1.v1;
+    const double* a2 = &o2.v1;
+    for (int q = 0; q < 9; q++) { sv1[q * n + p] = cs1 * h1 * a1[q]; sv2[q * n + p] = cs2 * h2 * a2[q]; }
+  } else if constexpr (M == 0) {
+    sv1[p] = cs1 * h1 * o1.v1; sv1[n + p] = cs1 * h1 * o1.v2; sv1[2 * n + p] = cs1 * h1 * o1.s11;
+    sv1[3 * n + p] = cs1 * h1 * o1.s12; sv1[4 * n + p] = cs1 * h1 * o1.s22;
+    sv2[p] = cs2 * h2 * o2.v1; sv2[n + p] = cs2 * h2 * o2.v2; sv2[2 * n + p] = cs2 * h2 * o2.s11;
+    sv2[3 * n + p] = cs2 * h2 * o2.s12; sv2[4 * n + p] = cs2 * h2 * o2.s22;
+    if (NF > 5) { sv1[5 * n + p] = 0.; sv2[5 * n + p] = 0.; }
+  } else {
+    sv1[p] = cs1 * h1 * o1.v3; sv1[n + p] = cs1 * h1 * o1.s13; sv1[2 * n + p] = cs1 * h1 * o1.s23;
+    sv2[p] = cs2 * h2 * o2.v3; sv2[n + p] = cs2 * h2 * o2.s13; sv2[2 * n + p] = cs2 * h2 * o2.s23;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// plasticity: fields::set_stress, block::calc_plastic (block.cpp:442-697),
+// fields::remove_stress, then the output round trip of problem.cpp:160-164.
+template <int M, typename R>
+__global__ void __launch_bounds__(256) plastic_kernel(BlockDev bd, GridDev g, double dt, R* __restrict__ f,
+                                                      R* __restrict__ fp, const R* __restrict__ mat,
+                                                      const double* __restrict__ hs) {
+  const long long npts = (long long)bd.n[0] * bd.n[1] * bd.n[2];
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= npts) return;
+  const int lk = (int)(t % bd.n[2]);
+  const long long r = t / bd.n[2];
+  const int lj = (int)(r % bd.n[1]);
+  const int li = (int)(r / bd.n[1]);
+  const long long N = g.N;
+  const long long id = ((long long)(bd.lo[0] + li) * g.Ny + (bd.lo[1] + lj)) * g.Nz + (bd.lo[2] + lk);
+  double sv[6];
+  for (int i = 0; i < g.ns; i++) {  // set_stress
+    double v = (double)f[(g.nv + i) * N + id];
+    v += g.s0[g.sidx[i]];
+    if (g.hetstress) v += hs[i * N + id];
+    sv[i] = v;
+  }
+  double sxx, sxy, sxz, syy, syz, szz;
+  if constexpr (M == 2) { sxx = sv[0]; sxy = sv[1]; sxz = sv[2]; syy = sv[3]; syz = sv[4]; szz = sv[5]; }
+  else if constexpr (M == 0) { sxx = sv[0]; sxy = sv[1]; sxz = 0.; syy = sv[2]; syz = 0.; szz = sv[3]; }
+  else { sxx = g.s0[0]; sxy = 0.; sxz = sv[0]; syy = g.s0[3]; syz = sv[1]; szz = g.s0[5]; }
+  double lambda = (double)fp[id], gammap = (double)fp[N + id];
+  double ep[6] = {0., 0., 0., 0., 0., 0.};
+  if (g.ptensor)
+    for (int q = 0; q < 6; q++) ep[q] = (double)fp[(2 + q) * N + id];
+  double k, G;
+  if (g.hetmat) {
+    k = (double)mat[N + id] + 2. / 3. * (double)mat[2 * N + id];
+    G = (double)mat[2 * N + id];
+  } else {
+    k = bd.lam + 2. / 3. * bd.G;
+    G = bd.G;
+  }
+  const double mu = bd.pmu, coh = bd.pc, beta = bd.pbeta, eta = bd.peta;
+  double sigma = (sxx + syy + szz) / 3.;
+  double tau = sqrt(((sxx - syy) * (sxx - syy) + (syy - szz) * (syy - szz) + (szz - sxx) * (szz - sxx)) / 6. + sxy * sxy +
+                    sxz * sxz + syz * syz);
+  double yf = coh - mu * sigma;
+  if (yf < 0.) yf = 0.;
+  if (tau - yf <= 0.) {
+    lambda = 0.;
+  } else {
+    if (tau * mu * beta * k > (mu * sigma - coh) * G) lambda = (tau + mu * sigma - coh) / (eta + dt * (G + mu * beta * k));
+    else lambda = tau / (eta + dt * G);
+    gammap = gammap + dt * lambda;
+    double sd[6] = {sxx - sigma, sxy, sxz, syy - sigma, syz, szz - sigma};
+    tau -= dt * lambda * G;
+    sigma -= dt * lambda * beta * k;
+    for (int i = 0; i < 6; i++) sd[i] *= tau / (tau + dt * lambda * G);
+    sxx = sd[0] + sigma; sxy = sd[1]; sxz = sd[2]; syy = sd[3] + sigma; syz = sd[4]; szz = sd[5] + sigma;
+    ep[0] = ep[0] + dt * lambda * (sd[0] / (2. * tau) + (beta / 3.));
+    ep[1] = ep[1] + dt * lambda * (sd[1] / (2. * tau));
+    ep[2] = ep[2] + dt * lambda * (sd[2] / (2. * tau));
+    ep[3] = ep[3] + dt * lambda * (sd[3] / (2. * tau) + (beta / 3.));
+    ep[4] = ep[4] + dt * lambda * (sd[4] / (2. * tau));
+    ep[5] = ep[5] + dt * lambda * (sd[5] / (2. * tau) + (beta / 3.));
+  }
+  if constexpr (M == 2) { sv[0] = sxx; sv[1] = sxy; sv[2] = sxz; sv[3] = syy; sv[4] = syz; sv[5] = szz; }
+  else if constexpr (M == 0) { sv[0] = sxx; sv[1] = sxy; sv[2] = syy; sv[3] = szz; }
+  else { sv[0] = sxz; sv[1] = syz; }
+  for (int i = 0; i < g.ns; i++) {
+    double v = sv[i];
+    const double s0 = g.s0[g.sidx[i]];
+    v -= s0;  // remove_stress
+    if (g.hetstress) v -= hs[i * N + id];
+    v += s0;  // set_stress / remove_stress around the output write
+    if (g.hetstress) v += hs[i * N + id];
+    v -= s0;
+    if (g.hetstress) v -= hs[i * N + id];
+    f[(g.nv + i) * N + id] = (R)v;
+  }
+  fp[id] = (R)lambda;
+  fp[N + id] = (R)gammap;
+  if (g.ptensor)
+    for (int q = 0; q < 6; q++) fp[(2 + q) * N + id] = (R)ep[q];
+}
+
+// ---------------------------------------------------------------------------
+// gathers for read-back and output units
+
+// box of one field (optionally absolute stress: (f + s0) + s) into dst (FP64)
+template <typename R>
+__global__ void gather_box(const R* __restrict__ src, const double* __restrict__ hs, double s0add, int hs_field,
+                           long long N, int Ny, int Nz, int lo0, int lo1, int lo2, int st0, int st1, int st2, int n0,
+                           int n1, int n2, double* __restrict__ dst) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long tot = (long long)n0 * n1 * n2;
+  if (t >= tot) return;
+  const int c = (int)(t % n2);
+  const long long r = t / n2;
+  const int b = (int)(r % n1);
+  const int a = (int)(r / n1);
+  const long long id = ((long long)(lo0 + a * st0) * Ny + (lo1 + b * st1)) * Nz + (lo2 + c * st2);
+  double v = (double)src[id];
+  if (s0add != 0. || hs_field >= 0) {
+    v += s0add;
+    if (hs_field >= 0) v += hs[hs_field * N + id];
+  }
+  dst[t] = v;
+}
+
+template <typename R>
+__global__ void convert_to_double(const R* __restrict__ src, double* __restrict__ dst, long long n) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) dst[t] = (double)src[t];
+}
+
+template <typename R>
+__global__ void convert_from_double(const double* __restrict__ src, R* __restrict__ dst, long long n) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) dst[t] = (R)src[t];
+}
+
+}  // namespace fdf

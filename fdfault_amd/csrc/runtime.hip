@@ -1,0 +1,921 @@
+// Device runtime and C ABI of the MI355X time-stepping core (see
+// include/fdfault_hip.h).  One HIP stream per handle; f is double-buffered on
+// the device so each stage reads f and writes the other buffer.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <ctime>
+#include <memory>
+#include <type_traits>
+#include <string>
+#include <vector>
+
+#include "../../include/fdfault_hip.h"
+#include "kernels.hpp"
+#include "setup.hpp"
+
+namespace fdf {
+
+static thread_local std::string g_last_error;
+
+#define HIPCHK(x)                                                                              \
+  do {                                                                                         \
+    hipError_t e_ = (x);                                                                       \
+    if (e_ != hipSuccess) throw Error{std::string("HIP error: ") + hipGetErrorString(e_) + " at " #x}; \
+  } while (0)
+
+template <typename T>
+struct DBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  void alloc(size_t count) {
+    free();
+    n = count;
+    if (count) HIPCHK(hipMalloc(&p, count * sizeof(T)));
+  }
+  void upload(const T* h, size_t count) {
+    alloc(count);
+    if (count) HIPCHK(hipMemcpy(p, h, count * sizeof(T), hipMemcpyHostToDevice));
+  }
+  void upload(const std::vector<T>& v) { upload(v.data(), v.size()); }
+  void zero(hipStream_t s) {
+    if (n) HIPCHK(hipMemsetAsync(p, 0, n * sizeof(T), s));
+  }
+  void free() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  ~DBuf() { free(); }
+};
+
+struct OutFile {
+  FILE* data = nullptr;
+  FILE* time = nullptr;
+};
+
+class SolverBase {
+ public:
+  virtual ~SolverBase() {}
+  virtual void step(int n) = 0;
+  virtual void stage(int istep, int st) = 0;
+  virtual void read_array(int which, double* h, long n) = 0;
+  virtual void write_array(int which, const double* h, long n) = 0;
+  virtual void read_field(int field, const int lo[3], const int hi[3], const int st[3], double* h) = 0;
+  virtual long read_iface(int iface, int which, double* h, long n) = 0;
+  virtual long read_front(int iface, double* h, long n) = 0;
+  virtual void finish() = 0;
+  virtual double bulk_ms(int* n) = 0;
+  SetupH S;
+  fdf_opts opts{};
+  int cur_step = 0;
+};
+
+template <typename R>
+class Solver : public SolverBase {
+ public:
+  explicit Solver(const std::string& path, const fdf_opts& o);
+  ~Solver() override;
+  void step(int n) override;
+  void stage(int istep, int st) override { do_stage(istep, st, false); sync_check(); }
+  void read_array(int which, double* h, long n) override;
+  void write_array(int which, const double* h, long n) override;
+  void read_field(int field, const int lo[3], const int hi[3], const int st[3], double* h) override;
+  long read_iface(int iface, int which, double* h, long n) override;
+  long read_front(int iface, double* h, long n) override;
+  void finish() override;
+  double bulk_ms(int* n) override;
+
+ private:
+  int mode_id() const { return S.ndim == 3 ? 2 : (S.mode == 2 ? 0 : 1); }
+  void do_stage(int istep, int st, bool full_step);
+  void launch_fault(int i, const FaultStageDev& fs);
+  void launch_bulk(int b, const StageDev& sd);
+  void launch_plastic(int b);
+  void write_outputs(int tstep);
+  void sync_check();
+
+  hipStream_t stream = nullptr;
+  static constexpr int NEV = 128;
+  hipEvent_t evs[2 * NEV] = {};
+  int nev = 0;  // recorded pairs not yet summed
+  double bulk_time_ms = 0.;
+  int bulk_count = 0;
+  void flush_events();
+  GridDev G{};
+  DBuf<R> f[2], df, fp, metric, jac, mat;
+  DBuf<double> hs, x;
+  int cur = 0;
+  std::vector<BlockDev> bdev;
+  struct FaceBufs {
+    DBuf<double> nrm, dl;
+  };
+  std::vector<FaceBufs> fbufs;  // per block*6
+  struct IfBufs {
+    DBuf<double> nrm, dl1, dl2, load_xy, load_par, lf, pert_xy, pert_par, pf;
+    DBuf<double> u, du, v, s, sn, ux, dux, vx, sx, state, dstate, sat, front;
+  };
+  std::vector<IfBufs> ibufs;
+  std::vector<FaultDev> fdev;
+  DBuf<int> err;
+  DBuf<double> stage_buf;
+  std::vector<OutFile> outs;
+};
+
+template <typename R>
+Solver<R>::Solver(const std::string& path, const fdf_opts& o) {
+  opts = o;
+  compile_problem(path, S, o.write_outputs != 0, o.quiet != 0);
+  HIPCHK(hipSetDevice(o.device));
+  HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  for (int e = 0; e < 2 * NEV; e++) HIPCHK(hipEventCreate(&evs[e]));
+  const size_t N = S.N;
+  G.ndim = S.ndim;
+  G.mode = S.mode;
+  G.nf = S.nf;
+  G.s = S.sbporder;
+  G.Ny = S.nx[1];
+  G.Nz = S.nx[2];
+  G.N = (long long)N;
+  G.plastic = S.plastic;
+  G.hetmat = S.hetmat;
+  G.hetstress = S.hetstress;
+  G.cdiss = S.cdiss;
+  for (int i = 0; i < 6; i++) G.s0[i] = S.s0[i];
+  G.ns = S.ns;
+  G.nv = S.nv;
+  for (int i = 0; i < 6; i++) G.sidx[i] = S.sidx[i];
+  G.nfp = S.nfp;
+  G.ptensor = S.plastic_tensor;
+  // SBP tables
+  HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(c_fd), S.fd.c, sizeof(S.fd.c)));
+  HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(c_diss), S.fd.d, sizeof(S.fd.d)));
+  // grid arrays in the bulk precision
+  auto cast = [](const std::vector<double>& v) {
+    std::vector<R> out(v.size());
+    for (size_t i = 0; i < v.size(); i++) out[i] = (R)v[i];
+    return out;
+  };
+  metric.upload(cast(S.metric));
+  jac.upload(cast(S.jac));
+  x.upload(S.x);
+  if (S.hetmat) mat.upload(cast(S.mat));
+  if (S.hetstress) hs.upload(S.s);
+  for (int b = 0; b < 2; b++) {
+    f[b].alloc((size_t)S.nf * N);
+    f[b].zero(stream);
+  }
+  df.alloc((size_t)S.nf * N);
+  df.zero(stream);
+  if (S.nfp) {
+    fp.alloc((size_t)S.nfp * N);
+    fp.zero(stream);
+  }
+  err.alloc(1);
+  err.zero(stream);
+  // blocks
+  const int nd = S.ndim;
+  fbufs.resize(S.blocks.size() * 6);
+  for (size_t bi = 0; bi < S.blocks.size(); bi++) {
+    const BlockH& B = S.blocks[bi];
+    BlockDev D{};
+    for (int d = 0; d < 3; d++) {
+      D.lo[d] = B.mlb[d];
+      D.n[d] = B.nx[d];
+      D.hi[d] = B.prb[d];
+      D.mc[d] = B.mc[d];
+      D.mrb[d] = B.mrb[d];
+      D.dx[d] = B.dx[d];
+      for (int q = 0; q < 4; q++) D.K[d][q] = B.K[d][q];
+    }
+    D.nu = 0.5 * B.lam / (B.lam + B.G);
+    D.lam = B.lam;
+    D.G = B.G;
+    D.pmu = B.pmu;
+    D.pc = B.pc;
+    D.pbeta = B.pbeta;
+    D.peta = B.peta;
+    for (int fc = 0; fc < 2 * nd; fc++) {
+      const FaceBC& F = B.face[fc];
+      FaceDev& FD = D.face[fc];
+      FD.active = F.active;
+      if (!F.active) continue;
+      FD.n0 = F.n0;
+      FD.n1 = F.n1;
+      FD.r = F.r;
+      FD.alpha = F.alpha;
+      FD.theta = F.theta;
+      FD.cp = F.cp;
+      FD.cs = F.cs;
+      FD.zp = F.zp;
+      FD.zs = F.zs;
+      FD.gamma = F.gamma;
+      FaceBufs& fb = fbufs[bi * 6 + fc];
+      fb.nrm.upload(F.nrm);
+      fb.dl.upload(F.dl);
+      FD.nrm = fb.nrm.p;
+      FD.dl = fb.dl.p;
+    }
+    bdev.push_back(D);
+  }
+  // interfaces
+  ibufs.resize(S.ifaces.size());
+  for (size_t ii = 0; ii < S.ifaces.size(); ii++) {
+    const IfaceH& I = S.ifaces[ii];
+    IfBufs& ib = ibufs[ii];
+    FaultDev F{};
+    F.law = I.law;
+    F.dir = I.dir;
+    F.a = (I.dir == 0) ? 1 : 0;
+    F.b = (I.dir == 2) ? 1 : 2;
+    F.n0 = I.n0;
+    F.n1 = I.n1;
+    F.n = I.n0 * I.n1;
+    for (int d = 0; d < 3; d++) F.org[d] = I.org[d];
+    F.cp1 = I.cp1; F.cs1 = I.cs1; F.zp1 = I.zp1; F.zs1 = I.zs1;
+    F.cp2 = I.cp2; F.cs2 = I.cs2; F.zp2 = I.zp2; F.zs2 = I.zs2;
+    F.gamma1 = I.gamma1;
+    F.gamma2 = I.gamma2;
+    ib.nrm.upload(I.nrm);
+    ib.dl1.upload(I.dl1);
+    ib.dl2.upload(I.dl2);
+    F.nrm = ib.nrm.p;
+    F.dl1 = ib.dl1.p;
+    F.dl2 = ib.dl2.p;
+    const size_t n = (size_t)F.n;
+    auto pack = [n](const std::vector<PertH>& P, DBuf<double>& xy, DBuf<double>& par) {
+      std::vector<double> hx, hp;
+      for (const PertH& p : P) {
+        hx.insert(hx.end(), p.xy.begin(), p.xy.end());
+        hp.push_back(p.t0);
+        for (int q = 0; q < 9; q++) hp.push_back(p.amp[q]);
+      }
+      xy.upload(hx);
+      par.upload(hp);
+    };
+    F.nloads = (int)I.loads.size();
+    pack(I.loads, ib.load_xy, ib.load_par);
+    F.load_xy = ib.load_xy.p;
+    F.load_par = ib.load_par.p;
+    F.load_file = I.load_file;
+    if (I.load_file) {
+      std::vector<double> all;
+      for (int q = 0; q < 3; q++) all.insert(all.end(), I.lf[q].begin(), I.lf[q].end());
+      ib.lf.upload(all);
+      F.lf = ib.lf.p;
+    }
+    F.nperts = (int)I.perts.size();
+    pack(I.perts, ib.pert_xy, ib.pert_par);
+    F.pert_xy = ib.pert_xy.p;
+    F.pert_par = ib.pert_par.p;
+    F.param_file = I.param_file;
+    if (I.param_file) {
+      std::vector<double> all;
+      for (int q = 0; q < 9; q++) all.insert(all.end(), I.pf[q].begin(), I.pf[q].end());
+      ib.pf.upload(all);
+      F.pf = ib.pf.p;
+    }
+    const size_t nc = (size_t)(S.ndim - 1) * n;
+    for (DBuf<double>* b : {&ib.u, &ib.du, &ib.v, &ib.s, &ib.sn}) { b->alloc(n); b->zero(stream); }
+    for (DBuf<double>* b : {&ib.ux, &ib.dux, &ib.vx, &ib.sx}) { b->alloc(nc); b->zero(stream); }
+    ib.state.alloc(n);
+    ib.state.zero(stream);
+    ib.dstate.alloc(n);
+    ib.dstate.zero(stream);
+    if (I.law == LAW_STZ) HIPCHK(hipMemcpy(ib.state.p, I.state0.data(), n * sizeof(double), hipMemcpyHostToDevice));
+    ib.sat.alloc((size_t)4 * S.nf * n);
+    ib.sat.zero(stream);
+    F.u = ib.u.p; F.du = ib.du.p; F.v = ib.v.p; F.s = ib.s.p; F.sn = ib.sn.p;
+    F.ux = ib.ux.p; F.dux = ib.dux.p; F.vx = ib.vx.p; F.sx = ib.sx.p;
+    F.state = ib.state.p; F.dstate = ib.dstate.p;
+    F.sat = ib.sat.p;
+    if (S.has_front && I.law != LAW_LOCKED) {
+      std::vector<double> m1(n, -1.);
+      ib.front.upload(m1);
+      F.front = ib.front.p;
+      F.front_value = S.front_value;
+      F.front_field = S.front_field;
+    }
+    fdev.push_back(F);
+    // register this interface on the two blocks' face lists (interface order)
+    for (int side = 0; side < 2; side++) {
+      BlockDev& D = bdev[side == 0 ? I.b1 : I.b2];
+      if (D.nif >= 6) throw Error{"too many interfaces on one block"};
+      IfFaceDev& X = D.ifc[D.nif++];
+      X.side = side;
+      X.dir = I.dir;
+      X.fixed = I.org[I.dir] + side;
+      X.a = F.a;
+      X.b = F.b;
+      X.org_a = I.org[F.a];
+      X.org_b = I.org[F.b];
+      X.n = F.n;
+      X.n1 = F.n1;
+      X.sat = ib.sat.p;
+    }
+  }
+  HIPCHK(hipStreamSynchronize(stream));
+  // t = 0: interface solve without SAT (domain.cpp:141-143) on f = 0
+  {
+    FaultStageDev fs{0., 0., 0., 0., 1, -1.};
+    for (size_t i = 0; i < S.ifaces.size(); i++)
+      if (S.ifaces[i].law != LAW_LOCKED) launch_fault((int)i, fs);
+  }
+  // set_stress / remove_stress around the t = 0 output (problem.cpp:74-76):
+  // f = ((0 + s0) + s - s0) - s, non-zero only with heterogeneous stress
+  if (S.hetstress) {
+    std::vector<double> h((size_t)S.nf * N, 0.);
+    for (int i = 0; i < S.ns; i++) {
+      const double s0 = S.s0[S.sidx[i]];
+      double* fq = h.data() + (size_t)(S.nv + i) * N;
+      const double* sq = S.s.data() + (size_t)i * N;
+      for (size_t j = 0; j < N; j++) {
+        double v = fq[j];
+        v += s0;
+        v += sq[j];
+        v -= s0;
+        v -= sq[j];
+        fq[j] = v;
+      }
+    }
+    HIPCHK(hipStreamSynchronize(stream));
+    write_array(0, h.data(), (long)h.size());
+  }
+  // outputs and front at t = 0 (problem.cpp:66-82)
+  if (opts.write_outputs) {
+    const char endian = [] {
+      const uint32_t v = 0x01020304;
+      unsigned char c[4];
+      memcpy(c, &v, 4);
+      return c[0] == 4 ? '<' : (c[0] == 1 ? '>' : '=');
+    }();
+    for (const OutputH& U : S.outputs) {
+      OutFile of;
+      const std::string base = S.datadir + S.name + "_" + U.name;
+      of.data = fopen((base + "_" + U.field + ".dat").c_str(), "wb");
+      if (!of.data) throw Error{"Error opening file in outputunit.cpp"};
+      static const char* xyz[3] = {"x", "y", "z"};
+      for (int c = 0; c < S.ndim; c++) {
+        FILE* xf = fopen((base + "_" + xyz[c] + ".dat").c_str(), "wb");
+        if (!xf) throw Error{"Error opening file in outputunit.cpp"};
+        std::vector<double> buf;
+        for (int a = U.xm[0]; a <= U.xp[0]; a += U.xs[0])
+          for (int b = U.xm[1]; b <= U.xp[1]; b += U.xs[1])
+            for (int cc = U.xm[2]; cc <= U.xp[2]; cc += U.xs[2]) buf.push_back(S.x[(size_t)c * N + S.idx(a, b, cc)]);
+        fwrite(buf.data(), sizeof(double), buf.size(), xf);
+        fclose(xf);
+      }
+      if (U.tp > U.tm) {
+        of.time = fopen((base + "_t.dat").c_str(), "wb");
+        if (!of.time) throw Error{"Error opening file in outputunit.cpp"};
+      }
+      const int ntout = (U.tp - U.tm) / U.ts + 1;
+      FILE* m = fopen((base + ".m").c_str(), "w");
+      if (!m) throw Error{"Error writing parameters to matlab file"};
+      fprintf(m, "endian = '%c';\nfield = '%s';\nnt = %d;\nnx = %d;\nny = %d;\nnz = %d;\n",
+              endian == '<' ? 'l' : (endian == '>' ? 'b' : 'n'), U.field.c_str(), ntout, U.nx[0], U.nx[1], U.nx[2]);
+      fclose(m);
+      FILE* py = fopen((base + ".py").c_str(), "w");
+      if (!py) throw Error{"Error writing parameters to python file"};
+      fprintf(py, "endian = '%c'\nfield = '%s';\nnt = %d\nnx = %d\nny = %d\nnz = %d\n", endian, U.field.c_str(), ntout,
+              U.nx[0], U.nx[1], U.nx[2]);
+      fclose(py);
+      outs.push_back(of);
+    }
+    write_outputs(0);
+  }
+  // front at t = 0 (front.cpp:103-125) from V of the initial interface solve
+  if (S.has_front && S.front_field == 0) {
+    for (size_t i = 0; i < S.ifaces.size(); i++) {
+      if (S.ifaces[i].law == LAW_LOCKED) continue;
+      IfBufs& ib = ibufs[i];
+      std::vector<double> v(ib.v.n), tv(ib.v.n);
+      HIPCHK(hipStreamSynchronize(stream));
+      HIPCHK(hipMemcpy(v.data(), ib.v.p, v.size() * sizeof(double), hipMemcpyDeviceToHost));
+      for (size_t q = 0; q < v.size(); q++) tv[q] = (v[q] >= S.front_value) ? 0. : -1.;
+      HIPCHK(hipMemcpy(ib.front.p, tv.data(), tv.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+  }
+  sync_check();
+}
+
+template <typename R>
+Solver<R>::~Solver() {
+  for (OutFile& o : outs) {
+    if (o.data) fclose(o.data);
+    if (o.time) fclose(o.time);
+  }
+  if (stream) (void)hipStreamSynchronize(stream);
+  for (int e = 0; e < 2 * NEV; e++)
+    if (evs[e]) (void)hipEventDestroy(evs[e]);
+  if (stream) (void)hipStreamDestroy(stream);
+}
+
+template <typename R>
+void Solver<R>::sync_check() {
+  HIPCHK(hipStreamSynchronize(stream));
+  int e = 0;
+  HIPCHK(hipMemcpy(&e, err.p, sizeof(int), hipMemcpyDeviceToHost));
+  if (e) {
+    std::string m;
+    if (e & 16) m = "Slip velocity exceeds melting value in stz.cpp";
+    else if (e & 1) m = "zero derivative in Newton's method in utilities.cpp";
+    else if (e & 2) m = "Newton's method failed to converge in utilities.cpp";
+    else if (e & 8) m = "STZ effective temperature must be positive";
+    else m = "bracket repair in Newton's method did not terminate";
+    int z = 0;
+    HIPCHK(hipMemcpy(err.p, &z, sizeof(int), hipMemcpyHostToDevice));
+    throw Error{m};
+  }
+}
+
+template <typename R>
+void Solver<R>::launch_fault(int i, const FaultStageDev& fs) {
+  const FaultDev& F = fdev[i];
+  const int nb = (F.n + 255) / 256;
+  const R* fc = f[cur].p;
+  switch (mode_id()) {
+    case 0: hipLaunchKernelGGL((fault_kernel<0, R>), dim3(nb), dim3(256), 0, stream, F, G, fs, fc, mat.p, hs.p, err.p, nullptr); break;
+    case 1: hipLaunchKernelGGL((fault_kernel<1, R>), dim3(nb), dim3(256), 0, stream, F, G, fs, fc, mat.p, hs.p, err.p, nullptr); break;
+    default: hipLaunchKernelGGL((fault_kernel<2, R>), dim3(nb), dim3(256), 0, stream, F, G, fs, fc, mat.p, hs.p, err.p, nullptr); break;
+  }
+  HIPCHK(hipGetLastError());
+}
+
+template <int M, typename R>
+static void bulk_dispatch(bool diss, bool het, const BlockDev& D, const GridDev& G, const StageDev& sd, const R* f,
+                          R* fn, R* df, const R* mt, const R* jac, const R* mat, const double* hs, hipStream_t s) {
+  const long long np = (long long)D.n[0] * D.n[1] * D.n[2];
+  const dim3 grid((unsigned)((np + 255) / 256)), blk(256);
+  if (diss) {
+    if (het) hipLaunchKernelGGL((bulk_kernel<M, R, R, true, true>), grid, blk, 0, s, D, G, sd, f, fn, df, mt, jac, mat, hs);
+    else hipLaunchKernelGGL((bulk_kernel<M, R, R, true, false>), grid, blk, 0, s, D, G, sd, f, fn, df, mt, jac, mat, hs);
+  } else {
+    if (het) hipLaunchKernelGGL((bulk_kernel<M, R, R, false, true>), grid, blk, 0, s, D, G, sd, f, fn, df, mt, jac, mat, hs);
+    else hipLaunchKernelGGL((bulk_kernel<M, R, R, false, false>), grid, blk, 0, s, D, G, sd, f, fn, df, mt, jac, mat, hs);
+  }
+}
+
+template <typename R>
+void Solver<R>::launch_bulk(int b, const StageDev& sd) {
+  const BlockDev& D = bdev[b];
+  const bool diss = S.cdiss > 0., het = S.hetmat;
+  switch (mode_id()) {
+    case 0: bulk_dispatch<0, R>(diss, het, D, G, sd, f[cur].p, f[cur ^ 1].p, df.p, metric.p, jac.p, mat.p, hs.p, stream); break;
+    case 1: bulk_dispatch<1, R>(diss, het, D, G, sd, f[cur].p, f[cur ^ 1].p, df.p, metric.p, jac.p, mat.p, hs.p, stream); break;
+    default: bulk_dispatch<2, R>(diss, het, D, G, sd, f[cur].p, f[cur ^ 1].p, df.p, metric.p, jac.p, mat.p, hs.p, stream); break;
+  }
+  HIPCHK(hipGetLastError());
+}
+
+template <typename R>
+void Solver<R>::launch_plastic(int b) {
+  const BlockDev& D = bdev[b];
+  const long long np = (long long)D.n[0] * D.n[1] * D.n[2];
+  const dim3 grid((unsigned)((np + 255) / 256)), blk(256);
+  switch (mode_id()) {
+    case 0: hipLaunchKernelGGL((plastic_kernel<0, R>), grid, blk, 0, stream, D, G, S.dt, f[cur].p, fp.p, mat.p, hs.p); break;
+    case 1: hipLaunchKernelGGL((plastic_kernel<1, R>), grid, blk, 0, stream, D, G, S.dt, f[cur].p, fp.p, mat.p, hs.p); break;
+    default: hipLaunchKernelGGL((plastic_kernel<2, R>), grid, blk, 0, stream, D, G, S.dt, f[cur].p, fp.p, mat.p, hs.p); break;
+  }
+  HIPCHK(hipGetLastError());
+}
+
+// domain::do_rk_stage, domain.cpp:228-305
+template <typename R>
+void Solver<R>::do_stage(int istep, int st, bool full_step) {
+  const bool last = (st + 1 == S.rk.nstages);
+  const double t = (double)istep * S.dt;
+  const double tst = t + S.rk.C[st] * S.dt;
+  const double tfront = (double)(istep + 1) * S.dt;
+  FaultStageDev fs{S.rk.A[st], S.rk.B[st], S.dt, tst, 0, (full_step && last && !S.plastic) ? tfront : -1.};
+  for (size_t i = 0; i < S.ifaces.size(); i++) launch_fault((int)i, fs);
+  StageDev sd{S.rk.A[st], S.rk.B[st], S.dt, (full_step && last && !S.plastic && sizeof(R) == 8) ? 1 : 0};
+  for (size_t b = 0; b < S.blocks.size(); b++) {
+    if (nev == NEV) flush_events();
+    HIPCHK(hipEventRecord(evs[2 * nev], stream));
+    launch_bulk((int)b, sd);
+    HIPCHK(hipEventRecord(evs[2 * nev + 1], stream));
+    nev++;
+  }
+  cur ^= 1;
+  if (last && S.plastic) {
+    for (size_t b = 0; b < S.blocks.size(); b++) launch_plastic((int)b);
+    FaultStageDev fs2{0., 0., S.dt, tst, 1, full_step ? tfront : -1.};
+    for (size_t i = 0; i < S.ifaces.size(); i++)
+      if (S.ifaces[i].law != LAW_LOCKED) launch_fault((int)i, fs2);
+  }
+}
+
+template <typename R>
+void Solver<R>::flush_events() {
+  for (int e = 0; e < nev; e++) {
+    HIPCHK(hipEventSynchronize(evs[2 * e + 1]));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, evs[2 * e], evs[2 * e + 1]));
+    bulk_time_ms += ms;
+    bulk_count += 1;
+  }
+  nev = 0;
+}
+
+template <typename R>
+void Solver<R>::step(int n) {
+  for (int k = 0; k < n; k++) {
+    for (int st = 0; st < S.rk.nstages; st++) do_stage(cur_step, st, true);
+    cur_step++;
+    if (opts.write_outputs) write_outputs(cur_step);
+    if (!opts.quiet && S.ninfo > 0 && cur_step % S.ninfo == 0) {
+      time_t raw;
+      time(&raw);
+      printf("Timestep %d of %d %s", cur_step, S.nt, asctime(localtime(&raw)));
+      fflush(stdout);
+    }
+  }
+  sync_check();
+}
+
+// outputunit::write_unit, outputunit.cpp:771-889 (stress fields absolute)
+template <typename R>
+void Solver<R>::write_outputs(int tstep) {
+  for (size_t u = 0; u < S.outputs.size(); u++) {
+    const OutputH& U = S.outputs[u];
+    if (tstep < U.tm || tstep > U.tp || (tstep - U.tm) % U.ts != 0) continue;
+    if (U.tp > U.tm) {
+      const double t = (double)tstep * S.dt;
+      fwrite(&t, sizeof(double), 1, outs[u].time);
+    }
+    const size_t cnt = (size_t)U.nx[0] * U.nx[1] * U.nx[2];
+    std::vector<double> host(cnt);
+    if (U.iface < 0) {
+      const int hi[3] = {U.xp[0], U.xp[1], U.xp[2]};
+      read_field(U.fidx, U.xm, hi, U.xs, host.data());
+      // absolute stress: the output runs between set_stress and remove_stress
+      if (U.fidx >= S.nv && U.fidx < S.nv + S.ns) {
+        // read_field returns f; add s0 (+ s) here in the same order
+        const int i = U.fidx - S.nv;
+        const double s0 = S.s0[S.sidx[i]];
+        size_t q = 0;
+        for (int a = U.xm[0]; a <= U.xp[0]; a += U.xs[0])
+          for (int b = U.xm[1]; b <= U.xp[1]; b += U.xs[1])
+            for (int c = U.xm[2]; c <= U.xp[2]; c += U.xs[2], q++) {
+              double v = host[q] + s0;
+              if (S.hetstress) v += S.s[(size_t)i * S.N + S.idx(a, b, c)];
+              host[q] = v;
+            }
+      }
+    } else {
+      const IfaceH& I = S.ifaces[U.iface];
+      const long n = (long)I.n0 * I.n1;
+      std::vector<double> arr((size_t)(S.ndim - 1) * n > (size_t)n ? (size_t)(S.ndim - 1) * n : (size_t)n);
+      read_iface(U.iface, U.fidx, arr.data(), (long)arr.size());
+      const int a0 = (I.dir == 0) ? 1 : 0, a1 = (I.dir == 2) ? 1 : 2;
+      size_t q = 0;
+      for (int a = U.xm[0]; a <= U.xp[0]; a += U.xs[0])
+        for (int b = U.xm[1]; b <= U.xp[1]; b += U.xs[1])
+          for (int c = U.xm[2]; c <= U.xp[2]; c += U.xs[2], q++) {
+            const int cc[3] = {a, b, c};
+            host[q] = arr[(size_t)U.comp * n + (size_t)(cc[a0] - I.xm[a0]) * I.n1 + (cc[a1] - I.xm[a1])];
+          }
+    }
+    fwrite(host.data(), sizeof(double), cnt, outs[u].data);
+  }
+}
+
+template <typename R>
+void Solver<R>::read_field(int field, const int lo[3], const int hi[3], const int st[3], double* h) {
+  int n[3];
+  for (int d = 0; d < 3; d++) {
+    if (lo[d] < 0 || hi[d] >= S.nx[d] || hi[d] < lo[d] || st[d] <= 0) throw Error{"read_field: bad index box"};
+    n[d] = (hi[d] - lo[d]) / st[d] + 1;
+  }
+  const size_t cnt = (size_t)n[0] * n[1] * n[2];
+  const R* src;
+  int q = field;
+  if (field < S.nf) src = f[cur].p + (size_t)field * S.N;
+  else if (field < S.nf + S.nfp) { src = fp.p + (size_t)(field - S.nf) * S.N; q = -1; }
+  else throw Error{"read_field: bad field"};
+  (void)q;
+  DBuf<double> tmp;
+  tmp.alloc(cnt);
+  const unsigned nb = (unsigned)((cnt + 255) / 256);
+  hipLaunchKernelGGL((gather_box<R>), dim3(nb), dim3(256), 0, stream, src, hs.p, 0., -1, G.N, G.Ny, G.Nz, lo[0], lo[1],
+                     lo[2], st[0], st[1], st[2], n[0], n[1], n[2], tmp.p);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(stream));
+  HIPCHK(hipMemcpy(h, tmp.p, cnt * sizeof(double), hipMemcpyDeviceToHost));
+}
+
+template <typename R>
+void Solver<R>::read_array(int which, double* h, long n) {
+  HIPCHK(hipStreamSynchronize(stream));
+  const size_t N = S.N;
+  auto fetch = [&](const R* src, size_t cnt, double* dst) {
+    if (std::is_same<R, double>::value) {
+      HIPCHK(hipMemcpy(dst, src, cnt * sizeof(double), hipMemcpyDeviceToHost));
+    } else {
+      std::vector<R> tmp(cnt);
+      HIPCHK(hipMemcpy(tmp.data(), src, cnt * sizeof(R), hipMemcpyDeviceToHost));
+      for (size_t i = 0; i < cnt; i++) dst[i] = (double)tmp[i];
+    }
+  };
+  switch (which) {
+    case 0: {
+      const size_t need = (size_t)(S.nf + S.nfp) * N;
+      if ((size_t)n != need) throw Error{"read_array: size mismatch"};
+      fetch(f[cur].p, (size_t)S.nf * N, h);
+      if (S.nfp) fetch(fp.p, (size_t)S.nfp * N, h + (size_t)S.nf * N);
+      break;
+    }
+    case 1:
+      if ((size_t)n != (size_t)S.nf * N) throw Error{"read_array: size mismatch"};
+      fetch(df.p, (size_t)n, h);
+      break;
+    case 2:
+      if ((size_t)n != S.x.size()) throw Error{"read_array: size mismatch"};
+      memcpy(h, S.x.data(), n * sizeof(double));
+      break;
+    case 3:
+      if ((size_t)n != S.metric.size()) throw Error{"read_array: size mismatch"};
+      memcpy(h, S.metric.data(), n * sizeof(double));
+      break;
+    case 4:
+      if ((size_t)n != S.jac.size()) throw Error{"read_array: size mismatch"};
+      memcpy(h, S.jac.data(), n * sizeof(double));
+      break;
+    default:
+      throw Error{"read_array: bad array id"};
+  }
+}
+
+template <typename R>
+void Solver<R>::write_array(int which, const double* h, long n) {
+  HIPCHK(hipStreamSynchronize(stream));
+  const size_t N = S.N;
+  auto put = [&](R* dst, const double* src, size_t cnt) {
+    std::vector<R> tmp(cnt);
+    for (size_t i = 0; i < cnt; i++) tmp[i] = (R)src[i];
+    HIPCHK(hipMemcpy(dst, tmp.data(), cnt * sizeof(R), hipMemcpyHostToDevice));
+  };
+  if (which == 0) {
+    if ((size_t)n != (size_t)(S.nf + S.nfp) * N) throw Error{"write_array: size mismatch"};
+    put(f[cur].p, h, (size_t)S.nf * N);
+    if (S.nfp) put(fp.p, h + (size_t)S.nf * N, (size_t)S.nfp * N);
+  } else if (which == 1) {
+    if ((size_t)n != (size_t)S.nf * N) throw Error{"write_array: size mismatch"};
+    put(df.p, h, (size_t)n);
+  } else {
+    throw Error{"write_array: only f and df are writable"};
+  }
+}
+
+template <typename R>
+long Solver<R>::read_iface(int iface, int which, double* h, long n) {
+  if (iface < 0 || iface >= (int)S.ifaces.size()) throw Error{"read_iface: bad interface"};
+  IfBufs& ib = ibufs[iface];
+  DBuf<double>* b = nullptr;
+  switch (which) {
+    case 0: b = &ib.u; break;
+    case 1: b = &ib.v; break;
+    case 2: b = &ib.s; break;
+    case 3: b = &ib.sn; break;
+    case 4: b = &ib.ux; break;
+    case 5: b = &ib.vx; break;
+    case 6: b = &ib.sx; break;
+    case 7: b = &ib.state; break;
+    case 8: b = &ib.du; break;
+    case 9: b = &ib.dux; break;
+    case 10: b = &ib.dstate; break;
+    default: throw Error{"read_iface: bad array id"};
+  }
+  if (S.ifaces[iface].law == LAW_LOCKED) throw Error{"read_iface: interface is not frictional"};
+  if ((which == 7 || which == 10) && S.ifaces[iface].law != LAW_STZ) throw Error{"read_iface: interface has no state"};
+  if (!h) return (long)b->n;
+  if (n < (long)b->n) throw Error{"read_iface: buffer too small"};
+  HIPCHK(hipStreamSynchronize(stream));
+  HIPCHK(hipMemcpy(h, b->p, b->n * sizeof(double), hipMemcpyDeviceToHost));
+  return (long)b->n;
+}
+
+template <typename R>
+long Solver<R>::read_front(int iface, double* h, long n) {
+  if (iface < 0 || iface >= (int)S.ifaces.size() || !ibufs[iface].front.p) return -1;
+  DBuf<double>& b = ibufs[iface].front;
+  if (!h) return (long)b.n;
+  if (n < (long)b.n) throw Error{"read_front: buffer too small"};
+  HIPCHK(hipStreamSynchronize(stream));
+  HIPCHK(hipMemcpy(h, b.p, b.n * sizeof(double), hipMemcpyDeviceToHost));
+  return (long)b.n;
+}
+
+// outputlist::close_list + frontlist::write_list (front.cpp:127-359)
+template <typename R>
+void Solver<R>::finish() {
+  for (OutFile& o : outs) {
+    if (o.data) fclose(o.data);
+    if (o.time) fclose(o.time);
+    o.data = o.time = nullptr;
+  }
+  if (!opts.write_outputs || !S.has_front) return;
+  const char endian = [] {
+    const uint32_t v = 0x01020304;
+    unsigned char c[4];
+    memcpy(c, &v, 4);
+    return c[0] == 4 ? '<' : (c[0] == 1 ? '>' : '=');
+  }();
+  for (size_t i = 0; i < S.ifaces.size(); i++) {
+    const IfaceH& I = S.ifaces[i];
+    if (I.law == LAW_LOCKED) continue;
+    const std::string pre = S.datadir + S.name + "_front_" + std::to_string(i);
+    std::vector<double> tv((size_t)I.n0 * I.n1);
+    read_front((int)i, tv.data(), (long)tv.size());
+    FILE* fpp = fopen((pre + "_t.dat").c_str(), "wb");
+    if (!fpp) throw Error{"Error opening file in front.cpp"};
+    fwrite(tv.data(), sizeof(double), tv.size(), fpp);
+    fclose(fpp);
+    static const char* xyz[3] = {"x", "y", "z"};
+    for (int c = 0; c < S.ndim; c++) {
+      FILE* xf = fopen((pre + "_" + xyz[c] + ".dat").c_str(), "wb");
+      if (!xf) throw Error{"Error opening file in outputunit.cpp"};
+      std::vector<double> buf;
+      for (int a = I.xm[0]; a <= I.xp[0]; a++)
+        for (int b = I.xm[1]; b <= I.xp[1]; b++)
+          for (int cc = I.xm[2]; cc <= I.xp[2]; cc++) buf.push_back(S.x[(size_t)c * S.N + S.idx(a, b, cc)]);
+      fwrite(buf.data(), sizeof(double), buf.size(), xf);
+      fclose(xf);
+    }
+    FILE* m = fopen((pre + ".m").c_str(), "w");
+    if (!m) throw Error{"Error writing parameters to matlab file"};
+    fprintf(m, "endian = '%c';\nnx = %d;\nny = %d;\n", endian == '<' ? 'l' : (endian == '>' ? 'b' : 'n'), I.n0, I.n1);
+    fclose(m);
+    FILE* py = fopen((pre + ".py").c_str(), "w");
+    if (!py) throw Error{"Error writing parameters to python file"};
+    fprintf(py, "endian = '%c'\nnx = %d\nny = %d\n", endian, I.n0, I.n1);
+    fclose(py);
+  }
+}
+
+template <typename R>
+double Solver<R>::bulk_ms(int* n) {
+  flush_events();
+  const double avg = bulk_count ? bulk_time_ms / bulk_count : 0.;
+  if (n) *n = bulk_count;
+  bulk_time_ms = 0.;
+  bulk_count = 0;
+  return avg;
+}
+
+}  // namespace fdf
+
+struct fdf_s {
+  std::unique_ptr<fdf::SolverBase> solver;
+};
+
+using fdf::Error;
+
+template <typename Fn>
+static int guarded(Fn&& fn) {
+  try {
+    fn();
+    return 0;
+  } catch (Error& e) {
+    fdf::g_last_error = e.msg;
+  } catch (std::exception& e) {
+    fdf::g_last_error = e.what();
+  } catch (...) {
+    fdf::g_last_error = "unknown error";
+  }
+  return -1;
+}
+
+extern "C" {
+
+const char* fdf_last_error(void) { return fdf::g_last_error.c_str(); }
+
+int fdf_open(const char* in_path, const fdf_opts* opts, fdf_t** out) {
+  if (!in_path || !out) {
+    fdf::g_last_error = "fdf_open: null argument";
+    return -2;
+  }
+  *out = nullptr;
+  fdf_opts o{};
+  o.quiet = 1;
+  if (opts) o = *opts;
+  return guarded([&] {
+    std::unique_ptr<fdf_s> h(new fdf_s());
+    if (o.precision == FDF_FP32) h->solver.reset(new fdf::Solver<float>(in_path, o));
+    else if (o.precision == FDF_FP64) h->solver.reset(new fdf::Solver<double>(in_path, o));
+    else throw Error{"fdf_open: unknown precision"};
+    *out = h.release();
+  });
+}
+
+void fdf_close(fdf_t* h) { delete h; }
+
+int fdf_step(fdf_t* h, int nsteps) {
+  return guarded([&] { h->solver->step(nsteps); });
+}
+
+int fdf_stage(fdf_t* h, int istep, int stage) {
+  return guarded([&] {
+    if (stage < 0 || stage >= h->solver->S.rk.nstages) throw Error{"fdf_stage: bad stage"};
+    h->solver->stage(istep, stage);
+  });
+}
+
+int fdf_run(const char* in_path, const fdf_opts* opts) {
+  fdf_opts o{};
+  if (opts) o = *opts;
+  o.write_outputs = 1;
+  fdf_t* h = nullptr;
+  int rc = fdf_open(in_path, &o, &h);
+  if (rc) return rc;
+  rc = fdf_step(h, h->solver->S.nt);
+  if (!rc) rc = fdf_finish(h);
+  fdf_close(h);
+  return rc;
+}
+
+int fdf_read_field(fdf_t* h, int field, const int lo[3], const int hi[3], const int st[3], double* host) {
+  return guarded([&] { h->solver->read_field(field, lo, hi, st, host); });
+}
+
+int fdf_read_array(fdf_t* h, int which, double* host, long n) {
+  return guarded([&] { h->solver->read_array(which, host, n); });
+}
+
+int fdf_write_array(fdf_t* h, int which, const double* host, long n) {
+  return guarded([&] { h->solver->write_array(which, host, n); });
+}
+
+long fdf_read_iface(fdf_t* h, int iface, int which, double* host, long n) {
+  long r = -1;
+  int rc = guarded([&] { r = h->solver->read_iface(iface, which, host, n); });
+  return rc ? -1 : r;
+}
+
+long fdf_read_front(fdf_t* h, int iface, double* host, long n) {
+  long r = -1;
+  int rc = guarded([&] { r = h->solver->read_front(iface, host, n); });
+  return rc ? -1 : r;
+}
+
+int fdf_finish(fdf_t* h) {
+  return guarded([&] { h->solver->finish(); });
+}
+
+int fdf_get_info(fdf_t* h, fdf_info* info) {
+  if (!h || !info) return -2;
+  const fdf::SetupH& S = h->solver->S;
+  info->nt = S.nt;
+  info->nstages = S.rk.nstages;
+  info->ndim = S.ndim;
+  info->mode = S.mode;
+  for (int i = 0; i < 3; i++) info->nx[i] = S.nx[i];
+  info->nfields = S.nf;
+  info->nfields_total = S.nf + S.nfp;
+  info->nifaces = (int)S.ifaces.size();
+  info->sbporder = S.sbporder;
+  info->plastic = S.plastic;
+  info->nblocks = (int)S.blocks.size();
+  info->step = h->solver->cur_step;
+  info->precision = h->solver->opts.precision;
+  info->dt = S.dt;
+  info->ttot = S.ttot;
+  info->cfl = S.cfl;
+  info->t = h->solver->cur_step * S.dt;
+  return 0;
+}
+
+double fdf_bulk_kernel_ms(fdf_t* h, int* nlaunch) { return h->solver->bulk_ms(nlaunch); }
+
+int fdf_host_setup(const char* in_path, fdf_info* info, double* x, double* metric, double* jac) {
+  return guarded([&] {
+    fdf::SetupH S;
+    fdf::compile_problem(in_path, S, false, true);
+    if (info) {
+      memset(info, 0, sizeof(*info));
+      info->nt = S.nt;
+      info->nstages = S.rk.nstages;
+      info->ndim = S.ndim;
+      info->mode = S.mode;
+      for (int i = 0; i < 3; i++) info->nx[i] = S.nx[i];
+      info->nfields = S.nf;
+      info->nfields_total = S.nf + S.nfp;
+      info->nifaces = (int)S.ifaces.size();
+      info->sbporder = S.sbporder;
+      info->plastic = S.plastic;
+      info->nblocks = (int)S.blocks.size();
+      info->dt = S.dt;
+      info->ttot = S.ttot;
+      info->cfl = S.cfl;
+    }
+    if (x) memcpy(x, S.x.data(), S.x.size() * sizeof(double));
+    if (metric) memcpy(metric, S.metric.data(), S.metric.size() * sizeof(double));
+    if (jac) memcpy(jac, S.jac.data(), S.jac.size() * sizeof(double));
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,207 @@
+"""ctypes mirror of the C ABI in include/fdfault_hip.h.
+
+Mirrors the reference's driver objects: :class:`Solver` plays
+``problem`` (src/problem.cpp) -- construct from a ``.in`` file, ``step()``
+runs ``problem::solve`` iterations, ``stage()`` one ``domain::do_rk_stage``;
+arrays are read back like the reference's friend-access output code does.
+Errors raise :class:`FdfaultError` with the reference's message text.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_lib", "libfdfault_hip.so")
+EXE_PATH = os.path.join(HERE, "_lib", "fdfault")
+
+FP64, FP32 = 0, 1
+IFACE_FIELDS = {"U": 0, "V": 1, "S": 2, "Sn": 3, "Ux": 4, "Vx": 5, "Sx": 6, "state": 7,
+                "du": 8, "dux": 9, "dstate": 10}
+ARRAYS = {"f": 0, "df": 1, "x": 2, "metric": 3, "jac": 4}
+
+
+class FdfaultError(RuntimeError):
+    pass
+
+
+class Opts(C.Structure):
+    _fields_ = [("precision", C.c_int), ("device", C.c_int), ("write_outputs", C.c_int),
+                ("quiet", C.c_int), ("use_graph", C.c_int)]
+
+
+class Info(C.Structure):
+    _fields_ = [("nt", C.c_int), ("nstages", C.c_int), ("ndim", C.c_int), ("mode", C.c_int),
+                ("nx", C.c_int * 3), ("nfields", C.c_int), ("nfields_total", C.c_int),
+                ("nifaces", C.c_int), ("sbporder", C.c_int), ("plastic", C.c_int), ("nblocks", C.c_int),
+                ("step", C.c_int), ("precision", C.c_int),
+                ("dt", C.c_double), ("ttot", C.c_double), ("cfl", C.c_double), ("t", C.c_double)]
+
+
+EXPORTS = ["fdf_open", "fdf_step", "fdf_stage", "fdf_run", "fdf_read_field", "fdf_read_array",
+           "fdf_write_array", "fdf_read_iface", "fdf_read_front", "fdf_finish", "fdf_get_info",
+           "fdf_host_setup", "fdf_last_error", "fdf_close", "fdf_bulk_kernel_ms"]
+
+_lib = None
+
+
+def load():
+    """Load the HIP library; raises if it was not built (no fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise FdfaultError("fdfault_amd HIP library not built: %s (run fdfault_amd/build.py)" % LIB_PATH)
+    L = C.CDLL(LIB_PATH)
+    dp = C.POINTER(C.c_double)
+    ip = C.POINTER(C.c_int)
+    L.fdf_open.argtypes = [C.c_char_p, C.POINTER(Opts), C.POINTER(C.c_void_p)]
+    L.fdf_step.argtypes = [C.c_void_p, C.c_int]
+    L.fdf_stage.argtypes = [C.c_void_p, C.c_int, C.c_int]
+    L.fdf_run.argtypes = [C.c_char_p, C.POINTER(Opts)]
+    L.fdf_read_field.argtypes = [C.c_void_p, C.c_int, ip, ip, ip, dp]
+    L.fdf_read_array.argtypes = [C.c_void_p, C.c_int, dp, C.c_long]
+    L.fdf_write_array.argtypes = [C.c_void_p, C.c_int, dp, C.c_long]
+    L.fdf_read_iface.restype = C.c_long
+    L.fdf_read_iface.argtypes = [C.c_void_p, C.c_int, C.c_int, dp, C.c_long]
+    L.fdf_read_front.restype = C.c_long
+    L.fdf_read_front.argtypes = [C.c_void_p, C.c_int, dp, C.c_long]
+    L.fdf_finish.argtypes = [C.c_void_p]
+    L.fdf_get_info.argtypes = [C.c_void_p, C.POINTER(Info)]
+    L.fdf_host_setup.argtypes = [C.c_char_p, C.POINTER(Info), dp, dp, dp]
+    L.fdf_last_error.restype = C.c_char_p
+    L.fdf_close.argtypes = [C.c_void_p]
+    L.fdf_bulk_kernel_ms.restype = C.c_double
+    L.fdf_bulk_kernel_ms.argtypes = [C.c_void_p, ip]
+    _lib = L
+    return L
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _err():
+    return FdfaultError(load().fdf_last_error().decode())
+
+
+def host_setup(inpath: str):
+    """Grid, metric, Jacobian and dt built by the library's host problem
+    compiler (no GPU needed)."""
+    L = load()
+    info = Info()
+    if L.fdf_host_setup(inpath.encode(), C.byref(info), None, None, None) != 0:
+        raise _err()
+    N = info.nx[0] * info.nx[1] * info.nx[2]
+    x = np.empty(info.ndim * N)
+    m = np.empty(info.ndim * info.ndim * N)
+    j = np.empty(N)
+    if L.fdf_host_setup(inpath.encode(), None, _dp(x), _dp(m), _dp(j)) != 0:
+        raise _err()
+    shp = tuple(info.nx)
+    return info, x.reshape((info.ndim,) + shp), m.reshape((info.ndim * info.ndim,) + shp), j.reshape(shp)
+
+
+class Solver:
+    """One problem on the GPU (mirror of the reference's ``problem`` class)."""
+
+    def __init__(self, inpath: str, precision: int = FP64, device: int = 0, write_outputs: bool = False,
+                 quiet: bool = True):
+        L = load()
+        self.opts = Opts(precision, device, 1 if write_outputs else 0, 1 if quiet else 0, 0)
+        h = C.c_void_p()
+        if L.fdf_open(inpath.encode(), C.byref(self.opts), C.byref(h)) != 0:
+            raise _err()
+        self.h = h
+        self.info = Info()
+        L.fdf_get_info(self.h, C.byref(self.info))
+        self.nx = tuple(self.info.nx)
+        self.N = self.nx[0] * self.nx[1] * self.nx[2]
+        self.ndim = self.info.ndim
+        self.nt = self.info.nt
+        self.dt = self.info.dt
+        self.nstages = self.info.nstages
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().fdf_close(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def step(self, n: int = 1):
+        if load().fdf_step(self.h, n) != 0:
+            raise _err()
+
+    def stage(self, istep: int, stage: int):
+        if load().fdf_stage(self.h, istep, stage) != 0:
+            raise _err()
+
+    def array(self, which: str):
+        i = self.info
+        n = {"f": i.nfields_total, "df": i.nfields, "x": i.ndim, "metric": i.ndim * i.ndim, "jac": 1}[which]
+        out = np.empty(n * self.N)
+        if load().fdf_read_array(self.h, ARRAYS[which], _dp(out), out.size) != 0:
+            raise _err()
+        return out.reshape((n,) + self.nx)
+
+    def set_array(self, which: str, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64).ravel()
+        if load().fdf_write_array(self.h, ARRAYS[which], _dp(a), a.size) != 0:
+            raise _err()
+
+    def field(self, q: int, lo, hi, st=(1, 1, 1)):
+        lo = (C.c_int * 3)(*lo)
+        hi_ = (C.c_int * 3)(*hi)
+        st = (C.c_int * 3)(*st)
+        n = [(hi_[d] - lo[d]) // st[d] + 1 for d in range(3)]
+        out = np.empty(n[0] * n[1] * n[2])
+        if load().fdf_read_field(self.h, q, lo, hi_, st, _dp(out)) != 0:
+            raise _err()
+        return out.reshape(n)
+
+    def iface(self, i: int, which: str):
+        L = load()
+        n = L.fdf_read_iface(self.h, i, IFACE_FIELDS[which], None, 0)
+        if n < 0:
+            raise _err()
+        out = np.empty(n)
+        if L.fdf_read_iface(self.h, i, IFACE_FIELDS[which], _dp(out), n) < 0:
+            raise _err()
+        return out
+
+    def front(self, i: int):
+        L = load()
+        n = L.fdf_read_front(self.h, i, None, 0)
+        if n < 0:
+            return None
+        out = np.empty(n)
+        L.fdf_read_front(self.h, i, _dp(out), n)
+        return out
+
+    def finish(self):
+        if load().fdf_finish(self.h) != 0:
+            raise _err()
+
+    def bulk_kernel_ms(self):
+        n = C.c_int(0)
+        ms = load().fdf_bulk_kernel_ms(self.h, C.byref(n))
+        return ms, n.value
+
+
+def run(inpath: str, precision: int = FP64, quiet: bool = True):
+    o = Opts(precision, 0, 1, 1 if quiet else 0, 0)
+    if load().fdf_run(inpath.encode(), C.byref(o)) != 0:
+        raise _err()

@@ -1,0 +1,107 @@
+/*
+ * fdfault_hip -- C ABI of the MI355X (gfx950) time-stepping core of fdfault.
+ *
+ * The reference solver (edaub/fdfault) has no library seam: its hot path is
+ * the C++ member function domain::do_rk_stage (src/domain.hpp:26,
+ * src/domain.cpp:228-305), driven by problem::solve (src/problem.cpp:136-192)
+ * and reached from the `fdfault <file.in>` executable (src/main.cpp:8-72).
+ * Each entry point below states which reference interface it replaces.  All
+ * entry points return 0 on success and a negative code on error; the message
+ * the reference would have printed before MPI_Abort is available from
+ * fdf_last_error().  No C++ exception crosses this boundary.  Handles are
+ * independent; one host thread per handle.
+ */
+#ifndef FDFAULT_HIP_H
+#define FDFAULT_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fdf_s fdf_t;
+
+/* precision of the bulk fields f/df; fault state is always FP64 */
+enum { FDF_FP64 = 0, FDF_FP32 = 1 };
+
+typedef struct fdf_opts {
+  int precision;     /* FDF_FP64 (default) or FDF_FP32                          */
+  int device;        /* HIP device ordinal                                      */
+  int write_outputs; /* 1: open the .in output list and front files (like the
+                        reference executable); 0: state stays on the device   */
+  int quiet;         /* 1: no stdout progress lines                             */
+  int use_graph;     /* 1: capture the per-step launch sequence in a hipGraph   */
+} fdf_opts;
+
+typedef struct fdf_info {
+  int nt, nstages, ndim, mode;
+  int nx[3];
+  int nfields;       /* evolved fields (RK updated)                             */
+  int nfields_total; /* including plastic fields                                */
+  int nifaces, sbporder, plastic, nblocks, step, precision;
+  double dt, ttot, cfl, t;
+} fdf_info;
+
+/* Replaces problem::problem(const char*) (src/problem.cpp:16-84): parses the
+ * .in file, builds grid/metric/normals, chooses dt, uploads to the device,
+ * performs the t=0 interface solve, initial outputs and front. */
+int fdf_open(const char* in_path, const fdf_opts* opts, fdf_t** out);
+
+/* Replaces nsteps iterations of the loop in problem::solve
+ * (src/problem.cpp:151-177): all RK stages via do_rk_stage, plasticity,
+ * outputs and front update. */
+int fdf_step(fdf_t* h, int nsteps);
+
+/* Replaces one call of domain::do_rk_stage(dt, stage, t, rk)
+ * (src/domain.cpp:228-305) for time step `istep` (t = istep*dt); no
+ * end-of-step work.  Used by kernel-level parity tests. */
+int fdf_stage(fdf_t* h, int istep, int stage);
+
+/* Whole-run equivalent of `fdfault <file.in>` (src/main.cpp:8-72). */
+int fdf_run(const char* in_path, const fdf_opts* opts);
+
+/* Read back a strided box of one field of f (the reference's outputunit
+ * reads f through friend access, src/outputunit.cpp:791): indices lo..hi
+ * inclusive with stride st, C order, as FP64. */
+int fdf_read_field(fdf_t* h, int field, const int lo[3], const int hi[3], const int st[3], double* host);
+
+/* Whole arrays, FP64 on the host: which = 0 f, 1 df, 2 x, 3 metric, 4 jac
+ * (fields::f/df/x/metric/jac, src/fields.hpp:55-62).  n = element count. */
+int fdf_read_array(fdf_t* h, int which, double* host, long n);
+int fdf_write_array(fdf_t* h, int which, const double* host, long n);
+
+/* Interface arrays (interface::u/v/s/sn/ux/vx/sx/state, src/interface.hpp:64-71,
+ * plus friction::du/dux/dstate): which = 0 U,1 V,2 S,3 Sn,4 Ux,5 Vx,6 Sx,
+ * 7 state,8 du,9 dux,10 dstate.  Returns the element count (host may be NULL
+ * to query it) or a negative code. */
+long fdf_read_iface(fdf_t* h, int iface, int which, double* host, long n);
+
+/* Rupture-front times of a frictional interface (front::tvals,
+ * src/front.cpp:103-125); returns the element count or a negative code. */
+long fdf_read_front(fdf_t* h, int iface, double* host, long n);
+
+/* Flushes and closes output files and writes front files
+ * (outputlist::close_list, frontlist::write_list; src/problem.cpp:182-187). */
+int fdf_finish(fdf_t* h);
+
+/* Problem summary (problem/domain members: nt, dt, ttot, cfl, sizes). */
+int fdf_get_info(fdf_t* h, fdf_info* info);
+
+/* Host-only set-up check: parses the .in file and builds grid, metric,
+ * Jacobian and dt on the host without touching a GPU (the reference does this
+ * in block::set_grid, src/block.cpp:769-1046, and problem::set_time_step,
+ * src/problem.cpp:94-134).  Any output pointer may be NULL; sizes are
+ * ndim*N, ndim*ndim*N and N with N = nx*ny*nz. */
+int fdf_host_setup(const char* in_path, fdf_info* info, double* x, double* metric, double* jac);
+const char* fdf_last_error(void);
+void fdf_close(fdf_t* h);
+
+/* Average device time (ms) of the bulk stencil kernel over the launches since
+ * the last call, measured with HIP events on the stream it runs on; count of
+ * launches in *nlaunch.  Measurement hook for the benchmark; no counterpart
+ * in the reference. */
+double fdf_bulk_kernel_ms(fdf_t* h, int* nlaunch);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,58 @@
+"""Problem files shared by the parity tests: small variants of the
+reference's example problems (problems/test2d.in, problems/tpv4.in) that
+exercise every branch of the hot path -- modes, SBP orders, RK orders,
+boundary types, interface laws, plasticity, dissipation, heterogeneous
+stress/material and curved (file) block surfaces."""
+from __future__ import annotations
+
+import os
+
+import numpy as np
+
+from fdfault_amd.problems import Output, Pert, two_block_fault
+
+STZ_PARAMS = dict(chi0=0.0, statefile="none",
+                  perts=[Pert("constant", 0., 0., 0., 0., 0., (1.e-6, 0.3, 0.01, 0.1, 1.e4, 1.e-3, 0.1, 0.4, 1.e4))])
+
+
+def write_case(tmpdir: str, name: str, **kw) -> str:
+    d = os.path.join(tmpdir, name)
+    os.makedirs(os.path.join(d, "data"), exist_ok=True)
+    kw.setdefault("datadir", os.path.join(d, "data") + "/")
+    kw.setdefault("name", name)
+    p = two_block_fault(**kw)
+    return p.write(os.path.join(d, name + ".in"))
+
+
+def curved_surfaces(tmpdir, nx, ny, dx, amp=0.05, ndim=2, nz=1):
+    """Block surfaces for a sinusoidal fault y = Ly + amp*sin(2 pi x / Lx):
+    block 1 (below) gets the curve as its top face, block 2 as its bottom face
+    (surface files [ndim][n][1], surface.cpp:11-50)."""
+    Lx = (nx - 1) * dx
+    Ly = (ny - 1) * dx
+    xs = np.linspace(0., Lx, nx)
+    yf = Ly + amp * np.sin(2 * np.pi * xs / Lx)
+    paths = []
+    for blk, y0 in ((0, 0.), (1, Ly)):
+        faces = []
+        for face in range(4):
+            if face in (0, 1):  # x faces: curve in y
+                x = 0. if face == 0 else Lx
+                yb = 0. if blk == 0 else yf[0 if face == 0 else -1]
+                yt = yf[0 if face == 0 else -1] if blk == 0 else 2 * Ly
+                arr = np.zeros((2, ny, 1))
+                arr[0] = x
+                arr[1, :, 0] = np.linspace(yb, yt, ny)
+            else:
+                if (blk == 0 and face == 3) or (blk == 1 and face == 2):
+                    yy = yf
+                else:
+                    yy = np.full(nx, 0. if face == 2 else 2 * Ly)
+                arr = np.zeros((2, nx, 1))
+                arr[0, :, 0] = xs
+                arr[1, :, 0] = yy
+            pth = os.path.join(tmpdir, "surf_b%d_f%d.bin" % (blk, face))
+            np.ascontiguousarray(arr).tofile(pth)
+            faces.append(pth)
+        paths.append(faces)
+    return paths

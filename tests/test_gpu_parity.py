@@ -1,0 +1,177 @@
+"""GPU parity: the HIP path against the CPU oracle on the same .in files.
+
+FP64 bar: every field, slip, slip rate, traction, state and rupture-front
+time within 1e-10 relative (max-norm of the difference over the max-norm of
+the oracle field; front times within 1e-10 of the run time).  In practice
+the FP64 path is bit-identical: kernels keep the reference's association and
+are built with -ffp-contract=off; transcendental calls (STZ exp/log) may
+differ by an ulp, hence the tolerance rather than equality.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from cases import STZ_PARAMS, curved_surfaces, write_case
+from fdfault_amd.problems import Output, Pert
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10
+
+
+def _rel(a, b):
+    scale = max(np.abs(b).max(), 1e-300)
+    return np.abs(a - b).max() / scale
+
+
+def _check(path, nsteps, iface_fields=("U", "V", "S", "Sn", "Ux", "Vx", "Sx"), tol=TOL, state=False):
+    from fdfault_amd import lib
+    print("case", os.path.basename(path), "steps", nsteps, flush=True)
+    o = O.Oracle(path)
+    g = lib.Solver(path)
+    o.step(nsteps)
+    g.step(nsteps)
+    fo, fg = o.array("f"), g.array("f")
+    worst = 0.0
+    for q in range(fo.shape[0]):
+        r = _rel(fg[q], fo[q])
+        worst = max(worst, r)
+        assert r <= tol, ("field", q, r)
+    r = _rel(g.array("df"), o.array("df"))
+    assert r <= tol, ("df", r)
+    for i in range(o.nifaces):
+        shp = o.iface_shape(i)
+        if shp[3] == 0:
+            continue
+        fields = list(iface_fields) + (["state"] if state else [])
+        for w in fields:
+            a, b = g.iface(i, w), o.iface(i, w)
+            r = _rel(a, b)
+            assert r <= tol, (w, r)
+        fo_, fg_ = o.front(i), g.front(i)
+        if fo_ is not None:
+            assert np.abs(fo_ - fg_).max() <= tol * max(1.0, nsteps * o.dt)
+    g.close()
+    o.close()
+    return worst
+
+
+def test_mode2_slipweak_sbp4_rk4(tmp_path):
+    p = write_case(str(tmp_path), "m2", nx=81, ny=41, dx=0.1, nt=30)
+    _check(p, 30)
+
+
+@pytest.mark.parametrize("sbp,rk", [(2, 1), (2, 2), (3, 3), (3, 4), (4, 2)])
+def test_mode2_orders(tmp_path, sbp, rk):
+    p = write_case(str(tmp_path), "m2o", nx=61, ny=31, dx=0.1, nt=12, sbporder=sbp, rkorder=rk)
+    _check(p, 12)
+
+
+@pytest.mark.parametrize("bc", ["free", "rigid"])
+def test_mode2_boundaries(tmp_path, bc):
+    p = write_case(str(tmp_path), "m2b", nx=61, ny=31, dx=0.1, nt=15, bc=bc)
+    _check(p, 15)
+
+
+def test_mode2_dissipation(tmp_path):
+    p = write_case(str(tmp_path), "m2d", nx=61, ny=31, dx=0.1, nt=15, cdiss=0.1)
+    _check(p, 15)
+
+
+def test_mode3_slipweak(tmp_path):
+    loads = [Pert("constant", 0., 0., 0., 0., 0., (-120., 0., 70.)),
+             Pert("boxcar", 0., 3.0, 0.6, 0., 0., (0., 0., 11.6))]
+    p = write_case(str(tmp_path), "m3", mode=3, nx=61, ny=31, dx=0.1, nt=20, loads=loads)
+    _check(p, 20)
+
+
+@pytest.mark.parametrize("law", ["locked", "frictionless"])
+def test_mode2_other_laws(tmp_path, law):
+    p = write_case(str(tmp_path), "m2l", nx=61, ny=31, dx=0.1, nt=12, law=law,
+                   s0=(-50., 20., 0., -60., 0., 0.), front=None if law == "locked" else ("V", 0.001))
+    fields = () if law == "locked" else ("U", "V", "S", "Sn", "Ux", "Vx", "Sx")
+    _check(p, 12, iface_fields=fields)
+
+
+def test_mode2_stz(tmp_path):
+    p = write_case(str(tmp_path), "stz2", nx=61, ny=31, dx=0.1, nt=15, law="stz",
+                   stz=dict(STZ_PARAMS, chi0=0.07))
+    _check(p, 15, state=True)
+
+
+NUC2 = [Pert("boxcar", 0., 3.0, 0.6, 0., 0., (0., 40., 0.))]
+NUC3 = [Pert("boxcar", 0., 1.2, 0.6, 0.8, 0.6, (0., 40., 0.))]
+
+
+def test_mode2_plastic(tmp_path):
+    """Drucker-Prager close to yield: the rupture drives points over it."""
+    plastic = dict(mu=0.68, c=0., beta=0.35, eta=0.05 * 32.04)
+    p = write_case(str(tmp_path), "pl2", nx=61, ny=31, dx=0.1, nt=15, material="plastic", plastic=plastic,
+                   s0=(-100., 52., 0., -120., 0., -55.), loads=NUC2)
+    _check(p, 15)
+
+
+def test_mode2_curved(tmp_path):
+    surf = curved_surfaces(str(tmp_path), 61, 31, 0.1, amp=0.1)
+    p = write_case(str(tmp_path), "curv", nx=61, ny=31, dx=0.1, nt=15, surf=surf)
+    _check(p, 15)
+
+
+def test_mode2_het_stress_material(tmp_path):
+    nx, ny = 61, 31
+    rng = np.random.default_rng(1)
+    N = nx * 2 * ny
+    s = rng.normal(0., 2., size=(3, nx, 2 * ny, 1))
+    sp = str(tmp_path / "hs.bin")
+    s.tofile(sp)
+    mat = np.empty((3, nx, 2 * ny, 1))
+    mat[0] = 2.67 * (1 + 0.05 * rng.random((nx, 2 * ny, 1)))
+    mat[1] = 32.04 * (1 + 0.05 * rng.random((nx, 2 * ny, 1)))
+    mat[2] = 32.04 * (1 + 0.05 * rng.random((nx, 2 * ny, 1)))
+    mp = str(tmp_path / "mat.bin")
+    mat.tofile(mp)
+    p = write_case(str(tmp_path), "het", nx=nx, ny=ny, dx=0.1, nt=12, hetstressfile=sp, matfile=mp)
+    _check(p, 12)
+
+
+def test_3d_slipweak(tmp_path):
+    loads = [Pert("constant", 0., 0., 0., 0., 0., (-120., 70., 0.)),
+             Pert("boxcar", 0., 1.5, 0.4, 1.0, 0.4, (0., 11.6, 0.))]
+    p = write_case(str(tmp_path), "d3", ndim=3, nx=31, ny=16, nz=21, dx=0.1, nt=8, loads=loads)
+    _check(p, 8)
+
+
+def test_3d_stz(tmp_path):
+    loads = [Pert("constant", 0., 0., 0., 0., 0., (-120., 70., 0.))]
+    p = write_case(str(tmp_path), "d3stz", ndim=3, nx=25, ny=14, nz=17, dx=0.1, nt=6, law="stz",
+                   stz=dict(STZ_PARAMS, chi0=0.07), loads=loads)
+    _check(p, 6, state=True)
+
+
+def test_3d_plastic(tmp_path):
+    plastic = dict(mu=0.675, c=0., beta=0.35, eta=0.05 * 32.04)
+    p = write_case(str(tmp_path), "d3pl", ndim=3, nx=25, ny=14, nz=17, dx=0.1, nt=6, material="plastic",
+                   plastic=plastic, s0=(-100., 52., 0., -120., 0., -55.), bc="free", loads=NUC3, plastic_tensor=1)
+    _check(p, 6)
+
+
+def test_kernel_level_random_state(tmp_path):
+    """One RK stage from a random (f, df): every branch of the stencil,
+    boundary and interface SAT is active at once."""
+    from fdfault_amd import lib
+    p = write_case(str(tmp_path), "rnd", nx=41, ny=21, dx=0.1, nt=2, cdiss=0.05)
+    o = O.Oracle(p)
+    g = lib.Solver(p)
+    rng = np.random.default_rng(7)
+    f = rng.normal(size=o.array("f").shape)
+    df = rng.normal(size=o.array("df").shape)
+    for h in (o, g):
+        h.set_array("f", f)
+        h.set_array("df", df)
+    for st in range(o.nstages):
+        o.stage(0, st)
+        g.stage(0, st)
+        assert _rel(g.array("df"), o.array("df")) <= TOL
+        assert _rel(g.array("f"), o.array("f")) <= TOL
