@@ -69,6 +69,14 @@ struct GridDev {
   int nfp, ptensor;
 };
 
+// Sub-boxes of a block a bulk_kernel launch covers (the whole block, or the
+// shell around the tiled interior): thread t maps into them by prefix offsets.
+struct Boxes {
+  int nbox;
+  int lo[6][3], n[6][3];
+  long long start[7];
+};
+
 struct StageDev {
   double A, B, dt;
   int roundtrip;  // end of an elastic step: the set_stress/remove_stress rounding of problem.cpp:160-164
@@ -319,20 +327,22 @@ __device__ void ext_sat(C* acc, const FaceDev& F, const R* __restrict__ f, const
 // Per point the accumulation order is the reference's: all x taps, then y,
 // then z; each tap adds every field term then the dissipation terms.
 template <int M, typename R, typename C, bool DISS, bool HET>
-__global__ void __launch_bounds__(256) bulk_kernel(BlockDev bd, GridDev g, StageDev st, const R* __restrict__ f,
+__global__ void __launch_bounds__(256) bulk_kernel(BlockDev bd, Boxes bx, GridDev g, StageDev st, const R* __restrict__ f,
                                                    R* __restrict__ fn, R* __restrict__ df, const R* __restrict__ mt,
                                                    const R* __restrict__ jac, const R* __restrict__ mat,
                                                    const double* __restrict__ hs) {
   constexpr int NFE = Nf<M>::v;
   constexpr int ND = (M == 2) ? 3 : 2;
-  const long long npts = (long long)bd.n[0] * bd.n[1] * bd.n[2];
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= npts) return;
-  const int lk = (int)(t % bd.n[2]);
-  const long long r = t / bd.n[2];
-  const int lj = (int)(r % bd.n[1]);
-  const int li = (int)(r / bd.n[1]);
-  const int c3[3] = {bd.lo[0] + li, bd.lo[1] + lj, bd.lo[2] + lk};
+  if (t >= bx.start[bx.nbox]) return;
+  int bb = 0;
+  while (t >= bx.start[bb + 1]) bb++;
+  const long long loc = t - bx.start[bb];
+  const int lk = (int)(loc % bx.n[bb][2]);
+  const long long r = loc / bx.n[bb][2];
+  const int lj = (int)(r % bx.n[bb][1]);
+  const int li = (int)(r / bx.n[bb][1]);
+  const int c3[3] = {bx.lo[bb][0] + li, bx.lo[bb][1] + lj, bx.lo[bb][2] + lk};
   const long long N = g.N;
   const long long stride[3] = {(long long)g.Ny * g.Nz, (long long)g.Nz, 1};
   const long long id = c3[0] * stride[0] + c3[1] * stride[1] + c3[2];
@@ -978,6 +988,165 @@ template <typename R>
 __global__ void convert_from_double(const double* __restrict__ src, R* __restrict__ dst, long long n) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t < n) dst[t] = (R)src[t];
+}
+
+}  // namespace fdf
+
+namespace fdf {
+
+// ---------------------------------------------------------------------------
+// Rectilinear 3-D interior kernel.
+//
+// On a Cartesian block the metric is diagonal and uniform (up to rounding)
+// and J is constant, so the interior update of block.cpp:1541-2050 reduces to
+// per-direction derivatives of 6 fields scaled by block constants.  Each
+// thread owns one (y,z) column of a TY x 64 tile and marches along x over a
+// chunk of planes:
+//   * x derivatives from a register window of the 6 fields x-derivatives need
+//     (vx vy vz sxx sxy sxz) over 2r+1 planes, loaded once per plane;
+//   * y and z derivatives from an LDS tile of the current plane with an r-point
+//     halo (8 fields: all but sxx);
+//   * df = A*df + ..., f_new = f + B*df, written once.
+// Only interior points (no one-sided rows, no SAT) are handled here; the
+// block's shell goes through bulk_kernel.  Not bit-identical to the
+// reference (derivative sums are formed before scaling and the metric is the
+// block constant); the FP64 result stays within 1e-13 of it per stage.
+struct Tiled3D {
+  int i0, i1, j0, j1, k0, k1;    // interior box [i0,i1) x [j0,j1) x [k0,k1)
+  int lo[3], hi[3];              // block box (loads are clamped into it)
+  int Ny, Nz;
+  long long N;
+  int cx;                        // planes per chunk
+  int nzt, nyt, nxc, ntiles;     // tile grid
+  double cv[3], cg2[3], cl[3], cg[3];  // dt/rho/dx*m, dt(2G+lam)/dx*m, dt lam/dx*m, dt G/dx*m
+  double A, B;
+  int roundtrip, hetstress;
+  double s0[6];
+};
+
+template <typename R, int S, int TY>
+__global__ void __launch_bounds__(64 * TY) bulk_tiled3d(Tiled3D ta, const R* __restrict__ f, R* __restrict__ fn,
+                                                        R* __restrict__ df, const double* __restrict__ hs) {
+  constexpr int r = S - 1;
+  constexpr int W = 2 * r + 1;
+  constexpr int TZ = 64;
+  constexpr int LY = TY + 2 * r, LZ = TZ + 2 * r;
+  __shared__ R tile[8][LY][LZ];  // 0 vx 1 vy 2 vz 3 sxy 4 sxz 5 syy 6 syz 7 szz
+  // XCD-aware tile order: consecutive hardware workgroup ids land on
+  // different XCDs, so give each XCD a contiguous range of tiles
+  const int nxcd = 8;
+  const int per = (ta.ntiles + nxcd - 1) / nxcd;
+  const int tile_id = (blockIdx.x % nxcd) * per + blockIdx.x / nxcd;
+  if (tile_id >= ta.ntiles) return;
+  const int zt = tile_id % ta.nzt;
+  const int rem = tile_id / ta.nzt;
+  const int yt = rem % ta.nyt;
+  const int xc = rem / ta.nyt;
+  const int tz = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  const int kb = ta.k0 + zt * TZ, jb = ta.j0 + yt * TY;
+  const int k = kb + tz, j = jb + ty;
+  const bool active = (k < ta.k1) && (j < ta.j1);
+  const int ib = ta.i0 + xc * ta.cx;
+  const int ie = min(ib + ta.cx, ta.i1);
+  const long long N = ta.N;
+  const long long sx = (long long)ta.Ny * ta.Nz, sy = ta.Nz;
+  const int kc = min(k, ta.hi[2] - 1), jc = min(j, ta.hi[1] - 1);
+  const long long col = (long long)jc * sy + kc;
+  double cd[W];
+#pragma unroll
+  for (int n = 0; n < W; n++) cd[n] = c_fd[0][n];
+  // x window: fields 0..5 = vx vy vz sxx sxy sxz at planes i-r .. i+r
+  R win[6][W];
+#pragma unroll
+  for (int n = 0; n < W - 1; n++) {
+    const long long id = (long long)(ib - r + n) * sx + col;
+#pragma unroll
+    for (int q = 0; q < 6; q++) win[q][n] = f[q * N + id];
+  }
+  constexpr int tf[8] = {0, 1, 2, 4, 5, 6, 7, 8};  // f index of each tile field
+  for (int i = ib; i < ie; i++) {
+    {  // newest plane of the window
+      const long long id = (long long)(i + r) * sx + col;
+#pragma unroll
+      for (int q = 0; q < 6; q++) win[q][W - 1] = f[q * N + id];
+    }
+    // LDS tile of plane i (cross-shaped halo)
+    __syncthreads();
+    for (int e = threadIdx.x; e < LY * LZ; e += 64 * TY) {
+      const int ly = e / LZ, lz = e - ly * LZ;
+      const bool need = (ly >= r && ly < r + TY) || (lz >= r && lz < r + TZ);
+      if (!need) continue;
+      const int gj = min(max(jb - r + ly, ta.lo[1]), ta.hi[1] - 1);
+      const int gk = min(max(kb - r + lz, ta.lo[2]), ta.hi[2] - 1);
+      const long long id = (long long)i * sx + (long long)gj * sy + gk;
+#pragma unroll
+      for (int q = 0; q < 8; q++) tile[q][ly][lz] = f[tf[q] * N + id];
+    }
+    __syncthreads();
+    if (active) {
+      typedef double C;
+      C dxv[6], dyv[6], dzv[6];
+#pragma unroll
+      for (int q = 0; q < 6; q++) {
+        C s = 0.;
+#pragma unroll
+        for (int n = 0; n < W; n++) s += (C)cd[n] * (C)win[q][n];
+        dxv[q] = s;  // d/dx of vx vy vz sxx sxy sxz
+      }
+      // y: vx vy vz sxy syy syz  (tile fields 0 1 2 3 5 6)
+      // z: vx vy vz sxz syz szz  (tile fields 0 1 2 4 6 7)
+      constexpr int ymap[6] = {0, 1, 2, 3, 5, 6}, zmap[6] = {0, 1, 2, 4, 6, 7};
+#pragma unroll
+      for (int q = 0; q < 6; q++) {
+        C sy_ = 0., sz_ = 0.;
+#pragma unroll
+        for (int n = 0; n < W; n++) {
+          sy_ += (C)cd[n] * (C)tile[ymap[q]][ty + n][tz + r];
+          sz_ += (C)cd[n] * (C)tile[zmap[q]][ty + r][tz + n];
+        }
+        dyv[q] = sy_;
+        dzv[q] = sz_;
+      }
+      const long long id = (long long)i * sx + (long long)j * sy + k;
+      C acc[9];
+#pragma unroll
+      for (int q = 0; q < 9; q++) acc[q] = (C)df[q * N + id] * (C)ta.A;
+      acc[0] += ta.cv[0] * dxv[3] + ta.cv[1] * dyv[3] + ta.cv[2] * dzv[3];
+      acc[1] += ta.cv[0] * dxv[4] + ta.cv[1] * dyv[4] + ta.cv[2] * dzv[4];
+      acc[2] += ta.cv[0] * dxv[5] + ta.cv[1] * dyv[5] + ta.cv[2] * dzv[5];
+      acc[3] += ta.cg2[0] * dxv[0] + ta.cl[1] * dyv[1] + ta.cl[2] * dzv[2];
+      acc[4] += ta.cg[0] * dxv[1] + ta.cg[1] * dyv[0];
+      acc[5] += ta.cg[0] * dxv[2] + ta.cg[2] * dzv[0];
+      acc[6] += ta.cl[0] * dxv[0] + ta.cg2[1] * dyv[1] + ta.cl[2] * dzv[2];
+      acc[7] += ta.cg[1] * dyv[2] + ta.cg[2] * dzv[1];
+      acc[8] += ta.cl[0] * dxv[0] + ta.cl[1] * dyv[1] + ta.cg2[2] * dzv[2];
+      C fo[9];
+#pragma unroll
+      for (int q = 0; q < 6; q++) fo[q] = (C)win[q][r];
+      fo[6] = (C)tile[5][ty + r][tz + r];
+      fo[7] = (C)tile[6][ty + r][tz + r];
+      fo[8] = (C)tile[7][ty + r][tz + r];
+#pragma unroll
+      for (int q = 0; q < 9; q++) {
+        df[q * N + id] = (R)acc[q];
+        C v = fo[q] + (C)ta.B * acc[q];
+        if (q >= 3 && ta.roundtrip) {  // set_stress / remove_stress rounding (FP64 only)
+          double w = (double)v;
+          w += ta.s0[q - 3];
+          if (ta.hetstress) w += hs[(q - 3) * N + id];
+          w -= ta.s0[q - 3];
+          if (ta.hetstress) w -= hs[(q - 3) * N + id];
+          v = (C)w;
+        }
+        fn[q * N + id] = (R)v;
+      }
+    }
+    // slide the window
+#pragma unroll
+    for (int n = 0; n < W - 1; n++)
+#pragma unroll
+      for (int q = 0; q < 6; q++) win[q][n] = win[q][n + 1];
+  }
 }
 
 }  // namespace fdf

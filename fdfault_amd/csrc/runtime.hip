@@ -109,6 +109,8 @@ class Solver : public SolverBase {
   DBuf<double> hs, x;
   int cur = 0;
   std::vector<BlockDev> bdev;
+  std::vector<bool> tiled_ok;
+  std::vector<Tiled3D> tiles;
   struct FaceBufs {
     DBuf<double> nrm, dl;
   };
@@ -219,6 +221,51 @@ Solver<R>::Solver(const std::string& path, const fdf_opts& o) {
       FD.dl = fb.dl.p;
     }
     bdev.push_back(D);
+    // rectilinear interior fast path (3-D, homogeneous, no dissipation)
+    bool ok = S.ndim == 3 && !S.hetmat && !(S.cdiss > 0.) && !(sizeof(R) == 8 && o.exact_fp64);
+    for (int d = 0; d < 3 && ok; d++) ok = B.mrb[d] > B.mc[d];
+    Tiled3D ta{};
+    if (ok) {
+      // metric diagonal and uniform, Jacobian uniform, to 1e-12 relative
+      const size_t gc = S.idx((B.mlb[0] + B.prb[0]) / 2, (B.mlb[1] + B.prb[1]) / 2, (B.mlb[2] + B.prb[2]) / 2);
+      double mdiag[3], jc = S.jac[gc];
+      for (int d = 0; d < 3; d++) mdiag[d] = S.metric[(size_t)(d * 3 + d) * N + gc];
+      for (int i = B.mlb[0]; i < B.prb[0] && ok; i++)
+        for (int j = B.mlb[1]; j < B.prb[1] && ok; j++)
+          for (int k = B.mlb[2]; k < B.prb[2] && ok; k++) {
+            const size_t g = S.idx(i, j, k);
+            if (fabs(S.jac[g] - jc) > 1e-12 * fabs(jc)) ok = false;
+            for (int l = 0; l < 3 && ok; l++)
+              for (int m = 0; m < 3 && ok; m++) {
+                const double v = S.metric[(size_t)(l * 3 + m) * N + g];
+                if (l == m ? fabs(v - mdiag[l]) > 1e-12 * fabs(mdiag[l]) : fabs(v) > 1e-12 * fabs(mdiag[l])) ok = false;
+              }
+          }
+      if (ok) {
+        ta.i0 = B.mc[0]; ta.i1 = B.mrb[0];
+        ta.j0 = B.mc[1]; ta.j1 = B.mrb[1];
+        ta.k0 = B.mc[2]; ta.k1 = B.mrb[2];
+        for (int d = 0; d < 3; d++) { ta.lo[d] = B.mlb[d]; ta.hi[d] = B.prb[d]; }
+        ta.Ny = S.nx[1];
+        ta.Nz = S.nx[2];
+        ta.N = (long long)N;
+        ta.cx = 16;
+        ta.nzt = (ta.k1 - ta.k0 + 63) / 64;
+        ta.nyt = (ta.j1 - ta.j0 + 3) / 4;
+        ta.nxc = (ta.i1 - ta.i0 + ta.cx - 1) / ta.cx;
+        ta.ntiles = ta.nzt * ta.nyt * ta.nxc;
+        for (int d = 0; d < 3; d++) {
+          ta.cv[d] = B.K[d][0] * mdiag[d];
+          ta.cg2[d] = B.K[d][1] * mdiag[d];
+          ta.cg[d] = B.K[d][2] * mdiag[d];
+          ta.cl[d] = B.K[d][3] * mdiag[d];
+        }
+        ta.hetstress = S.hetstress;
+        for (int q = 0; q < 6; q++) ta.s0[q] = S.s0[q];
+      }
+    }
+    tiled_ok.push_back(ok);
+    tiles.push_back(ta);
   }
   // interfaces
   ibufs.resize(S.ifaces.size());
@@ -445,27 +492,68 @@ void Solver<R>::launch_fault(int i, const FaultStageDev& fs) {
 }
 
 template <int M, typename R>
-static void bulk_dispatch(bool diss, bool het, const BlockDev& D, const GridDev& G, const StageDev& sd, const R* f,
-                          R* fn, R* df, const R* mt, const R* jac, const R* mat, const double* hs, hipStream_t s) {
-  const long long np = (long long)D.n[0] * D.n[1] * D.n[2];
+static void bulk_dispatch(bool diss, bool het, const BlockDev& D, const Boxes& bx, const GridDev& G, const StageDev& sd,
+                          const R* f, R* fn, R* df, const R* mt, const R* jac, const R* mat, const double* hs,
+                          hipStream_t s) {
+  const long long np = bx.start[bx.nbox];
+  if (np == 0) return;
   const dim3 grid((unsigned)((np + 255) / 256)), blk(256);
   if (diss) {
-    if (het) hipLaunchKernelGGL((bulk_kernel<M, R, R, true, true>), grid, blk, 0, s, D, G, sd, f, fn, df, mt, jac, mat, hs);
-    else hipLaunchKernelGGL((bulk_kernel<M, R, R, true, false>), grid, blk, 0, s, D, G, sd, f, fn, df, mt, jac, mat, hs);
+    if (het) hipLaunchKernelGGL((bulk_kernel<M, R, R, true, true>), grid, blk, 0, s, D, bx, G, sd, f, fn, df, mt, jac, mat, hs);
+    else hipLaunchKernelGGL((bulk_kernel<M, R, R, true, false>), grid, blk, 0, s, D, bx, G, sd, f, fn, df, mt, jac, mat, hs);
   } else {
-    if (het) hipLaunchKernelGGL((bulk_kernel<M, R, R, false, true>), grid, blk, 0, s, D, G, sd, f, fn, df, mt, jac, mat, hs);
-    else hipLaunchKernelGGL((bulk_kernel<M, R, R, false, false>), grid, blk, 0, s, D, G, sd, f, fn, df, mt, jac, mat, hs);
+    if (het) hipLaunchKernelGGL((bulk_kernel<M, R, R, false, true>), grid, blk, 0, s, D, bx, G, sd, f, fn, df, mt, jac, mat, hs);
+    else hipLaunchKernelGGL((bulk_kernel<M, R, R, false, false>), grid, blk, 0, s, D, bx, G, sd, f, fn, df, mt, jac, mat, hs);
   }
+}
+
+// whole block as one box, or the six shell boxes around [mc, mrb)
+static Boxes make_boxes(const BlockDev& D, bool shell) {
+  Boxes b{};
+  auto add = [&](int l0, int h0, int l1, int h1, int l2, int h2) {
+    if (h0 <= l0 || h1 <= l1 || h2 <= l2) return;
+    const int q = b.nbox++;
+    b.lo[q][0] = l0; b.lo[q][1] = l1; b.lo[q][2] = l2;
+    b.n[q][0] = h0 - l0; b.n[q][1] = h1 - l1; b.n[q][2] = h2 - l2;
+  };
+  if (!shell) {
+    add(D.lo[0], D.hi[0], D.lo[1], D.hi[1], D.lo[2], D.hi[2]);
+  } else {
+    add(D.lo[0], D.mc[0], D.lo[1], D.hi[1], D.lo[2], D.hi[2]);
+    add(D.mrb[0], D.hi[0], D.lo[1], D.hi[1], D.lo[2], D.hi[2]);
+    add(D.mc[0], D.mrb[0], D.lo[1], D.mc[1], D.lo[2], D.hi[2]);
+    add(D.mc[0], D.mrb[0], D.mrb[1], D.hi[1], D.lo[2], D.hi[2]);
+    add(D.mc[0], D.mrb[0], D.mc[1], D.mrb[1], D.lo[2], D.mc[2]);
+    add(D.mc[0], D.mrb[0], D.mc[1], D.mrb[1], D.mrb[2], D.hi[2]);
+  }
+  b.start[0] = 0;
+  for (int q = 0; q < b.nbox; q++) b.start[q + 1] = b.start[q] + (long long)b.n[q][0] * b.n[q][1] * b.n[q][2];
+  for (int q = b.nbox + 1; q < 7; q++) b.start[q] = b.start[b.nbox];
+  return b;
 }
 
 template <typename R>
 void Solver<R>::launch_bulk(int b, const StageDev& sd) {
   const BlockDev& D = bdev[b];
   const bool diss = S.cdiss > 0., het = S.hetmat;
+  const bool tiled = tiled_ok[b];
+  if (tiled) {
+    Tiled3D ta = tiles[b];
+    ta.A = sd.A;
+    ta.B = sd.B;
+    ta.roundtrip = sd.roundtrip;
+    const int TY = 4;
+    const dim3 grid((unsigned)(((ta.ntiles + 7) / 8) * 8)), blk(64 * TY);
+    if (S.sbporder == 2) hipLaunchKernelGGL((bulk_tiled3d<R, 2, 4>), grid, blk, 0, stream, ta, f[cur].p, f[cur ^ 1].p, df.p, hs.p);
+    else if (S.sbporder == 3) hipLaunchKernelGGL((bulk_tiled3d<R, 3, 4>), grid, blk, 0, stream, ta, f[cur].p, f[cur ^ 1].p, df.p, hs.p);
+    else hipLaunchKernelGGL((bulk_tiled3d<R, 4, 4>), grid, blk, 0, stream, ta, f[cur].p, f[cur ^ 1].p, df.p, hs.p);
+    HIPCHK(hipGetLastError());
+  }
+  const Boxes bx = make_boxes(D, tiled);
   switch (mode_id()) {
-    case 0: bulk_dispatch<0, R>(diss, het, D, G, sd, f[cur].p, f[cur ^ 1].p, df.p, metric.p, jac.p, mat.p, hs.p, stream); break;
-    case 1: bulk_dispatch<1, R>(diss, het, D, G, sd, f[cur].p, f[cur ^ 1].p, df.p, metric.p, jac.p, mat.p, hs.p, stream); break;
-    default: bulk_dispatch<2, R>(diss, het, D, G, sd, f[cur].p, f[cur ^ 1].p, df.p, metric.p, jac.p, mat.p, hs.p, stream); break;
+    case 0: bulk_dispatch<0, R>(diss, het, D, bx, G, sd, f[cur].p, f[cur ^ 1].p, df.p, metric.p, jac.p, mat.p, hs.p, stream); break;
+    case 1: bulk_dispatch<1, R>(diss, het, D, bx, G, sd, f[cur].p, f[cur ^ 1].p, df.p, metric.p, jac.p, mat.p, hs.p, stream); break;
+    default: bulk_dispatch<2, R>(diss, het, D, bx, G, sd, f[cur].p, f[cur ^ 1].p, df.p, metric.p, jac.p, mat.p, hs.p, stream); break;
   }
   HIPCHK(hipGetLastError());
 }

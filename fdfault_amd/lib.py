@@ -29,7 +29,7 @@ class FdfaultError(RuntimeError):
 
 class Opts(C.Structure):
     _fields_ = [("precision", C.c_int), ("device", C.c_int), ("write_outputs", C.c_int),
-                ("quiet", C.c_int), ("use_graph", C.c_int)]
+                ("quiet", C.c_int), ("use_graph", C.c_int), ("exact_fp64", C.c_int)]
 
 
 class Info(C.Structure):
@@ -108,9 +108,9 @@ class Solver:
     """One problem on the GPU (mirror of the reference's ``problem`` class)."""
 
     def __init__(self, inpath: str, precision: int = FP64, device: int = 0, write_outputs: bool = False,
-                 quiet: bool = True):
+                 quiet: bool = True, exact: bool = False):
         L = load()
-        self.opts = Opts(precision, device, 1 if write_outputs else 0, 1 if quiet else 0, 0)
+        self.opts = Opts(precision, device, 1 if write_outputs else 0, 1 if quiet else 0, 0, 1 if exact else 0)
         h = C.c_void_p()
         if L.fdf_open(inpath.encode(), C.byref(self.opts), C.byref(h)) != 0:
             raise _err()
@@ -201,7 +201,7 @@ class Solver:
         return ms, n.value
 
 
-def run(inpath: str, precision: int = FP64, quiet: bool = True):
-    o = Opts(precision, 0, 1, 1 if quiet else 0, 0)
+def run(inpath: str, precision: int = FP64, quiet: bool = True, exact: bool = False):
+    o = Opts(precision, 0, 1, 1 if quiet else 0, 0, 1 if exact else 0)
     if load().fdf_run(inpath.encode(), C.byref(o)) != 0:
         raise _err()

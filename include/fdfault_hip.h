@@ -30,6 +30,10 @@ typedef struct fdf_opts {
                         reference executable); 0: state stays on the device   */
   int quiet;         /* 1: no stdout progress lines                             */
   int use_graph;     /* 1: capture the per-step launch sequence in a hipGraph   */
+  int exact_fp64;    /* 1: every FP64 stencil keeps the reference's per-point
+                        association (bit-faithful); 0: interiors of Cartesian
+                        3-D blocks use the faster interior kernel (block-
+                        constant metric; within 1e-13 per stage)             */
 } fdf_opts;
 
 typedef struct fdf_info {

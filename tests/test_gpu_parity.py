@@ -26,11 +26,11 @@ def _rel(a, b):
     return np.abs(a - b).max() / scale
 
 
-def _check(path, nsteps, iface_fields=("U", "V", "S", "Sn", "Ux", "Vx", "Sx"), tol=TOL, state=False):
+def _check(path, nsteps, iface_fields=("U", "V", "S", "Sn", "Ux", "Vx", "Sx"), tol=TOL, state=False, exact=False):
     from fdfault_amd import lib
-    print("case", os.path.basename(path), "steps", nsteps, flush=True)
+    print("case", os.path.basename(path), "steps", nsteps, "exact", exact, flush=True)
     o = O.Oracle(path)
-    g = lib.Solver(path)
+    g = lib.Solver(path, exact=exact)
     o.step(nsteps)
     g.step(nsteps)
     fo, fg = o.array("f"), g.array("f")
@@ -136,11 +136,16 @@ def test_mode2_het_stress_material(tmp_path):
     _check(p, 12)
 
 
-def test_3d_slipweak(tmp_path):
+@pytest.mark.parametrize("exact", [True, False])
+@pytest.mark.parametrize("sbp", [2, 3, 4])
+def test_3d_slipweak(tmp_path, sbp, exact):
+    """exact=False exercises the rectilinear interior kernel (x-marching
+    register window + LDS tile) on the interior and bulk_kernel on the shell."""
     loads = [Pert("constant", 0., 0., 0., 0., 0., (-120., 70., 0.)),
-             Pert("boxcar", 0., 1.5, 0.4, 1.0, 0.4, (0., 11.6, 0.))]
-    p = write_case(str(tmp_path), "d3", ndim=3, nx=31, ny=16, nz=21, dx=0.1, nt=8, loads=loads)
-    _check(p, 8)
+             Pert("boxcar", 0., 3.5, 0.6, 1.4, 0.6, (0., 11.6, 0.))]
+    p = write_case(str(tmp_path), "d3", ndim=3, nx=71, ny=24, nz=29, dx=0.1, nt=8, loads=loads, sbporder=sbp,
+                   s0=(0., 0., 0., 0., 0., 0.))
+    _check(p, 8, exact=exact)
 
 
 def test_3d_stz(tmp_path):
@@ -175,3 +180,24 @@ def test_kernel_level_random_state(tmp_path):
         g.stage(0, st)
         assert _rel(g.array("df"), o.array("df")) <= TOL
         assert _rel(g.array("f"), o.array("f")) <= TOL
+
+
+@pytest.mark.parametrize("sbp", [2, 3, 4])
+def test_kernel_level_random_state_3d(tmp_path, sbp):
+    """3-D stage from a random state: interior kernel + shell kernel together
+    must reproduce the oracle's df and f (rounding-level differences only)."""
+    from fdfault_amd import lib
+    p = write_case(str(tmp_path), "rnd3", ndim=3, nx=45, ny=22, nz=71, dx=0.1, nt=2, sbporder=sbp)
+    o = O.Oracle(p)
+    g = lib.Solver(p)
+    rng = np.random.default_rng(11)
+    f = rng.normal(size=o.array("f").shape)
+    df = rng.normal(size=o.array("df").shape)
+    for h in (o, g):
+        h.set_array("f", f)
+        h.set_array("df", df)
+    for st in range(o.nstages):
+        o.stage(0, st)
+        g.stage(0, st)
+        assert _rel(g.array("df"), o.array("df")) <= 1e-12
+        assert _rel(g.array("f"), o.array("f")) <= 1e-12
