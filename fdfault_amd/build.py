@@ -25,7 +25,7 @@ ARCH = os.environ.get("FDF_OFFLOAD_ARCH", "gfx950")
 HIP_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wno-unused-result", "-Wno-unused-command-line-argument"]
-HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall"]
+HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-fopenmp"]
 
 
 def _run(cmd):
@@ -40,7 +40,7 @@ def build(verbose: bool = True) -> str:
     runtime_o = os.path.join(OUT, "runtime.o")
     _run([HIPCC] + HIP_FLAGS + ["-c", os.path.join(SRC, "runtime.hip"), "-o", runtime_o])
     tmp = LIB + ".tmp"
-    _run([HIPCC] + ["--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp, runtime_o, setup_o])
+    _run([HIPCC] + ["--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp, runtime_o, setup_o, "-lgomp"])
     os.replace(tmp, LIB)
     _run(["g++", "-O2", "-std=c++17", os.path.join(SRC, "main.cpp"), "-o", EXE, "-L" + OUT, "-lfdfault_hip",
           "-Wl,-rpath,$ORIGIN"])

@@ -501,7 +501,27 @@ struct FaultStageDev {
   double A, B, dt, t;
   int no_sat;
   double front_t;  // < 0: no front update
+  // when step != null the times come from the device step counter (graph
+  // replay): t = step*dt + C*dt (domain.cpp:254), front time (step+1)*dt
+  const int* step;
+  double C;
+  int front;
 };
+
+__device__ __forceinline__ void stage_times(const FaultStageDev& fs, double& t, double& tfront) {
+  if (fs.step) {
+    const double ti = (double)(*fs.step) * fs.dt;
+    t = ti + fs.C * fs.dt;
+    tfront = fs.front ? (double)(*fs.step + 1) * fs.dt : -1.;
+  } else {
+    t = fs.t;
+    tfront = fs.front_t;
+  }
+}
+
+__global__ void advance_step(int* step) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *step += 1;
+}
 
 __device__ __forceinline__ double pert_val(const double* xy, const double* par, int k, int q, int n, int p, double t) {
   const double t0 = par[k * 10];
@@ -675,7 +695,8 @@ __global__ void __launch_bounds__(256) fault_kernel(FaultDev F, GridDev g, Fault
     const double v12 = r1.v2, v22 = r2.v2, v13 = r1.v3, v23 = r2.v3;
     const double z1 = zs1, z2 = zs2;
     const double eta = z1 * z2 / (z1 + z2);
-    const double t = fs.t;
+    double t, tfront;
+    stage_times(fs, t, tfront);
     const int n = F.n;
     if (F.load_file) {
       snc += F.lf[p];
@@ -816,8 +837,8 @@ __global__ void __launch_bounds__(256) fault_kernel(FaultDev F, GridDev g, Fault
         F.state[p] += fs.B * F.dstate[p];
       }
     }
-    if (F.front && fs.front_t >= 0. && F.front_field == 0 && F.front[p] < 0. && V >= F.front_value)
-      F.front[p] = fs.front_t;
+    if (F.front && tfront >= 0. && F.front_field == 0 && F.front[p] < 0. && V >= F.front_value)
+      F.front[p] = tfront;
   }
   if (fs.no_sat) return;
   // SAT terms for both sides (interface.cpp:802-962)

@@ -124,6 +124,13 @@ class Solver : public SolverBase {
   DBuf<int> err;
   DBuf<double> stage_buf;
   std::vector<OutFile> outs;
+  // graph replay state
+  void graph_step();
+  hipGraphExec_t graphs[2] = {nullptr, nullptr};
+  DBuf<int> dstep;
+  const int* graph_step_ptr = nullptr;
+  bool capturing = false, dstep_valid = false;
+  int graph_flip = 0;
 };
 
 template <typename R>
@@ -177,6 +184,8 @@ Solver<R>::Solver(const std::string& path, const fdf_opts& o) {
   }
   err.alloc(1);
   err.zero(stream);
+  dstep.alloc(1);
+  dstep.zero(stream);
   // blocks
   const int nd = S.ndim;
   fbufs.resize(S.blocks.size() * 6);
@@ -366,7 +375,7 @@ Solver<R>::Solver(const std::string& path, const fdf_opts& o) {
   HIPCHK(hipStreamSynchronize(stream));
   // t = 0: interface solve without SAT (domain.cpp:141-143) on f = 0
   {
-    FaultStageDev fs{0., 0., 0., 0., 1, -1.};
+    FaultStageDev fs{0., 0., 0., 0., 1, -1., nullptr, 0., 0};
     for (size_t i = 0; i < S.ifaces.size(); i++)
       if (S.ifaces[i].law != LAW_LOCKED) launch_fault((int)i, fs);
   }
@@ -457,6 +466,8 @@ Solver<R>::~Solver() {
   if (stream) (void)hipStreamSynchronize(stream);
   for (int e = 0; e < 2 * NEV; e++)
     if (evs[e]) (void)hipEventDestroy(evs[e]);
+  for (int q = 0; q < 2; q++)
+    if (graphs[q]) (void)hipGraphExecDestroy(graphs[q]);
   if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -578,10 +589,15 @@ void Solver<R>::do_stage(int istep, int st, bool full_step) {
   const double t = (double)istep * S.dt;
   const double tst = t + S.rk.C[st] * S.dt;
   const double tfront = (double)(istep + 1) * S.dt;
-  FaultStageDev fs{S.rk.A[st], S.rk.B[st], S.dt, tst, 0, (full_step && last && !S.plastic) ? tfront : -1.};
+  FaultStageDev fs{S.rk.A[st], S.rk.B[st], S.dt, tst, 0, (full_step && last && !S.plastic) ? tfront : -1.,
+                   capturing ? graph_step_ptr : nullptr, S.rk.C[st], (full_step && last && !S.plastic) ? 1 : 0};
   for (size_t i = 0; i < S.ifaces.size(); i++) launch_fault((int)i, fs);
   StageDev sd{S.rk.A[st], S.rk.B[st], S.dt, (full_step && last && !S.plastic && sizeof(R) == 8) ? 1 : 0};
   for (size_t b = 0; b < S.blocks.size(); b++) {
+    if (capturing) {
+      launch_bulk((int)b, sd);
+      continue;
+    }
     if (nev == NEV) flush_events();
     HIPCHK(hipEventRecord(evs[2 * nev], stream));
     launch_bulk((int)b, sd);
@@ -591,7 +607,8 @@ void Solver<R>::do_stage(int istep, int st, bool full_step) {
   cur ^= 1;
   if (last && S.plastic) {
     for (size_t b = 0; b < S.blocks.size(); b++) launch_plastic((int)b);
-    FaultStageDev fs2{0., 0., S.dt, tst, 1, full_step ? tfront : -1.};
+    FaultStageDev fs2{0., 0., S.dt, tst, 1, full_step ? tfront : -1., capturing ? graph_step_ptr : nullptr, S.rk.C[st],
+                      full_step ? 1 : 0};
     for (size_t i = 0; i < S.ifaces.size(); i++)
       if (S.ifaces[i].law != LAW_LOCKED) launch_fault((int)i, fs2);
   }
@@ -609,10 +626,44 @@ void Solver<R>::flush_events() {
   nev = 0;
 }
 
+// one time step captured as a hipGraph (one graph per f-buffer parity);
+// times come from the device step counter so the graph replays unchanged
+template <typename R>
+void Solver<R>::graph_step() {
+  const int par = cur;
+  if (!graphs[par]) {
+    HIPCHK(hipStreamSynchronize(stream));
+    graph_step_ptr = dstep.p;
+    capturing = true;
+    HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    for (int st = 0; st < S.rk.nstages; st++) do_stage(cur_step, st, true);
+    hipLaunchKernelGGL(advance_step, dim3(1), dim3(1), 0, stream, dstep.p);
+    hipGraph_t g;
+    HIPCHK(hipStreamEndCapture(stream, &g));
+    capturing = false;
+    HIPCHK(hipGraphInstantiate(&graphs[par], g, nullptr, nullptr, 0));
+    HIPCHK(hipGraphDestroy(g));
+    cur = par;  // capture did not execute; replay below flips the buffers
+    graph_flip = (S.rk.nstages & 1) ? 1 : 0;
+  }
+  if (!dstep_valid) {
+    HIPCHK(hipStreamSynchronize(stream));
+    HIPCHK(hipMemcpy(dstep.p, &cur_step, sizeof(int), hipMemcpyHostToDevice));
+    dstep_valid = true;
+  }
+  HIPCHK(hipGraphLaunch(graphs[par], stream));
+  cur ^= graph_flip;
+}
+
 template <typename R>
 void Solver<R>::step(int n) {
   for (int k = 0; k < n; k++) {
-    for (int st = 0; st < S.rk.nstages; st++) do_stage(cur_step, st, true);
+    if (opts.use_graph) {
+      graph_step();
+    } else {
+      dstep_valid = false;
+      for (int st = 0; st < S.rk.nstages; st++) do_stage(cur_step, st, true);
+    }
     cur_step++;
     if (opts.write_outputs) write_outputs(cur_step);
     if (!opts.quiet && S.ninfo > 0 && cur_step % S.ninfo == 0) {

@@ -291,6 +291,7 @@ void build_block_grid(SetupH& S, BlockH& B, const SurfView* F) {
   for (int i = 0; i < 3; i++) B.dx[i] = 1. / (double)(B.nx[i] - 1);
   const int ny = B.nx[1], nz = B.nx[2];
   for (int c = 0; c < S.ndim; c++)
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < B.nx[0]; i++)
       for (int j = 0; j < B.nx[1]; j++)
         for (int k = 0; k < B.nx[2]; k++) {
@@ -318,6 +319,7 @@ void build_block_grid(SetupH& S, BlockH& B, const SurfView* F) {
     for (size_t q = 0; q < nb; q++) J9[(size_t)(c * 3 + c) * nb + q] = 1.;
   const size_t gstride[3] = {(size_t)S.nx[1] * S.nx[2], (size_t)S.nx[2], 1};
   for (int d = 0; d < S.ndim; d++)
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < B.nx[0]; i++)
       for (int j = 0; j < B.nx[1]; j++)
         for (int k = 0; k < B.nx[2]; k++) {
@@ -341,6 +343,8 @@ void build_block_grid(SetupH& S, BlockH& B, const SurfView* F) {
             J9[(size_t)(c * 3 + d) * nb + lidx] = acc;
           }
         }
+  bool bad_jac = false;
+#pragma omp parallel for schedule(static) reduction(|| : bad_jac)
   for (int i = 0; i < B.nx[0]; i++)
     for (int j = 0; j < B.nx[1]; j++)
       for (int k = 0; k < B.nx[2]; k++) {
@@ -348,7 +352,7 @@ void build_block_grid(SetupH& S, BlockH& B, const SurfView* F) {
         auto X = [&](int c, int d) { return J9[(size_t)(c * 3 + d) * nb + lidx]; };
         const double jac = (X(0, 0) * (X(1, 1) * X(2, 2) - X(1, 2) * X(2, 1)) - X(1, 0) * (X(0, 1) * X(2, 2) - X(0, 2) * X(2, 1)) +
                             X(2, 0) * (X(0, 1) * X(1, 2) - X(0, 2) * X(1, 1)));
-        if (!(jac > 0.)) fail("non-positive Jacobian in block grid (grid is not smooth enough)");
+        if (!(jac > 0.)) bad_jac = true;
         const size_t g = S.idx(B.xm[0] + i, B.xm[1] + j, B.xm[2] + k);
         S.jac[g] = jac;
         for (int l = 0; l < S.ndim; l++)
@@ -356,6 +360,7 @@ void build_block_grid(SetupH& S, BlockH& B, const SurfView* F) {
             S.metric[(size_t)(l * S.ndim + m) * S.N + g] =
                 ((X((m + 1) % 3, (l + 1) % 3) * X((m + 2) % 3, (l + 2) % 3) - X((m + 1) % 3, (l + 2) % 3) * X((m + 2) % 3, (l + 1) % 3)) / jac);
       }
+  if (bad_jac) fail("non-positive Jacobian in block grid (grid is not smooth enough)");
 }
 
 // unit normal and SAT scale from one metric row at point g (boundary.cpp:206-236)
@@ -398,6 +403,7 @@ double min_dx_over_cs(const SetupH& S) {  // block.cpp:368-403, domain.cpp:199-2
   double best = 0.;
   for (const BlockH& B : S.blocks) {
     double worst = 0.;
+#pragma omp parallel for schedule(static) reduction(max : worst)
     for (int i = B.mlb[0]; i < B.prb[0]; i++)
       for (int j = B.mlb[1]; j < B.prb[1]; j++)
         for (int k = B.mlb[2]; k < B.prb[2]; k++) {

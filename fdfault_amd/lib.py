@@ -108,9 +108,10 @@ class Solver:
     """One problem on the GPU (mirror of the reference's ``problem`` class)."""
 
     def __init__(self, inpath: str, precision: int = FP64, device: int = 0, write_outputs: bool = False,
-                 quiet: bool = True, exact: bool = False):
+                 quiet: bool = True, exact: bool = False, graph: bool = False):
         L = load()
-        self.opts = Opts(precision, device, 1 if write_outputs else 0, 1 if quiet else 0, 0, 1 if exact else 0)
+        self.opts = Opts(precision, device, 1 if write_outputs else 0, 1 if quiet else 0, 1 if graph else 0,
+                         1 if exact else 0)
         h = C.c_void_p()
         if L.fdf_open(inpath.encode(), C.byref(self.opts), C.byref(h)) != 0:
             raise _err()

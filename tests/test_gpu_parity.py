@@ -201,3 +201,27 @@ def test_kernel_level_random_state_3d(tmp_path, sbp):
         g.stage(0, st)
         assert _rel(g.array("df"), o.array("df")) <= 1e-12
         assert _rel(g.array("f"), o.array("f")) <= 1e-12
+
+
+@pytest.mark.parametrize("plastic", [False, True])
+def test_graph_replay_matches_eager(tmp_path, plastic):
+    """hipGraph replay of whole steps (device step counter) gives the same
+    fields, slip and fronts as eager launches, over several replays with
+    host reads in between."""
+    from fdfault_amd import lib
+    kw = dict(nx=61, ny=31, dx=0.1, nt=12)
+    if plastic:
+        kw.update(material="plastic", plastic=dict(mu=0.68, c=0., beta=0.35, eta=0.05 * 32.04),
+                  s0=(-100., 52., 0., -120., 0., -55.), loads=NUC2)
+    p = write_case(str(tmp_path), "gr", **kw)
+    a = lib.Solver(p)
+    b = lib.Solver(p, graph=True)
+    for k in range(4):
+        a.step(3)
+        b.step(3)
+        assert np.array_equal(a.array("f"), b.array("f"))
+        for w in ("U", "V", "S", "Sn"):
+            assert np.array_equal(a.iface(0, w), b.iface(0, w))
+        assert np.array_equal(a.front(0), b.front(0))
+    a.close()
+    b.close()
