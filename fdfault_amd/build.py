@@ -44,6 +44,7 @@ def build(verbose: bool = True) -> str:
     os.replace(tmp, LIB)
     _run(["g++", "-O2", "-std=c++17", os.path.join(SRC, "main.cpp"), "-o", EXE, "-L" + OUT, "-lfdfault_hip",
           "-Wl,-rpath,$ORIGIN"])
+    _run([HIPCC] + HIP_FLAGS + [os.path.join(SRC, "copy_bw.hip"), "-o", os.path.join(OUT, "copy_bw")])
     return LIB
 
 

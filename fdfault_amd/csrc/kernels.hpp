@@ -1045,13 +1045,14 @@ struct Tiled3D {
   double s0[6];
 };
 
-template <typename R, int S, int TY>
+template <typename R, typename C, int S, int TY>
 __global__ void __launch_bounds__(64 * TY) bulk_tiled3d(Tiled3D ta, const R* __restrict__ f, R* __restrict__ fn,
                                                         R* __restrict__ df, const double* __restrict__ hs) {
   constexpr int r = S - 1;
   constexpr int W = 2 * r + 1;
   constexpr int TZ = 64;
   constexpr int LY = TY + 2 * r, LZ = TZ + 2 * r;
+  constexpr int NT = 64 * TY;
   __shared__ R tile[8][LY][LZ];  // 0 vx 1 vy 2 vz 3 sxy 4 sxz 5 syy 6 syz 7 szz
   // XCD-aware tile order: consecutive hardware workgroup ids land on
   // different XCDs, so give each XCD a contiguous range of tiles
@@ -1073,9 +1074,11 @@ __global__ void __launch_bounds__(64 * TY) bulk_tiled3d(Tiled3D ta, const R* __r
   const long long sx = (long long)ta.Ny * ta.Nz, sy = ta.Nz;
   const int kc = min(k, ta.hi[2] - 1), jc = min(j, ta.hi[1] - 1);
   const long long col = (long long)jc * sy + kc;
-  double cd[W];
+  C cd[W];
 #pragma unroll
-  for (int n = 0; n < W; n++) cd[n] = c_fd[0][n];
+  for (int n = 0; n < W; n++) cd[n] = (C)c_fd[0][n];
+  // halo entries of the LDS cross: z-halo (TY rows x 2r) then y-halo (2r rows x TZ)
+  constexpr int NZH = TY * 2 * r, NH = NZH + 2 * r * TZ;
   // x window: fields 0..5 = vx vy vz sxx sxy sxz at planes i-r .. i+r
   R win[6][W];
 #pragma unroll
@@ -1086,32 +1089,52 @@ __global__ void __launch_bounds__(64 * TY) bulk_tiled3d(Tiled3D ta, const R* __r
   }
   constexpr int tf[8] = {0, 1, 2, 4, 5, 6, 7, 8};  // f index of each tile field
   for (int i = ib; i < ie; i++) {
+    const long long pl = (long long)i * sx;
     {  // newest plane of the window
       const long long id = (long long)(i + r) * sx + col;
 #pragma unroll
       for (int q = 0; q < 6; q++) win[q][W - 1] = f[q * N + id];
     }
-    // LDS tile of plane i (cross-shaped halo)
+    R own[3];
+#pragma unroll
+    for (int q = 0; q < 3; q++) own[q] = f[(6 + q) * N + pl + col];  // syy syz szz
     __syncthreads();
-    for (int e = threadIdx.x; e < LY * LZ; e += 64 * TY) {
-      const int ly = e / LZ, lz = e - ly * LZ;
-      const bool need = (ly >= r && ly < r + TY) || (lz >= r && lz < r + TZ);
-      if (!need) continue;
+    // own points of the tile from registers
+    tile[0][ty + r][tz + r] = win[0][r];
+    tile[1][ty + r][tz + r] = win[1][r];
+    tile[2][ty + r][tz + r] = win[2][r];
+    tile[3][ty + r][tz + r] = win[4][r];
+    tile[4][ty + r][tz + r] = win[5][r];
+    tile[5][ty + r][tz + r] = own[0];
+    tile[6][ty + r][tz + r] = own[1];
+    tile[7][ty + r][tz + r] = own[2];
+    // halo ring
+    for (int e = threadIdx.x; e < NH; e += NT) {
+      int ly, lz;
+      if (e < NZH) {
+        ly = r + e / (2 * r);
+        const int h = e % (2 * r);
+        lz = (h < r) ? h : TZ + h;
+      } else {
+        const int e2 = e - NZH;
+        const int h = e2 / TZ;
+        ly = (h < r) ? h : TY + h;
+        lz = r + e2 % TZ;
+      }
       const int gj = min(max(jb - r + ly, ta.lo[1]), ta.hi[1] - 1);
       const int gk = min(max(kb - r + lz, ta.lo[2]), ta.hi[2] - 1);
-      const long long id = (long long)i * sx + (long long)gj * sy + gk;
+      const long long id = pl + (long long)gj * sy + gk;
 #pragma unroll
       for (int q = 0; q < 8; q++) tile[q][ly][lz] = f[tf[q] * N + id];
     }
     __syncthreads();
     if (active) {
-      typedef double C;
       C dxv[6], dyv[6], dzv[6];
 #pragma unroll
       for (int q = 0; q < 6; q++) {
         C s = 0.;
 #pragma unroll
-        for (int n = 0; n < W; n++) s += (C)cd[n] * (C)win[q][n];
+        for (int n = 0; n < W; n++) s += cd[n] * (C)win[q][n];
         dxv[q] = s;  // d/dx of vx vy vz sxx sxy sxz
       }
       // y: vx vy vz sxy syy syz  (tile fields 0 1 2 3 5 6)
@@ -1122,31 +1145,31 @@ __global__ void __launch_bounds__(64 * TY) bulk_tiled3d(Tiled3D ta, const R* __r
         C sy_ = 0., sz_ = 0.;
 #pragma unroll
         for (int n = 0; n < W; n++) {
-          sy_ += (C)cd[n] * (C)tile[ymap[q]][ty + n][tz + r];
-          sz_ += (C)cd[n] * (C)tile[zmap[q]][ty + r][tz + n];
+          sy_ += cd[n] * (C)tile[ymap[q]][ty + n][tz + r];
+          sz_ += cd[n] * (C)tile[zmap[q]][ty + r][tz + n];
         }
         dyv[q] = sy_;
         dzv[q] = sz_;
       }
-      const long long id = (long long)i * sx + (long long)j * sy + k;
+      const long long id = pl + (long long)j * sy + k;
       C acc[9];
 #pragma unroll
       for (int q = 0; q < 9; q++) acc[q] = (C)df[q * N + id] * (C)ta.A;
-      acc[0] += ta.cv[0] * dxv[3] + ta.cv[1] * dyv[3] + ta.cv[2] * dzv[3];
-      acc[1] += ta.cv[0] * dxv[4] + ta.cv[1] * dyv[4] + ta.cv[2] * dzv[4];
-      acc[2] += ta.cv[0] * dxv[5] + ta.cv[1] * dyv[5] + ta.cv[2] * dzv[5];
-      acc[3] += ta.cg2[0] * dxv[0] + ta.cl[1] * dyv[1] + ta.cl[2] * dzv[2];
-      acc[4] += ta.cg[0] * dxv[1] + ta.cg[1] * dyv[0];
-      acc[5] += ta.cg[0] * dxv[2] + ta.cg[2] * dzv[0];
-      acc[6] += ta.cl[0] * dxv[0] + ta.cg2[1] * dyv[1] + ta.cl[2] * dzv[2];
-      acc[7] += ta.cg[1] * dyv[2] + ta.cg[2] * dzv[1];
-      acc[8] += ta.cl[0] * dxv[0] + ta.cl[1] * dyv[1] + ta.cg2[2] * dzv[2];
+      acc[0] += (C)ta.cv[0] * dxv[3] + (C)ta.cv[1] * dyv[3] + (C)ta.cv[2] * dzv[3];
+      acc[1] += (C)ta.cv[0] * dxv[4] + (C)ta.cv[1] * dyv[4] + (C)ta.cv[2] * dzv[4];
+      acc[2] += (C)ta.cv[0] * dxv[5] + (C)ta.cv[1] * dyv[5] + (C)ta.cv[2] * dzv[5];
+      acc[3] += (C)ta.cg2[0] * dxv[0] + (C)ta.cl[1] * dyv[1] + (C)ta.cl[2] * dzv[2];
+      acc[4] += (C)ta.cg[0] * dxv[1] + (C)ta.cg[1] * dyv[0];
+      acc[5] += (C)ta.cg[0] * dxv[2] + (C)ta.cg[2] * dzv[0];
+      acc[6] += (C)ta.cl[0] * dxv[0] + (C)ta.cg2[1] * dyv[1] + (C)ta.cl[2] * dzv[2];
+      acc[7] += (C)ta.cg[1] * dyv[2] + (C)ta.cg[2] * dzv[1];
+      acc[8] += (C)ta.cl[0] * dxv[0] + (C)ta.cl[1] * dyv[1] + (C)ta.cg2[2] * dzv[2];
       C fo[9];
 #pragma unroll
       for (int q = 0; q < 6; q++) fo[q] = (C)win[q][r];
-      fo[6] = (C)tile[5][ty + r][tz + r];
-      fo[7] = (C)tile[6][ty + r][tz + r];
-      fo[8] = (C)tile[7][ty + r][tz + r];
+      fo[6] = (C)own[0];
+      fo[7] = (C)own[1];
+      fo[8] = (C)own[2];
 #pragma unroll
       for (int q = 0; q < 9; q++) {
         df[q * N + id] = (R)acc[q];

@@ -1,0 +1,55 @@
+"""The drop-in executable `fdfault_amd/_lib/fdfault <file.in>` against the
+oracle's whole-run equivalent: same set of output and front files, same
+metadata text, byte-identical data (FP64 path, 2-D mode II, field and
+interface units, strided boxes, fronts)."""
+import filecmp
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from cases import write_case
+from fdfault_amd.problems import Output
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_executable_files_match_oracle(tmp_path):
+    outs = [Output("vx", "vx", (0, 20, 5), (0, 60, 2), (0, 61, 3), (0, 0, 1)),
+            Output("sxy", "sxy", (0, 20, 1), (30, 30, 1), (20, 40, 1), (0, 0, 1)),
+            Output("V", "V", (0, 20, 2), (0, 60, 1), (31, 31, 1), (0, 0, 1)),
+            Output("U", "U", (0, 20, 20), (0, 60, 3), (30, 30, 1), (0, 0, 1)),
+            Output("Sn", "Sn", (0, 20, 4), (10, 50, 1), (31, 31, 1), (0, 0, 1)),
+            Output("Vx", "Vx", (5, 15, 5), (0, 60, 1), (31, 31, 1), (0, 0, 1))]
+    p = write_case(str(tmp_path), "run", nx=61, ny=31, dx=0.1, nt=20, outputs=outs, front=("V", 0.001))
+    d = os.path.dirname(p)
+    gdir = os.path.join(d, "data")
+    odir = os.path.join(d, "odata")
+    os.makedirs(odir)
+    txt = open(p).read().replace(gdir + "/", odir + "/")
+    po = os.path.join(d, "run_oracle.in")
+    open(po, "w").write(txt)
+    assert O.run(po) == 0
+    exe = os.path.join(ROOT, "fdfault_amd", "_lib", "fdfault")
+    r = subprocess.run([exe, p], capture_output=True, text=True, timeout=120)
+    print(r.stdout[-500:], r.stderr[-500:])
+    assert r.returncode == 0
+    assert "Problem finished" in r.stdout
+    gfiles, ofiles = sorted(os.listdir(gdir)), sorted(os.listdir(odir))
+    assert gfiles == ofiles and len(gfiles) >= 30
+    for fn in gfiles:
+        assert filecmp.cmp(os.path.join(gdir, fn), os.path.join(odir, fn), shallow=False), fn
+
+
+def test_executable_error_message(tmp_path):
+    bad = tmp_path / "bad.in"
+    bad.write_text("[fdfault.problem]\nx\ndata/\n1\n0\n0\n0.3\n1\n7\n")
+    exe = os.path.join(ROOT, "fdfault_amd", "_lib", "fdfault")
+    r = subprocess.run([exe, str(bad)], capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0
+    assert "Error in initializing RK method" in r.stderr
