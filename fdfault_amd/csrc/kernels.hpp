@@ -333,7 +333,12 @@ __global__ void __launch_bounds__(256) bulk_kernel(BlockDev bd, Boxes bx, GridDe
                                                    const double* __restrict__ hs) {
   constexpr int NFE = Nf<M>::v;
   constexpr int ND = (M == 2) ? 3 : 2;
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  // XCD-aware order: hardware deals workgroups round-robin over the 8 XCDs;
+  // give each XCD a contiguous 1/8 of the points so x-neighbour planes are
+  // reused from that XCD's L2 (grid is a multiple of 8 workgroups)
+  const unsigned per = gridDim.x >> 3;
+  const unsigned wg = (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
+  const long long t = (long long)wg * blockDim.x + threadIdx.x;
   if (t >= bx.start[bx.nbox]) return;
   int bb = 0;
   while (t >= bx.start[bb + 1]) bb++;

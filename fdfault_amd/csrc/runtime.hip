@@ -264,7 +264,8 @@ Solver<R>::Solver(const std::string& path, const fdf_opts& o) {
         ta.Ny = S.nx[1];
         ta.Nz = S.nx[2];
         ta.N = (long long)N;
-        ta.cx = 16;
+        ta.cx = 48;
+        if (const char* e = getenv("FDF_TILE_CX")) ta.cx = atoi(e) > 0 ? atoi(e) : 48;
         ta.nzt = (ta.k1 - ta.k0 + 63) / 64;
         ta.nyt = (ta.j1 - ta.j0 + 3) / 4;
         ta.nxc = (ta.i1 - ta.i0 + ta.cx - 1) / ta.cx;
@@ -514,7 +515,7 @@ static void bulk_dispatch(bool diss, bool het, const BlockDev& D, const Boxes& b
                           hipStream_t s) {
   const long long np = bx.start[bx.nbox];
   if (np == 0) return;
-  const dim3 grid((unsigned)((np + 255) / 256)), blk(256);
+  const dim3 grid((unsigned)((((np + 255) / 256) + 7) / 8 * 8)), blk(256);
   if (diss) {
     if (het) hipLaunchKernelGGL((bulk_kernel<M, R, R, true, true>), grid, blk, 0, s, D, bx, G, sd, f, fn, df, mt, jac, mat, hs);
     else hipLaunchKernelGGL((bulk_kernel<M, R, R, true, false>), grid, blk, 0, s, D, bx, G, sd, f, fn, df, mt, jac, mat, hs);
