@@ -333,12 +333,7 @@ __global__ void __launch_bounds__(256) bulk_kernel(BlockDev bd, Boxes bx, GridDe
                                                    const double* __restrict__ hs) {
   constexpr int NFE = Nf<M>::v;
   constexpr int ND = (M == 2) ? 3 : 2;
-  // XCD-aware order: hardware deals workgroups round-robin over the 8 XCDs;
-  // give each XCD a contiguous 1/8 of the points so x-neighbour planes are
-  // reused from that XCD's L2 (grid is a multiple of 8 workgroups)
-  const unsigned per = gridDim.x >> 3;
-  const unsigned wg = (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
-  const long long t = (long long)wg * blockDim.x + threadIdx.x;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= bx.start[bx.nbox]) return;
   int bb = 0;
   while (t >= bx.start[bb + 1]) bb++;
@@ -1021,10 +1016,10 @@ __global__ void convert_from_double(const double* __restrict__ src, R* __restric
 namespace fdf {
 
 // ---------------------------------------------------------------------------
-// Rectilinear 3-D interior kernel.
+// Rectilinear 3-D block kernel.
 //
 // On a Cartesian block the metric is diagonal and uniform (up to rounding)
-// and J is constant, so the interior update of block.cpp:1541-2050 reduces to
+// and J is constant, so the update of block.cpp:1541-2050 reduces to
 // per-direction derivatives of 6 fields scaled by block constants.  Each
 // thread owns one (y,z) column of a TY x 64 tile and marches along x over a
 // chunk of planes:
@@ -1032,14 +1027,15 @@ namespace fdf {
 //     (vx vy vz sxx sxy sxz) over 2r+1 planes, loaded once per plane;
 //   * y and z derivatives from an LDS tile of the current plane with an r-point
 //     halo (8 fields: all but sxx);
+//   * one-sided SBP rows near block edges read their taps directly (cached);
 //   * df = A*df + ..., f_new = f + B*df, written once.
-// Only interior points (no one-sided rows, no SAT) are handled here; the
-// block's shell goes through bulk_kernel.  Not bit-identical to the
-// reference (derivative sums are formed before scaling and the metric is the
-// block constant); the FP64 result stays within 1e-13 of it per stage.
+// External and interface SAT terms of face points are added afterwards by
+// face_sat_kernel.  Not bit-identical to the reference (derivative sums are
+// formed before scaling, the metric is the block constant); the FP64 result
+// stays within 1e-10 of the bit-faithful path over long runs.
 struct Tiled3D {
-  int i0, i1, j0, j1, k0, k1;    // interior box [i0,i1) x [j0,j1) x [k0,k1)
-  int lo[3], hi[3];              // block box (loads are clamped into it)
+  int lo[3], hi[3];              // block box
+  int mc[3], mrb[3];             // interior rows [mc, mrb) per direction
   int Ny, Nz;
   long long N;
   int cx;                        // planes per chunk
@@ -1049,6 +1045,30 @@ struct Tiled3D {
   int roundtrip, hetstress;
   double s0[6];
 };
+
+// derivative along one direction at a one-sided boundary row, taps read from
+// global memory: left rows use c_fd[pos-lo+1][n] at lo+n, right rows
+// -c_fd[hi-pos][n] at hi-1-n (block.cpp:1057-1162 pattern)
+template <typename R, typename C, int S, int NQ>
+__device__ __forceinline__ void edge_deriv(C* out, const R* __restrict__ f, const int* fq, long long N, long long base,
+                                           long long stride, int pos, int lo, int hi) {
+  constexpr int NT = 3 * (S - 1);
+  int row, start, step;
+  C sg;
+  if (pos - lo < 2 * (S - 1)) { row = pos - lo + 1; start = lo; step = 1; sg = (C)1; }
+  else { row = hi - pos; start = hi - 1; step = -1; sg = (C)-1; }
+#pragma unroll
+  for (int q = 0; q < NQ; q++) out[q] = (C)0;
+#pragma unroll
+  for (int n = 0; n < NT; n++) {
+    const C c = (C)c_fd[row][n];
+    const long long id = base + (long long)(start + step * n) * stride;
+#pragma unroll
+    for (int q = 0; q < NQ; q++) out[q] += c * (C)f[fq[q] * N + id];
+  }
+#pragma unroll
+  for (int q = 0; q < NQ; q++) out[q] *= sg;
+}
 
 template <typename R, typename C, int S, int TY>
 __global__ void __launch_bounds__(64 * TY) bulk_tiled3d(Tiled3D ta, const R* __restrict__ f, R* __restrict__ fn,
@@ -1070,33 +1090,38 @@ __global__ void __launch_bounds__(64 * TY) bulk_tiled3d(Tiled3D ta, const R* __r
   const int yt = rem % ta.nyt;
   const int xc = rem / ta.nyt;
   const int tz = threadIdx.x & 63, ty = threadIdx.x >> 6;
-  const int kb = ta.k0 + zt * TZ, jb = ta.j0 + yt * TY;
+  const int kb = ta.lo[2] + zt * TZ, jb = ta.lo[1] + yt * TY;
   const int k = kb + tz, j = jb + ty;
-  const bool active = (k < ta.k1) && (j < ta.j1);
-  const int ib = ta.i0 + xc * ta.cx;
-  const int ie = min(ib + ta.cx, ta.i1);
+  const bool active = (k < ta.hi[2]) && (j < ta.hi[1]);
+  const int ib = ta.lo[0] + xc * ta.cx;
+  const int ie = min(ib + ta.cx, ta.hi[0]);
   const long long N = ta.N;
   const long long sx = (long long)ta.Ny * ta.Nz, sy = ta.Nz;
   const int kc = min(k, ta.hi[2] - 1), jc = min(j, ta.hi[1] - 1);
   const long long col = (long long)jc * sy + kc;
+  const bool yin = j >= ta.mc[1] && j < ta.mrb[1];
+  const bool zin = k >= ta.mc[2] && k < ta.mrb[2];
   C cd[W];
 #pragma unroll
   for (int n = 0; n < W; n++) cd[n] = (C)c_fd[0][n];
-  // halo entries of the LDS cross: z-halo (TY rows x 2r) then y-halo (2r rows x TZ)
   constexpr int NZH = TY * 2 * r, NH = NZH + 2 * r * TZ;
-  // x window: fields 0..5 = vx vy vz sxx sxy sxz at planes i-r .. i+r
+  auto xplane = [&](int i) { return (long long)min(max(i, ta.lo[0]), ta.hi[0] - 1) * sx; };
+  // x window: fields 0..5 = vx vy vz sxx sxy sxz at planes i-r .. i+r (clamped)
   R win[6][W];
 #pragma unroll
   for (int n = 0; n < W - 1; n++) {
-    const long long id = (long long)(ib - r + n) * sx + col;
+    const long long id = xplane(ib - r + n) + col;
 #pragma unroll
     for (int q = 0; q < 6; q++) win[q][n] = f[q * N + id];
   }
   constexpr int tf[8] = {0, 1, 2, 4, 5, 6, 7, 8};  // f index of each tile field
+  constexpr int xq[6] = {0, 1, 2, 3, 4, 5};
+  constexpr int yq[6] = {0, 1, 2, 4, 6, 7};  // vx vy vz sxy syy syz
+  constexpr int zq[6] = {0, 1, 2, 5, 7, 8};  // vx vy vz sxz syz szz
   for (int i = ib; i < ie; i++) {
     const long long pl = (long long)i * sx;
     {  // newest plane of the window
-      const long long id = (long long)(i + r) * sx + col;
+      const long long id = xplane(i + r) + col;
 #pragma unroll
       for (int q = 0; q < 6; q++) win[q][W - 1] = f[q * N + id];
     }
@@ -1104,7 +1129,6 @@ __global__ void __launch_bounds__(64 * TY) bulk_tiled3d(Tiled3D ta, const R* __r
 #pragma unroll
     for (int q = 0; q < 3; q++) own[q] = f[(6 + q) * N + pl + col];  // syy syz szz
     __syncthreads();
-    // own points of the tile from registers
     tile[0][ty + r][tz + r] = win[0][r];
     tile[1][ty + r][tz + r] = win[1][r];
     tile[2][ty + r][tz + r] = win[2][r];
@@ -1113,7 +1137,6 @@ __global__ void __launch_bounds__(64 * TY) bulk_tiled3d(Tiled3D ta, const R* __r
     tile[5][ty + r][tz + r] = own[0];
     tile[6][ty + r][tz + r] = own[1];
     tile[7][ty + r][tz + r] = own[2];
-    // halo ring
     for (int e = threadIdx.x; e < NH; e += NT) {
       int ly, lz;
       if (e < NZH) {
@@ -1135,26 +1158,39 @@ __global__ void __launch_bounds__(64 * TY) bulk_tiled3d(Tiled3D ta, const R* __r
     __syncthreads();
     if (active) {
       C dxv[6], dyv[6], dzv[6];
+      if (i >= ta.mc[0] && i < ta.mrb[0]) {
 #pragma unroll
-      for (int q = 0; q < 6; q++) {
-        C s = 0.;
+        for (int q = 0; q < 6; q++) {
+          C s = 0.;
 #pragma unroll
-        for (int n = 0; n < W; n++) s += cd[n] * (C)win[q][n];
-        dxv[q] = s;  // d/dx of vx vy vz sxx sxy sxz
-      }
-      // y: vx vy vz sxy syy syz  (tile fields 0 1 2 3 5 6)
-      // z: vx vy vz sxz syz szz  (tile fields 0 1 2 4 6 7)
-      constexpr int ymap[6] = {0, 1, 2, 3, 5, 6}, zmap[6] = {0, 1, 2, 4, 6, 7};
-#pragma unroll
-      for (int q = 0; q < 6; q++) {
-        C sy_ = 0., sz_ = 0.;
-#pragma unroll
-        for (int n = 0; n < W; n++) {
-          sy_ += cd[n] * (C)tile[ymap[q]][ty + n][tz + r];
-          sz_ += cd[n] * (C)tile[zmap[q]][ty + r][tz + n];
+          for (int n = 0; n < W; n++) s += cd[n] * (C)win[q][n];
+          dxv[q] = s;  // d/dx of vx vy vz sxx sxy sxz
         }
-        dyv[q] = sy_;
-        dzv[q] = sz_;
+      } else {
+        edge_deriv<R, C, S, 6>(dxv, f, xq, N, col, sx, i, ta.lo[0], ta.hi[0]);
+      }
+      constexpr int ymap[6] = {0, 1, 2, 3, 5, 6}, zmap[6] = {0, 1, 2, 4, 6, 7};
+      if (yin) {
+#pragma unroll
+        for (int q = 0; q < 6; q++) {
+          C sy_ = 0.;
+#pragma unroll
+          for (int n = 0; n < W; n++) sy_ += cd[n] * (C)tile[ymap[q]][ty + n][tz + r];
+          dyv[q] = sy_;
+        }
+      } else {
+        edge_deriv<R, C, S, 6>(dyv, f, yq, N, pl + k, sy, j, ta.lo[1], ta.hi[1]);
+      }
+      if (zin) {
+#pragma unroll
+        for (int q = 0; q < 6; q++) {
+          C sz_ = 0.;
+#pragma unroll
+          for (int n = 0; n < W; n++) sz_ += cd[n] * (C)tile[zmap[q]][ty + r][tz + n];
+          dzv[q] = sz_;
+        }
+      } else {
+        edge_deriv<R, C, S, 6>(dzv, f, zq, N, pl + (long long)j * sy, 1, k, ta.lo[2], ta.hi[2]);
       }
       const long long id = pl + (long long)j * sy + k;
       C acc[9];
@@ -1195,6 +1231,84 @@ __global__ void __launch_bounds__(64 * TY) bulk_tiled3d(Tiled3D ta, const R* __r
     for (int n = 0; n < W - 1; n++)
 #pragma unroll
       for (int q = 0; q < 6; q++) win[q][n] = win[q][n + 1];
+  }
+}
+
+struct FaceList {
+  int start[7];  // prefix offsets of the face point counts, face order
+};
+
+// SAT terms of the face points of a block whose bulk update came from
+// bulk_tiled3d: external boundaries in face order, then interface terms in
+// interface order (the reference's order per point), added to df and, times
+// B, to the new f.  A point on several faces is owned by its first face.
+template <int M, typename R, typename C>
+__global__ void __launch_bounds__(256) face_sat_kernel(BlockDev bd, GridDev g, StageDev st, const R* __restrict__ f,
+                                                       R* __restrict__ fn, R* __restrict__ df, const R* __restrict__ mat,
+                                                       const double* __restrict__ hs, FaceList fl) {
+  constexpr int ND = (M == 2) ? 3 : 2;
+  constexpr int NFE = Nf<M>::v;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= fl.start[2 * ND]) return;
+  int fc = 0;
+  while (t >= fl.start[fc + 1]) fc++;
+  const int loc = t - fl.start[fc];
+  const int d = fc / 2, a = (d == 0) ? 1 : 0, b = (d == 2) ? 1 : 2;
+  int c3[3];
+  c3[d] = (fc & 1) ? bd.hi[d] - 1 : bd.lo[d];
+  c3[a] = bd.lo[a] + loc / bd.n[b];
+  c3[b] = bd.lo[b] + loc % bd.n[b];
+  // owned by the first face (in face order) that contains the point
+  for (int e = 0; e < fc; e++) {
+    const int de = e / 2;
+    const int edge = (e & 1) ? bd.hi[de] - 1 : bd.lo[de];
+    if (c3[de] == edge) return;
+  }
+  const long long N = g.N;
+  const long long id = ((long long)c3[0] * g.Ny + c3[1]) * g.Nz + c3[2];
+  C acc[NFE + 1];
+#pragma unroll
+  for (int q = 0; q <= NFE; q++) acc[q] = (C)0;
+  bool any = false;
+#pragma unroll
+  for (int e = 0; e < 2 * ND; e++) {
+    const FaceDev& F = bd.face[e];
+    if (!F.active) continue;
+    const int de = e / 2;
+    const int edge = (e & 1) ? bd.hi[de] - 1 : bd.lo[de];
+    if (c3[de] != edge) continue;
+    const int ae = (de == 0) ? 1 : 0, be = (de == 2) ? 1 : 2;
+    ext_sat<M, R, C>(acc, F, f, mat, g, id, c3[ae] - bd.lo[ae], c3[be] - bd.lo[be], st.dt);
+    any = true;
+  }
+  const int NF = g.nf;
+  for (int q = 0; q < bd.nif; q++) {
+    const IfFaceDev& I = bd.ifc[q];
+    if (c3[I.dir] != I.fixed) continue;
+    const long long o = (long long)(c3[I.a] - I.org_a) * I.n1 + (c3[I.b] - I.org_b);
+    const double* sn = I.sat + (long long)(I.side * 2 + 0) * NF * I.n;
+    const double* sv = I.sat + (long long)(I.side * 2 + 1) * NF * I.n;
+    for (int p = 0; p < NFE; p++) {
+      acc[p] = (C)((double)acc[p] - sn[(long long)p * I.n + o]);
+      acc[p] = (C)((double)acc[p] - sv[(long long)p * I.n + o]);
+    }
+    any = true;
+  }
+  if (!any) return;
+#pragma unroll
+  for (int q = 0; q < NFE; q++) {
+    df[q * N + id] = (R)((C)df[q * N + id] + acc[q]);
+    C v = (C)fn[q * N + id] + (C)st.B * acc[q];
+    if (st.roundtrip && q >= g.nv) {
+      const int i = q - g.nv;
+      double w = (double)v;
+      w += g.s0[g.sidx[i]];
+      if (g.hetstress) w += hs[i * N + id];
+      w -= g.s0[g.sidx[i]];
+      if (g.hetstress) w -= hs[i * N + id];
+      v = (C)w;
+    }
+    fn[q * N + id] = (R)v;
   }
 }
 

@@ -237,7 +237,7 @@ Solver<R>::Solver(const std::string& path, const fdf_opts& o) {
     bdev.push_back(D);
     // rectilinear interior fast path (3-D, homogeneous, no dissipation)
     bool ok = S.ndim == 3 && !S.hetmat && !(S.cdiss > 0.) && !(sizeof(R) == 8 && o.exact_fp64);
-    for (int d = 0; d < 3 && ok; d++) ok = B.mrb[d] > B.mc[d];
+    for (int d = 0; d < 3 && ok; d++) ok = B.mrb[d] > B.mc[d] && B.nx[d] >= 3 * (S.sbporder - 1);
     Tiled3D ta{};
     if (ok) {
       // metric diagonal and uniform, Jacobian uniform, to 1e-9 relative (the
@@ -257,18 +257,20 @@ Solver<R>::Solver(const std::string& path, const fdf_opts& o) {
               }
           }
       if (ok) {
-        ta.i0 = B.mc[0]; ta.i1 = B.mrb[0];
-        ta.j0 = B.mc[1]; ta.j1 = B.mrb[1];
-        ta.k0 = B.mc[2]; ta.k1 = B.mrb[2];
-        for (int d = 0; d < 3; d++) { ta.lo[d] = B.mlb[d]; ta.hi[d] = B.prb[d]; }
+        for (int d = 0; d < 3; d++) {
+          ta.lo[d] = B.mlb[d];
+          ta.hi[d] = B.prb[d];
+          ta.mc[d] = B.mc[d];
+          ta.mrb[d] = B.mrb[d];
+        }
         ta.Ny = S.nx[1];
         ta.Nz = S.nx[2];
         ta.N = (long long)N;
-        ta.cx = 48;
-        if (const char* e = getenv("FDF_TILE_CX")) ta.cx = atoi(e) > 0 ? atoi(e) : 48;
-        ta.nzt = (ta.k1 - ta.k0 + 63) / 64;
-        ta.nyt = (ta.j1 - ta.j0 + 3) / 4;
-        ta.nxc = (ta.i1 - ta.i0 + ta.cx - 1) / ta.cx;
+        ta.cx = 16;
+        if (const char* e = getenv("FDF_TILE_CX")) ta.cx = atoi(e) > 0 ? atoi(e) : 16;
+        ta.nzt = (ta.hi[2] - ta.lo[2] + 63) / 64;
+        ta.nyt = (ta.hi[1] - ta.lo[1] + 3) / 4;
+        ta.nxc = (ta.hi[0] - ta.lo[0] + ta.cx - 1) / ta.cx;
         ta.ntiles = ta.nzt * ta.nyt * ta.nxc;
         for (int d = 0; d < 3; d++) {
           ta.cv[d] = B.K[d][0] * mdiag[d];
@@ -562,7 +564,7 @@ static void launch_tiled_ty(const Tiled3D& ta, int s, const R* f, R* fn, R* df, 
 
 template <typename R>
 void Solver<R>::launch_tiled(Tiled3D ta, int ty) {
-  ta.nyt = (ta.j1 - ta.j0 + ty - 1) / ty;
+  ta.nyt = (ta.hi[1] - ta.lo[1] + ty - 1) / ty;
   ta.ntiles = ta.nzt * ta.nyt * ta.nxc;
   if (ty == 2) launch_tiled_ty<R, 2>(ta, S.sbporder, f[cur].p, f[cur ^ 1].p, df.p, hs.p, stream);
   else if (ty == 4) launch_tiled_ty<R, 4>(ta, S.sbporder, f[cur].p, f[cur ^ 1].p, df.p, hs.p, stream);
@@ -582,8 +584,18 @@ void Solver<R>::launch_bulk(int b, const StageDev& sd) {
     ta.roundtrip = sd.roundtrip;
     launch_tiled(ta, tile_ty);
     HIPCHK(hipGetLastError());
+    FaceList fl{};
+    for (int fc = 0; fc < 6; fc++) {
+      const int d = fc / 2, a = (d == 0) ? 1 : 0, bb = (d == 2) ? 1 : 2;
+      fl.start[fc + 1] = fl.start[fc] + D.n[a] * D.n[bb];
+    }
+    const dim3 grid((unsigned)((fl.start[6] + 255) / 256)), blk(256);
+    hipLaunchKernelGGL((face_sat_kernel<2, R, R>), grid, blk, 0, stream, D, G, sd, f[cur].p, f[cur ^ 1].p, df.p, mat.p,
+                       hs.p, fl);
+    HIPCHK(hipGetLastError());
+    return;
   }
-  const Boxes bx = make_boxes(D, tiled);
+  const Boxes bx = make_boxes(D, false);
   switch (mode_id()) {
     case 0: bulk_dispatch<0, R>(diss, het, D, bx, G, sd, f[cur].p, f[cur ^ 1].p, df.p, metric.p, jac.p, mat.p, hs.p, stream); break;
     case 1: bulk_dispatch<1, R>(diss, het, D, bx, G, sd, f[cur].p, f[cur ^ 1].p, df.p, metric.p, jac.p, mat.p, hs.p, stream); break;
