@@ -1044,6 +1044,10 @@ struct Tiled3D {
   double A, B;
   int roundtrip, hetstress;
   double s0[6];
+  int face_active[6];            // external SAT faces (matrix in facemat)
+  int nif;                       // interface faces touching the block
+  int if_dir[6], if_fixed[6], if_a[6], if_b[6], if_orga[6], if_orgb[6], if_n[6], if_n1[6], if_side[6];
+  const double* if_sat[6];
 };
 
 // derivative along one direction at a one-sided boundary row, taps read from
@@ -1072,7 +1076,8 @@ __device__ __forceinline__ void edge_deriv(C* out, const R* __restrict__ f, cons
 
 template <typename R, typename C, int S, int TY>
 __global__ void __launch_bounds__(64 * TY) bulk_tiled3d(Tiled3D ta, const R* __restrict__ f, R* __restrict__ fn,
-                                                        R* __restrict__ df, const double* __restrict__ hs) {
+                                                        R* __restrict__ df, const double* __restrict__ hs,
+                                                        const double* __restrict__ facemat) {
   constexpr int r = S - 1;
   constexpr int W = 2 * r + 1;
   constexpr int TZ = 64;
@@ -1118,16 +1123,35 @@ __global__ void __launch_bounds__(64 * TY) bulk_tiled3d(Tiled3D ta, const R* __r
   constexpr int xq[6] = {0, 1, 2, 3, 4, 5};
   constexpr int yq[6] = {0, 1, 2, 4, 6, 7};  // vx vy vz sxy syy syz
   constexpr int zq[6] = {0, 1, 2, 5, 7, 8};  // vx vy vz sxz syz szz
+  // prefetch of the first plane: newest window entry, own syy syz szz, df
+  R nwin[6], nown[3], ndf[9];
+  {
+    const long long id = xplane(ib + r) + col, pl0 = (long long)ib * sx + col;
+#pragma unroll
+    for (int q = 0; q < 6; q++) nwin[q] = f[q * N + id];
+#pragma unroll
+    for (int q = 0; q < 3; q++) nown[q] = f[(6 + q) * N + pl0];
+#pragma unroll
+    for (int q = 0; q < 9; q++) ndf[q] = df[q * N + pl0];
+  }
   for (int i = ib; i < ie; i++) {
     const long long pl = (long long)i * sx;
-    {  // newest plane of the window
-      const long long id = xplane(i + r) + col;
 #pragma unroll
-      for (int q = 0; q < 6; q++) win[q][W - 1] = f[q * N + id];
+    for (int q = 0; q < 6; q++) win[q][W - 1] = nwin[q];
+    R own[3], odf[9];
+#pragma unroll
+    for (int q = 0; q < 3; q++) own[q] = nown[q];
+#pragma unroll
+    for (int q = 0; q < 9; q++) odf[q] = ndf[q];
+    if (i + 1 < ie) {  // issue the next plane's loads before this plane's work
+      const long long id = xplane(i + 1 + r) + col, pl1 = pl + sx + col;
+#pragma unroll
+      for (int q = 0; q < 6; q++) nwin[q] = f[q * N + id];
+#pragma unroll
+      for (int q = 0; q < 3; q++) nown[q] = f[(6 + q) * N + pl1];
+#pragma unroll
+      for (int q = 0; q < 9; q++) ndf[q] = df[q * N + pl1];
     }
-    R own[3];
-#pragma unroll
-    for (int q = 0; q < 3; q++) own[q] = f[(6 + q) * N + pl + col];  // syy syz szz
     __syncthreads();
     tile[0][ty + r][tz + r] = win[0][r];
     tile[1][ty + r][tz + r] = win[1][r];
@@ -1195,7 +1219,7 @@ __global__ void __launch_bounds__(64 * TY) bulk_tiled3d(Tiled3D ta, const R* __r
       const long long id = pl + (long long)j * sy + k;
       C acc[9];
 #pragma unroll
-      for (int q = 0; q < 9; q++) acc[q] = (C)df[q * N + id] * (C)ta.A;
+      for (int q = 0; q < 9; q++) acc[q] = (C)odf[q] * (C)ta.A;
       acc[0] += (C)ta.cv[0] * dxv[3] + (C)ta.cv[1] * dyv[3] + (C)ta.cv[2] * dzv[3];
       acc[1] += (C)ta.cv[0] * dxv[4] + (C)ta.cv[1] * dyv[4] + (C)ta.cv[2] * dzv[4];
       acc[2] += (C)ta.cv[0] * dxv[5] + (C)ta.cv[1] * dyv[5] + (C)ta.cv[2] * dzv[5];
@@ -1211,6 +1235,33 @@ __global__ void __launch_bounds__(64 * TY) bulk_tiled3d(Tiled3D ta, const R* __r
       fo[6] = (C)own[0];
       fo[7] = (C)own[1];
       fo[8] = (C)own[2];
+      // external SAT: on a Cartesian face with homogeneous material the
+      // boundary.cpp:265-492 penalty is a fixed linear map of the point's fields
+      const int c3[3] = {i, j, k};
+#pragma unroll
+      for (int e = 0; e < 6; e++) {
+        if (!ta.face_active[e]) continue;
+        const int de = e >> 1;
+        if (c3[de] != ((e & 1) ? ta.hi[de] - 1 : ta.lo[de])) continue;
+        const double* Mx = facemat + e * 81;
+#pragma unroll
+        for (int q = 0; q < 9; q++) {
+          C sq = 0.;
+#pragma unroll
+          for (int p = 0; p < 9; p++) sq += (C)Mx[q * 9 + p] * fo[p];
+          acc[q] += sq;
+        }
+      }
+      // interface SAT terms written by fault_kernel (interface.cpp:826-962)
+      for (int e = 0; e < ta.nif; e++) {
+        if (c3[ta.if_dir[e]] != ta.if_fixed[e]) continue;
+        const long long o = (long long)(c3[ta.if_a[e]] - ta.if_orga[e]) * ta.if_n1[e] + (c3[ta.if_b[e]] - ta.if_orgb[e]);
+        const long long n = ta.if_n[e];
+        const double* sn = ta.if_sat[e] + (long long)(ta.if_side[e] * 2 + 0) * 9 * n;
+        const double* sv = ta.if_sat[e] + (long long)(ta.if_side[e] * 2 + 1) * 9 * n;
+#pragma unroll
+        for (int q = 0; q < 9; q++) acc[q] -= (C)(sn[q * n + o] + sv[q * n + o]);
+      }
 #pragma unroll
       for (int q = 0; q < 9; q++) {
         df[q * N + id] = (R)acc[q];
@@ -1237,6 +1288,21 @@ __global__ void __launch_bounds__(64 * TY) bulk_tiled3d(Tiled3D ta, const R* __r
 struct FaceList {
   int start[7];  // prefix offsets of the face point counts, face order
 };
+
+// Linear map of the external SAT on one face point: column q is ext_sat
+// applied to the unit state e_q (9 fields), so M . f reproduces the penalty
+// added to df for state f at that point.  Run once at set-up.
+__global__ void face_matrix_kernel(FaceDev F, GridDev g, double dt, int u, int v, const double* __restrict__ unit,
+                                   double* __restrict__ out) {
+  const int q = threadIdx.x;
+  if (q >= 9) return;
+  GridDev gg = g;
+  gg.N = 9;
+  gg.hetmat = 0;
+  double acc[10] = {0., 0., 0., 0., 0., 0., 0., 0., 0., 0.};
+  ext_sat<2, double, double>(acc, F, unit, nullptr, gg, q, u, v, dt);
+  for (int p = 0; p < 9; p++) out[p * 9 + q] = acc[p];
+}
 
 // SAT terms of the face points of a block whose bulk update came from
 // bulk_tiled3d: external boundaries in face order, then interface terms in

@@ -94,7 +94,8 @@ class Solver : public SolverBase {
   void do_stage(int istep, int st, bool full_step);
   void launch_fault(int i, const FaultStageDev& fs);
   void launch_bulk(int b, const StageDev& sd);
-  void launch_tiled(Tiled3D ta, int ty);
+  void launch_tiled(Tiled3D ta, int ty, const double* fm);
+  std::vector<DBuf<double>> facemats;  // per block [6][9][9] external SAT maps
   int tile_ty = 4;
   void launch_plastic(int b);
   void write_outputs(int tstep);
@@ -142,7 +143,7 @@ Solver<R>::Solver(const std::string& path, const fdf_opts& o) {
   compile_problem(path, S, o.write_outputs != 0, o.quiet != 0);
   HIPCHK(hipSetDevice(o.device));
   if (const char* e = getenv("FDF_TILE_TY")) tile_ty = atoi(e);
-  if (tile_ty != 2 && tile_ty != 4 && tile_ty != 8 && tile_ty != 16) tile_ty = 4;
+  if (tile_ty != 2 && tile_ty != 4 && tile_ty != 8) tile_ty = 4;
   HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
   for (int e = 0; e < 2 * NEV; e++) HIPCHK(hipEventCreate(&evs[e]));
   const size_t N = S.N;
@@ -280,10 +281,12 @@ Solver<R>::Solver(const std::string& path, const fdf_opts& o) {
         }
         ta.hetstress = S.hetstress;
         for (int q = 0; q < 6; q++) ta.s0[q] = S.s0[q];
+        for (int e = 0; e < 6; e++) ta.face_active[e] = D.face[e].active;
       }
     }
     tiled_ok.push_back(ok);
     tiles.push_back(ta);
+    facemats.emplace_back();
   }
   // interfaces
   ibufs.resize(S.ifaces.size());
@@ -380,6 +383,41 @@ Solver<R>::Solver(const std::string& path, const fdf_opts& o) {
       X.n1 = F.n1;
       X.sat = ib.sat.p;
     }
+  }
+  // rectilinear blocks: interface faces and external SAT maps for the tiled kernel
+  {
+    std::vector<double> unit(81, 0.);
+    for (int q = 0; q < 9; q++) unit[q * 9 + q] = 1.;  // field q of point q
+    DBuf<double> dunit;
+    dunit.upload(unit);
+    for (size_t b = 0; b < bdev.size(); b++) {
+      if (!tiled_ok[b]) continue;
+      const BlockDev& D = bdev[b];
+      Tiled3D& ta = tiles[b];
+      ta.nif = D.nif;
+      for (int e = 0; e < D.nif; e++) {
+        const IfFaceDev& X = D.ifc[e];
+        ta.if_dir[e] = X.dir;
+        ta.if_fixed[e] = X.fixed;
+        ta.if_a[e] = X.a;
+        ta.if_b[e] = X.b;
+        ta.if_orga[e] = X.org_a;
+        ta.if_orgb[e] = X.org_b;
+        ta.if_n[e] = X.n;
+        ta.if_n1[e] = X.n1;
+        ta.if_side[e] = X.side;
+        ta.if_sat[e] = X.sat;
+      }
+      facemats[b].alloc(6 * 81);
+      facemats[b].zero(stream);
+      for (int e = 0; e < 6; e++) {
+        if (!D.face[e].active) continue;
+        hipLaunchKernelGGL(face_matrix_kernel, dim3(1), dim3(64), 0, stream, D.face[e], G, S.dt, D.face[e].n0 / 2,
+                           D.face[e].n1 / 2, dunit.p, facemats[b].p + e * 81);
+        HIPCHK(hipGetLastError());
+      }
+    }
+    HIPCHK(hipStreamSynchronize(stream));
   }
   HIPCHK(hipStreamSynchronize(stream));
   // t = 0: interface solve without SAT (domain.cpp:141-143) on f = 0
@@ -555,21 +593,21 @@ static Boxes make_boxes(const BlockDev& D, bool shell) {
 // interior kernel launch; the compute type is FP64 for FP64 storage and
 // FP32 for FP32 storage
 template <typename R, int TY>
-static void launch_tiled_ty(const Tiled3D& ta, int s, const R* f, R* fn, R* df, const double* hs, hipStream_t st) {
+static void launch_tiled_ty(const Tiled3D& ta, int s, const R* f, R* fn, R* df, const double* hs, const double* fm,
+                            hipStream_t st) {
   const dim3 grid((unsigned)(((ta.ntiles + 7) / 8) * 8)), blk(64 * TY);
-  if (s == 2) hipLaunchKernelGGL((bulk_tiled3d<R, R, 2, TY>), grid, blk, 0, st, ta, f, fn, df, hs);
-  else if (s == 3) hipLaunchKernelGGL((bulk_tiled3d<R, R, 3, TY>), grid, blk, 0, st, ta, f, fn, df, hs);
-  else hipLaunchKernelGGL((bulk_tiled3d<R, R, 4, TY>), grid, blk, 0, st, ta, f, fn, df, hs);
+  if (s == 2) hipLaunchKernelGGL((bulk_tiled3d<R, R, 2, TY>), grid, blk, 0, st, ta, f, fn, df, hs, fm);
+  else if (s == 3) hipLaunchKernelGGL((bulk_tiled3d<R, R, 3, TY>), grid, blk, 0, st, ta, f, fn, df, hs, fm);
+  else hipLaunchKernelGGL((bulk_tiled3d<R, R, 4, TY>), grid, blk, 0, st, ta, f, fn, df, hs, fm);
 }
 
 template <typename R>
-void Solver<R>::launch_tiled(Tiled3D ta, int ty) {
+void Solver<R>::launch_tiled(Tiled3D ta, int ty, const double* fm) {
   ta.nyt = (ta.hi[1] - ta.lo[1] + ty - 1) / ty;
   ta.ntiles = ta.nzt * ta.nyt * ta.nxc;
-  if (ty == 2) launch_tiled_ty<R, 2>(ta, S.sbporder, f[cur].p, f[cur ^ 1].p, df.p, hs.p, stream);
-  else if (ty == 4) launch_tiled_ty<R, 4>(ta, S.sbporder, f[cur].p, f[cur ^ 1].p, df.p, hs.p, stream);
-  else if (ty == 8) launch_tiled_ty<R, 8>(ta, S.sbporder, f[cur].p, f[cur ^ 1].p, df.p, hs.p, stream);
-  else launch_tiled_ty<R, 16>(ta, S.sbporder, f[cur].p, f[cur ^ 1].p, df.p, hs.p, stream);
+  if (ty == 2) launch_tiled_ty<R, 2>(ta, S.sbporder, f[cur].p, f[cur ^ 1].p, df.p, hs.p, fm, stream);
+  else if (ty == 4) launch_tiled_ty<R, 4>(ta, S.sbporder, f[cur].p, f[cur ^ 1].p, df.p, hs.p, fm, stream);
+  else launch_tiled_ty<R, 8>(ta, S.sbporder, f[cur].p, f[cur ^ 1].p, df.p, hs.p, fm, stream);
 }
 
 template <typename R>
@@ -582,16 +620,7 @@ void Solver<R>::launch_bulk(int b, const StageDev& sd) {
     ta.A = sd.A;
     ta.B = sd.B;
     ta.roundtrip = sd.roundtrip;
-    launch_tiled(ta, tile_ty);
-    HIPCHK(hipGetLastError());
-    FaceList fl{};
-    for (int fc = 0; fc < 6; fc++) {
-      const int d = fc / 2, a = (d == 0) ? 1 : 0, bb = (d == 2) ? 1 : 2;
-      fl.start[fc + 1] = fl.start[fc] + D.n[a] * D.n[bb];
-    }
-    const dim3 grid((unsigned)((fl.start[6] + 255) / 256)), blk(256);
-    hipLaunchKernelGGL((face_sat_kernel<2, R, R>), grid, blk, 0, stream, D, G, sd, f[cur].p, f[cur ^ 1].p, df.p, mat.p,
-                       hs.p, fl);
+    launch_tiled(ta, tile_ty, facemats[b].p);
     HIPCHK(hipGetLastError());
     return;
   }
