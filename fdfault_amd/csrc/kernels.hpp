@@ -1033,6 +1033,14 @@ namespace fdf {
 // face_sat_kernel.  Not bit-identical to the reference (derivative sums are
 // formed before scaling, the metric is the block constant); the FP64 result
 // stays within 1e-10 of the bit-faithful path over long runs.
+// Workgroup barrier that orders LDS only: __syncthreads() also waits for every
+// outstanding global load, which would stall the next-plane prefetch.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 struct Tiled3D {
   int lo[3], hi[3];              // block box
   int mc[3], mrb[3];             // interior rows [mc, mrb) per direction
@@ -1152,7 +1160,7 @@ __global__ void __launch_bounds__(64 * TY) bulk_tiled3d(Tiled3D ta, const R* __r
 #pragma unroll
       for (int q = 0; q < 9; q++) ndf[q] = df[q * N + pl1];
     }
-    __syncthreads();
+    lds_barrier();
     tile[0][ty + r][tz + r] = win[0][r];
     tile[1][ty + r][tz + r] = win[1][r];
     tile[2][ty + r][tz + r] = win[2][r];
@@ -1179,7 +1187,7 @@ __global__ void __launch_bounds__(64 * TY) bulk_tiled3d(Tiled3D ta, const R* __r
 #pragma unroll
       for (int q = 0; q < 8; q++) tile[q][ly][lz] = f[tf[q] * N + id];
     }
-    __syncthreads();
+    lds_barrier();
     if (active) {
       C dxv[6], dyv[6], dzv[6];
       if (i >= ta.mc[0] && i < ta.mrb[0]) {
