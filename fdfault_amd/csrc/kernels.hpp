@@ -1131,17 +1131,48 @@ __global__ void __launch_bounds__(64 * TY) bulk_tiled3d(Tiled3D ta, const R* __r
   constexpr int xq[6] = {0, 1, 2, 3, 4, 5};
   constexpr int yq[6] = {0, 1, 2, 4, 6, 7};  // vx vy vz sxy syy syz
   constexpr int zq[6] = {0, 1, 2, 5, 7, 8};  // vx vy vz sxz syz szz
-  // prefetch of the first plane: newest window entry, own syy syz szz, df
-  R nwin[6], nown[3], ndf[9];
-  {
-    const long long id = xplane(ib + r) + col, pl0 = (long long)ib * sx + col;
+  // halo entries this thread loads each plane (cross-shaped ring of the tile)
+  constexpr int NHT = (NH + NT - 1) / NT;
+  int hly[NHT], hlz[NHT];
+  long long hoff[NHT];
+#pragma unroll
+  for (int u = 0; u < NHT; u++) {
+    const int e = threadIdx.x + u * NT;
+    int ly = 0, lz = 0;
+    if (e < NZH) {
+      ly = r + e / (2 * r);
+      const int h = e % (2 * r);
+      lz = (h < r) ? h : TZ + h;
+    } else if (e < NH) {
+      const int e2 = e - NZH;
+      const int h = e2 / TZ;
+      ly = (h < r) ? h : TY + h;
+      lz = r + e2 % TZ;
+    }
+    hly[u] = (e < NH) ? ly : -1;
+    hlz[u] = lz;
+    const int gj = min(max(jb - r + ly, ta.lo[1]), ta.hi[1] - 1);
+    const int gk = min(max(kb - r + lz, ta.lo[2]), ta.hi[2] - 1);
+    hoff[u] = (long long)gj * sy + gk;
+  }
+  // all loads of one plane are issued together, one plane ahead of its use
+  R nwin[6], nown[3], ndf[9], nhal[NHT][8];
+  auto issue = [&](int ip) {
+    const long long id = xplane(ip + r) + col, pc = (long long)ip * sx;
 #pragma unroll
     for (int q = 0; q < 6; q++) nwin[q] = f[q * N + id];
 #pragma unroll
-    for (int q = 0; q < 3; q++) nown[q] = f[(6 + q) * N + pl0];
+    for (int q = 0; q < 3; q++) nown[q] = f[(6 + q) * N + pc + col];
 #pragma unroll
-    for (int q = 0; q < 9; q++) ndf[q] = df[q * N + pl0];
-  }
+    for (int q = 0; q < 9; q++) ndf[q] = df[q * N + pc + col];
+#pragma unroll
+    for (int u = 0; u < NHT; u++)
+      if (hly[u] >= 0) {
+#pragma unroll
+        for (int q = 0; q < 8; q++) nhal[u][q] = f[tf[q] * N + pc + hoff[u]];
+      }
+  };
+  issue(ib);
   for (int i = ib; i < ie; i++) {
     const long long pl = (long long)i * sx;
 #pragma unroll
@@ -1151,16 +1182,6 @@ __global__ void __launch_bounds__(64 * TY) bulk_tiled3d(Tiled3D ta, const R* __r
     for (int q = 0; q < 3; q++) own[q] = nown[q];
 #pragma unroll
     for (int q = 0; q < 9; q++) odf[q] = ndf[q];
-    if (i + 1 < ie) {  // issue the next plane's loads before this plane's work
-      const long long id = xplane(i + 1 + r) + col, pl1 = pl + sx + col;
-#pragma unroll
-      for (int q = 0; q < 6; q++) nwin[q] = f[q * N + id];
-#pragma unroll
-      for (int q = 0; q < 3; q++) nown[q] = f[(6 + q) * N + pl1];
-#pragma unroll
-      for (int q = 0; q < 9; q++) ndf[q] = df[q * N + pl1];
-    }
-    lds_barrier();
     tile[0][ty + r][tz + r] = win[0][r];
     tile[1][ty + r][tz + r] = win[1][r];
     tile[2][ty + r][tz + r] = win[2][r];
@@ -1169,25 +1190,14 @@ __global__ void __launch_bounds__(64 * TY) bulk_tiled3d(Tiled3D ta, const R* __r
     tile[5][ty + r][tz + r] = own[0];
     tile[6][ty + r][tz + r] = own[1];
     tile[7][ty + r][tz + r] = own[2];
-    for (int e = threadIdx.x; e < NH; e += NT) {
-      int ly, lz;
-      if (e < NZH) {
-        ly = r + e / (2 * r);
-        const int h = e % (2 * r);
-        lz = (h < r) ? h : TZ + h;
-      } else {
-        const int e2 = e - NZH;
-        const int h = e2 / TZ;
-        ly = (h < r) ? h : TY + h;
-        lz = r + e2 % TZ;
-      }
-      const int gj = min(max(jb - r + ly, ta.lo[1]), ta.hi[1] - 1);
-      const int gk = min(max(kb - r + lz, ta.lo[2]), ta.hi[2] - 1);
-      const long long id = pl + (long long)gj * sy + gk;
 #pragma unroll
-      for (int q = 0; q < 8; q++) tile[q][ly][lz] = f[tf[q] * N + id];
-    }
+    for (int u = 0; u < NHT; u++)
+      if (hly[u] >= 0) {
+#pragma unroll
+        for (int q = 0; q < 8; q++) tile[q][hly[u]][hlz[u]] = nhal[u][q];
+      }
     lds_barrier();
+    if (i + 1 < ie) issue(i + 1);
     if (active) {
       C dxv[6], dyv[6], dzv[6];
       if (i >= ta.mc[0] && i < ta.mrb[0]) {
@@ -1285,6 +1295,7 @@ __global__ void __launch_bounds__(64 * TY) bulk_tiled3d(Tiled3D ta, const R* __r
         fn[q * N + id] = (R)v;
       }
     }
+    lds_barrier();  // every wave is done reading this plane's tile
     // slide the window
 #pragma unroll
     for (int n = 0; n < W - 1; n++)
