@@ -14,7 +14,62 @@ __global__ void copy_kernel(const double4* __restrict__ a, double4* __restrict__
   for (; i < n; i += stride) b[i] = a[i];
 }
 
+// SoA stream: `nin` input and `nout` output arrays of n floats (the bulk
+// stage's access shape: f and df in, df and f_new out), out_q = sum of inputs
+template <int NIN, int NOUT>
+__global__ void soa_stream(const float* const* __restrict__ in, float* const* __restrict__ out, long long n) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) {
+    float s = 0.f;
+#pragma unroll
+    for (int q = 0; q < NIN; q++) s += in[q][i];
+#pragma unroll
+    for (int q = 0; q < NOUT; q++) out[q][i] = s + (float)q;
+  }
+}
+
+template <int NIN, int NOUT>
+static void soa_run(long long n, int reps) {
+  float** hin = new float*[NIN];
+  float** hout = new float*[NOUT];
+  for (int q = 0; q < NIN; q++) { (void)hipMalloc(&hin[q], n * 4); (void)hipMemset(hin[q], 0, n * 4); }
+  for (int q = 0; q < NOUT; q++) { (void)hipMalloc(&hout[q], n * 4); (void)hipMemset(hout[q], 0, n * 4); }
+  float **din, **dout;
+  (void)hipMalloc(&din, NIN * sizeof(float*));
+  (void)hipMalloc(&dout, NOUT * sizeof(float*));
+  (void)hipMemcpy(din, hin, NIN * sizeof(float*), hipMemcpyHostToDevice);
+  (void)hipMemcpy(dout, hout, NOUT * sizeof(float*), hipMemcpyHostToDevice);
+  hipEvent_t e0, e1;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  hipLaunchKernelGGL((soa_stream<NIN, NOUT>), dim3(256 * 8), dim3(256), 0, 0, (const float* const*)din, dout, n);
+  (void)hipEventRecord(e0, 0);
+  for (int r = 0; r < reps; r++)
+    hipLaunchKernelGGL((soa_stream<NIN, NOUT>), dim3(256 * 8), dim3(256), 0, 0, (const float* const*)din, dout, n);
+  (void)hipEventRecord(e1, 0);
+  (void)hipEventSynchronize(e1);
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  const double per = ms / reps, bytes = (double)(NIN + NOUT) * n * 4;
+  printf("soa stream %d in + %d out arrays of %lld floats: %.4f ms per launch, %.1f GB/s\n", NIN, NOUT, n, per,
+         bytes / (per * 1e-3) / 1e9);
+  for (int q = 0; q < NIN; q++) (void)hipFree(hin[q]);
+  for (int q = 0; q < NOUT; q++) (void)hipFree(hout[q]);
+  (void)hipFree(din);
+  (void)hipFree(dout);
+  delete[] hin;
+  delete[] hout;
+}
+
 int main(int argc, char** argv) {
+  if (argc > 1 && argv[1][0] == 's') {  // "soa <npoints>": bulk-stage stream shapes
+    const long long n = argc > 2 ? atoll(argv[2]) : 16200801;
+    soa_run<2, 1>(n, 10);
+    soa_run<9, 9>(n, 10);
+    soa_run<18, 18>(n, 10);
+    return 0;
+  }
   const long long mib = argc > 1 ? atoll(argv[1]) : 1024;
   const int reps = argc > 2 ? atoi(argv[2]) : 20;
   const long long bytes = mib << 20;
