@@ -1212,6 +1212,7 @@ __global__ void __launch_bounds__(64 * TY) bulk_tiled3d(Tiled3D ta, const R* __r
         edge_deriv<R, C, S, 6>(dxv, f, xq, N, col, sx, i, ta.lo[0], ta.hi[0]);
       }
       constexpr int ymap[6] = {0, 1, 2, 3, 5, 6}, zmap[6] = {0, 1, 2, 4, 6, 7};
+      constexpr int NTB = 3 * (S - 1);  // taps of a one-sided row
       if (yin) {
 #pragma unroll
         for (int q = 0; q < 6; q++) {
@@ -1221,7 +1222,24 @@ __global__ void __launch_bounds__(64 * TY) bulk_tiled3d(Tiled3D ta, const R* __r
           dyv[q] = sy_;
         }
       } else {
-        edge_deriv<R, C, S, 6>(dyv, f, yq, N, pl + k, sy, j, ta.lo[1], ta.hi[1]);
+        // one-sided row: taps from the LDS tile when they lie inside it
+        const bool left = (j - ta.lo[1]) < 2 * (S - 1);
+        const int row = left ? j - ta.lo[1] + 1 : ta.hi[1] - j;
+        const int t0 = left ? ta.lo[1] : ta.hi[1] - NTB;  // lowest tap
+        if (t0 >= jb - r && t0 + NTB <= jb + TY + r) {
+#pragma unroll
+          for (int q = 0; q < 6; q++) {
+            C sy_ = 0.;
+#pragma unroll
+            for (int n = 0; n < NTB; n++) {
+              const int tj = left ? ta.lo[1] + n : ta.hi[1] - 1 - n;
+              sy_ += (C)c_fd[row][n] * (C)tile[ymap[q]][tj - (jb - r)][tz + r];
+            }
+            dyv[q] = left ? sy_ : -sy_;
+          }
+        } else {
+          edge_deriv<R, C, S, 6>(dyv, f, yq, N, pl + k, sy, j, ta.lo[1], ta.hi[1]);
+        }
       }
       if (zin) {
 #pragma unroll
@@ -1232,7 +1250,23 @@ __global__ void __launch_bounds__(64 * TY) bulk_tiled3d(Tiled3D ta, const R* __r
           dzv[q] = sz_;
         }
       } else {
-        edge_deriv<R, C, S, 6>(dzv, f, zq, N, pl + (long long)j * sy, 1, k, ta.lo[2], ta.hi[2]);
+        const bool left = (k - ta.lo[2]) < 2 * (S - 1);
+        const int row = left ? k - ta.lo[2] + 1 : ta.hi[2] - k;
+        const int t0 = left ? ta.lo[2] : ta.hi[2] - NTB;
+        if (t0 >= kb - r && t0 + NTB <= kb + TZ + r) {
+#pragma unroll
+          for (int q = 0; q < 6; q++) {
+            C sz_ = 0.;
+#pragma unroll
+            for (int n = 0; n < NTB; n++) {
+              const int tk = left ? ta.lo[2] + n : ta.hi[2] - 1 - n;
+              sz_ += (C)c_fd[row][n] * (C)tile[zmap[q]][ty + r][tk - (kb - r)];
+            }
+            dzv[q] = left ? sz_ : -sz_;
+          }
+        } else {
+          edge_deriv<R, C, S, 6>(dzv, f, zq, N, pl + (long long)j * sy, 1, k, ta.lo[2], ta.hi[2]);
+        }
       }
       const long long id = pl + (long long)j * sy + k;
       C acc[9];
