@@ -1050,9 +1050,15 @@ struct Tiled3D {
   int nzt, nyt, nxc, ntiles;     // tile grid
   double cv[3], cg2[3], cl[3], cg[3];  // dt/rho/dx*m, dt(2G+lam)/dx*m, dt lam/dx*m, dt G/dx*m
   double A, B;
+  float fcv[3], fcg2[3], fcl[3], fcg[3], fA, fB;  // the same, rounded once for the FP32 path
   int roundtrip, hetstress;
   double s0[6];
-  int face_active[6];            // external SAT faces (matrix in facemat)
+  int face_active[6];            // external SAT faces
+  // external SAT of a Cartesian face normal to axis d (other axes a < b) as
+  // its 16 non-zero linear-map entries, [6 faces][16], in the bulk precision,
+  // in device memory: rows (v_d, s_dd, s_aa, s_bb) x cols (v_d, s_dd), rows
+  // (v_a, s_da) x cols (v_a, s_da), same for b
+  const void* fcoef;
   int nif;                       // interface faces touching the block
   int if_dir[6], if_fixed[6], if_a[6], if_b[6], if_orga[6], if_orgb[6], if_n[6], if_n1[6], if_side[6];
   const double* if_sat[6];
@@ -1061,6 +1067,79 @@ struct Tiled3D {
 // derivative along one direction at a one-sided boundary row, taps read from
 // global memory: left rows use c_fd[pos-lo+1][n] at lo+n, right rows
 // -c_fd[hi-pos][n] at hi-1-n (block.cpp:1057-1162 pattern)
+// Raw buffer access (gfx9 buffer descriptor in scalar registers, 32-bit
+// per-lane byte offset plus a wave-uniform scalar byte offset): no per-lane
+// 64-bit address arithmetic, and no address registers for the compiler to
+// hoist out of a loop.  One descriptor per field array of `count` elements.
+typedef __amdgpu_buffer_rsrc_t rsrc_t;
+template <typename T>
+__device__ __forceinline__ rsrc_t buf_rsrc(const T* p, long long count) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<T*>(p), (short)0, (int)(unsigned)(count * (long long)sizeof(T)),
+                                           0x00020000);
+}
+template <typename T>
+__device__ __forceinline__ T bld(rsrc_t r, unsigned voff, unsigned soff) {
+  if constexpr (sizeof(T) == 4) return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+  else return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+template <typename T>
+__device__ __forceinline__ void bst(rsrc_t r, T v, unsigned voff, unsigned soff) {
+  if constexpr (sizeof(T) == 4) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, voff, soff, 0);
+  } else {
+    typedef unsigned u2 __attribute__((ext_vector_type(2)));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), r, voff, soff, 0);
+  }
+}
+
+// one-sided SBP row whose taps lie in one x plane: taps at element offsets
+// lane0 + tap*stride from the wave-uniform plane pointer fpl; left rows use
+// c_fd[pos-lo+1][n] at lo+n, right rows -c_fd[hi-pos][n] at hi-1-n
+template <typename R, typename C, int S>
+__device__ __forceinline__ void edge_plane(C* out, rsrc_t rf, unsigned fb, unsigned plb, const int* fq, int lane0,
+                                           int stride, int pos, int lo, int hi, const C (*sfd)[3 * (S - 1)]) {
+  constexpr int NT = 3 * (S - 1);
+  const bool left = pos - lo < 2 * (S - 1);
+  const int row = left ? pos - lo + 1 : hi - pos;
+#pragma unroll
+  for (int n = 0; n < NT; n++) {
+    const int tap = left ? lo + n : hi - 1 - n;
+    const C c = sfd[row - 1][n];
+    const unsigned off = (unsigned)((lane0 + tap * stride) * (int)sizeof(R));
+#pragma unroll
+    for (int q = 0; q < 6; q++) {
+      const C v = (C)bld<R>(rf, off, fq[q] * fb + plb);
+      out[q] = n == 0 ? c * v : out[q] + c * v;
+    }
+  }
+  if (!left)
+#pragma unroll
+    for (int q = 0; q < 6; q++) out[q] = -out[q];
+}
+
+// one-sided SBP row along x at the wave-uniform plane i: taps are whole planes
+template <typename R, typename C, int S>
+__device__ __forceinline__ void edge_x(C* out, rsrc_t rf, unsigned fb, unsigned psb, unsigned colb, int i, int lo,
+                                       int hi) {
+  constexpr int NT = 3 * (S - 1);
+  const bool left = i - lo < 2 * (S - 1);
+  const int row = left ? i - lo + 1 : hi - i;
+#pragma unroll
+  for (int n = 0; n < NT; n++) {
+    const int tap = left ? lo + n : hi - 1 - n;
+    const C c = (C)c_fd[row][n];
+    const unsigned plb = (unsigned)tap * psb;
+#pragma unroll
+    for (int q = 0; q < 6; q++) {
+      const C v = (C)bld<R>(rf, colb, q * fb + plb);
+      out[q] = n == 0 ? c * v : out[q] + c * v;
+    }
+  }
+  if (!left)
+#pragma unroll
+    for (int q = 0; q < 6; q++) out[q] = -out[q];
+}
+
 template <typename R, typename C, int S, int NQ>
 __device__ __forceinline__ void edge_deriv(C* out, const R* __restrict__ f, const int* fq, long long N, long long base,
                                            long long stride, int pos, int lo, int hi) {
@@ -1082,16 +1161,34 @@ __device__ __forceinline__ void edge_deriv(C* out, const R* __restrict__ f, cons
   for (int q = 0; q < NQ; q++) out[q] *= sg;
 }
 
-template <typename R, typename C, int S, int TY>
-__global__ void __launch_bounds__(64 * TY) bulk_tiled3d(Tiled3D ta, const R* __restrict__ f, R* __restrict__ fn,
-                                                        R* __restrict__ df, const double* __restrict__ hs,
-                                                        const double* __restrict__ facemat) {
+// interior first derivative from 2r+1 samples u[0..2r] centred on u[r]; the
+// interior SBP stencil is antisymmetric (fd.cpp:52-121), cd[n] = c_fd[0][r+n]
+template <typename C, int r, typename F>
+__device__ __forceinline__ C central(const C* cd, F u) {
+  C s = cd[1] * (u(r + 1) - u(r - 1));
+#pragma unroll
+  for (int n = 2; n <= r; n++) s += cd[n] * (u(r + n) - u(r - n));
+  return s;
+}
+
+// WPE: minimum waves per SIMD the register allocation must allow
+template <typename R, typename C, int S, int TY, int WPE>
+__global__ void __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(WPE))) bulk_tiled3d(Tiled3D ta, const R* __restrict__ f, R* __restrict__ fn,
+                                                        R* __restrict__ df, const double* __restrict__ hs) {
+  // This kernel serves the FP32 path and the fast FP64 path, neither of which
+  // keeps the reference's association, so fused multiply-adds are allowed.
+#pragma clang fp contract(fast)
   constexpr int r = S - 1;
   constexpr int W = 2 * r + 1;
   constexpr int TZ = 64;
   constexpr int LY = TY + 2 * r, LZ = TZ + 2 * r;
   constexpr int NT = 64 * TY;
   __shared__ R tile[8][LY][LZ];  // 0 vx 1 vy 2 vz 3 sxy 4 sxz 5 syy 6 syz 7 szz
+  // one-sided SBP rows in LDS: read inside the loop with per-lane rows
+  // (constant memory would be hoisted out of the loop into registers)
+  constexpr int NRB = 2 * (S - 1), NTB = 3 * (S - 1);
+  __shared__ C sfd[NRB][NTB];
+  if (threadIdx.x < NRB * NTB) sfd[threadIdx.x / NTB][threadIdx.x % NTB] = (C)c_fd[1 + threadIdx.x / NTB][threadIdx.x % NTB];
   // XCD-aware tile order: consecutive hardware workgroup ids land on
   // different XCDs, so give each XCD a contiguous range of tiles
   const int nxcd = 8;
@@ -1112,29 +1209,45 @@ __global__ void __launch_bounds__(64 * TY) bulk_tiled3d(Tiled3D ta, const R* __r
   const long long sx = (long long)ta.Ny * ta.Nz, sy = ta.Nz;
   const int kc = min(k, ta.hi[2] - 1), jc = min(j, ta.hi[1] - 1);
   const long long col = (long long)jc * sy + kc;
+  const unsigned colb = (unsigned)(col * (long long)sizeof(R));  // < one plane in bytes
+  const unsigned psb = (unsigned)(sx * (long long)sizeof(R));    // plane size in bytes
+  // one descriptor per array (the host keeps 9 N elements under 4 GiB);
+  // field q of plane i lies at scalar byte offset q*fb + i*psb
+  const rsrc_t rf = buf_rsrc(f, 9 * N), rd = buf_rsrc(df, 9 * N), rn = buf_rsrc(fn, 9 * N);
+  const unsigned fb = (unsigned)(N * (long long)sizeof(R));
   const bool yin = j >= ta.mc[1] && j < ta.mrb[1];
   const bool zin = k >= ta.mc[2] && k < ta.mrb[2];
-  C cd[W];
+  // block constants in the compute precision (uniform, scalar registers)
+  C kcv[3], kcg2[3], kcl[3], kcg[3], kA, kB;
 #pragma unroll
-  for (int n = 0; n < W; n++) cd[n] = (C)c_fd[0][n];
+  for (int d = 0; d < 3; d++) {
+    if constexpr (sizeof(C) == 4) {
+      kcv[d] = ta.fcv[d]; kcg2[d] = ta.fcg2[d]; kcl[d] = ta.fcl[d]; kcg[d] = ta.fcg[d];
+    } else {
+      kcv[d] = ta.cv[d]; kcg2[d] = ta.cg2[d]; kcl[d] = ta.cl[d]; kcg[d] = ta.cg[d];
+    }
+  }
+  if constexpr (sizeof(C) == 4) { kA = ta.fA; kB = ta.fB; } else { kA = ta.A; kB = ta.B; }
+  C cd[r + 1];
+#pragma unroll
+  for (int n = 0; n <= r; n++) cd[n] = (C)c_fd[0][r + n];
   constexpr int NZH = TY * 2 * r, NH = NZH + 2 * r * TZ;
-  auto xplane = [&](int i) { return (long long)min(max(i, ta.lo[0]), ta.hi[0] - 1) * sx; };
+  auto xplane = [&](int i) { return (unsigned)min(max(i, ta.lo[0]), ta.hi[0] - 1) * psb; };
   // x window: fields 0..5 = vx vy vz sxx sxy sxz at planes i-r .. i+r (clamped)
   R win[6][W];
 #pragma unroll
   for (int n = 0; n < W - 1; n++) {
-    const long long id = xplane(ib - r + n) + col;
+    const unsigned pw = xplane(ib - r + n);
 #pragma unroll
-    for (int q = 0; q < 6; q++) win[q][n] = f[q * N + id];
+    for (int q = 0; q < 6; q++) win[q][n] = bld<R>(rf, colb, q * fb + pw);
   }
   constexpr int tf[8] = {0, 1, 2, 4, 5, 6, 7, 8};  // f index of each tile field
-  constexpr int xq[6] = {0, 1, 2, 3, 4, 5};
   constexpr int yq[6] = {0, 1, 2, 4, 6, 7};  // vx vy vz sxy syy syz
   constexpr int zq[6] = {0, 1, 2, 5, 7, 8};  // vx vy vz sxz syz szz
   // halo entries this thread loads each plane (cross-shaped ring of the tile)
   constexpr int NHT = (NH + NT - 1) / NT;
   int hly[NHT], hlz[NHT];
-  long long hoff[NHT];
+  unsigned hoff[NHT];  // byte offset of the halo entry within a plane
 #pragma unroll
   for (int u = 0; u < NHT; u++) {
     const int e = threadIdx.x + u * NT;
@@ -1153,35 +1266,32 @@ __global__ void __launch_bounds__(64 * TY) bulk_tiled3d(Tiled3D ta, const R* __r
     hlz[u] = lz;
     const int gj = min(max(jb - r + ly, ta.lo[1]), ta.hi[1] - 1);
     const int gk = min(max(kb - r + lz, ta.lo[2]), ta.hi[2] - 1);
-    hoff[u] = (long long)gj * sy + gk;
+    hoff[u] = (unsigned)(((long long)gj * sy + gk) * (long long)sizeof(R));
   }
   // all loads of one plane are issued together, one plane ahead of its use
-  R nwin[6], nown[3], ndf[9], nhal[NHT][8];
+  R nwin[6], nown[3], nhal[NHT][8];
   auto issue = [&](int ip) {
-    const long long id = xplane(ip + r) + col, pc = (long long)ip * sx;
+    const unsigned pw = xplane(ip + r), pp = (unsigned)ip * psb;
 #pragma unroll
-    for (int q = 0; q < 6; q++) nwin[q] = f[q * N + id];
+    for (int q = 0; q < 6; q++) nwin[q] = bld<R>(rf, colb, q * fb + pw);
 #pragma unroll
-    for (int q = 0; q < 3; q++) nown[q] = f[(6 + q) * N + pc + col];
-#pragma unroll
-    for (int q = 0; q < 9; q++) ndf[q] = df[q * N + pc + col];
+    for (int q = 0; q < 3; q++) nown[q] = bld<R>(rf, colb, (6 + q) * fb + pp);
 #pragma unroll
     for (int u = 0; u < NHT; u++)
       if (hly[u] >= 0) {
 #pragma unroll
-        for (int q = 0; q < 8; q++) nhal[u][q] = f[tf[q] * N + pc + hoff[u]];
+        for (int q = 0; q < 8; q++) nhal[u][q] = bld<R>(rf, hoff[u], tf[q] * fb + pp);
       }
   };
   issue(ib);
   for (int i = ib; i < ie; i++) {
     const long long pl = (long long)i * sx;
+    const unsigned plb = (unsigned)i * psb;
 #pragma unroll
     for (int q = 0; q < 6; q++) win[q][W - 1] = nwin[q];
-    R own[3], odf[9];
+    R own[3];
 #pragma unroll
     for (int q = 0; q < 3; q++) own[q] = nown[q];
-#pragma unroll
-    for (int q = 0; q < 9; q++) odf[q] = ndf[q];
     tile[0][ty + r][tz + r] = win[0][r];
     tile[1][ty + r][tz + r] = win[1][r];
     tile[2][ty + r][tz + r] = win[2][r];
@@ -1198,29 +1308,26 @@ __global__ void __launch_bounds__(64 * TY) bulk_tiled3d(Tiled3D ta, const R* __r
       }
     lds_barrier();
     if (i + 1 < ie) issue(i + 1);
+    // df of this plane is first needed at the end of the update: loading it
+    // here rather than a plane ahead keeps it out of the loop-carried state
+    R odf[9];
+#pragma unroll
+    for (int q = 0; q < 9; q++) odf[q] = bld<R>(rd, colb, q * fb + plb);
     if (active) {
       C dxv[6], dyv[6], dzv[6];
+      const int jw = __builtin_amdgcn_readfirstlane(j);  // a wave is one y row
       if (i >= ta.mc[0] && i < ta.mrb[0]) {
 #pragma unroll
-        for (int q = 0; q < 6; q++) {
-          C s = 0.;
-#pragma unroll
-          for (int n = 0; n < W; n++) s += cd[n] * (C)win[q][n];
-          dxv[q] = s;  // d/dx of vx vy vz sxx sxy sxz
-        }
+        for (int q = 0; q < 6; q++)  // d/dx of vx vy vz sxx sxy sxz
+          dxv[q] = central<C, r>(cd, [&](int n) { return (C)win[q][n]; });
       } else {
-        edge_deriv<R, C, S, 6>(dxv, f, xq, N, col, sx, i, ta.lo[0], ta.hi[0]);
+        edge_x<R, C, S>(dxv, rf, fb, psb, colb, i, ta.lo[0], ta.hi[0]);
       }
       constexpr int ymap[6] = {0, 1, 2, 3, 5, 6}, zmap[6] = {0, 1, 2, 4, 6, 7};
-      constexpr int NTB = 3 * (S - 1);  // taps of a one-sided row
       if (yin) {
 #pragma unroll
-        for (int q = 0; q < 6; q++) {
-          C sy_ = 0.;
-#pragma unroll
-          for (int n = 0; n < W; n++) sy_ += cd[n] * (C)tile[ymap[q]][ty + n][tz + r];
-          dyv[q] = sy_;
-        }
+        for (int q = 0; q < 6; q++)
+          dyv[q] = central<C, r>(cd, [&](int n) { return (C)tile[ymap[q]][ty + n][tz + r]; });
       } else {
         // one-sided row: taps from the LDS tile when they lie inside it
         const bool left = (j - ta.lo[1]) < 2 * (S - 1);
@@ -1233,22 +1340,18 @@ __global__ void __launch_bounds__(64 * TY) bulk_tiled3d(Tiled3D ta, const R* __r
 #pragma unroll
             for (int n = 0; n < NTB; n++) {
               const int tj = left ? ta.lo[1] + n : ta.hi[1] - 1 - n;
-              sy_ += (C)c_fd[row][n] * (C)tile[ymap[q]][tj - (jb - r)][tz + r];
+              sy_ += sfd[row - 1][n] * (C)tile[ymap[q]][tj - (jb - r)][tz + r];
             }
             dyv[q] = left ? sy_ : -sy_;
           }
         } else {
-          edge_deriv<R, C, S, 6>(dyv, f, yq, N, pl + k, sy, j, ta.lo[1], ta.hi[1]);
+          edge_plane<R, C, S>(dyv, rf, fb, plb, yq, k, (int)sy, jw, ta.lo[1], ta.hi[1], sfd);
         }
       }
       if (zin) {
 #pragma unroll
-        for (int q = 0; q < 6; q++) {
-          C sz_ = 0.;
-#pragma unroll
-          for (int n = 0; n < W; n++) sz_ += cd[n] * (C)tile[zmap[q]][ty + r][tz + n];
-          dzv[q] = sz_;
-        }
+        for (int q = 0; q < 6; q++)
+          dzv[q] = central<C, r>(cd, [&](int n) { return (C)tile[zmap[q]][ty + r][tz + n]; });
       } else {
         const bool left = (k - ta.lo[2]) < 2 * (S - 1);
         const int row = left ? k - ta.lo[2] + 1 : ta.hi[2] - k;
@@ -1260,27 +1363,27 @@ __global__ void __launch_bounds__(64 * TY) bulk_tiled3d(Tiled3D ta, const R* __r
 #pragma unroll
             for (int n = 0; n < NTB; n++) {
               const int tk = left ? ta.lo[2] + n : ta.hi[2] - 1 - n;
-              sz_ += (C)c_fd[row][n] * (C)tile[zmap[q]][ty + r][tk - (kb - r)];
+              sz_ += sfd[row - 1][n] * (C)tile[zmap[q]][ty + r][tk - (kb - r)];
             }
             dzv[q] = left ? sz_ : -sz_;
           }
         } else {
-          edge_deriv<R, C, S, 6>(dzv, f, zq, N, pl + (long long)j * sy, 1, k, ta.lo[2], ta.hi[2]);
+          edge_plane<R, C, S>(dzv, rf, fb, plb, zq, jw * (int)sy, 1, k, ta.lo[2], ta.hi[2], sfd);
         }
       }
       const long long id = pl + (long long)j * sy + k;
       C acc[9];
 #pragma unroll
-      for (int q = 0; q < 9; q++) acc[q] = (C)odf[q] * (C)ta.A;
-      acc[0] += (C)ta.cv[0] * dxv[3] + (C)ta.cv[1] * dyv[3] + (C)ta.cv[2] * dzv[3];
-      acc[1] += (C)ta.cv[0] * dxv[4] + (C)ta.cv[1] * dyv[4] + (C)ta.cv[2] * dzv[4];
-      acc[2] += (C)ta.cv[0] * dxv[5] + (C)ta.cv[1] * dyv[5] + (C)ta.cv[2] * dzv[5];
-      acc[3] += (C)ta.cg2[0] * dxv[0] + (C)ta.cl[1] * dyv[1] + (C)ta.cl[2] * dzv[2];
-      acc[4] += (C)ta.cg[0] * dxv[1] + (C)ta.cg[1] * dyv[0];
-      acc[5] += (C)ta.cg[0] * dxv[2] + (C)ta.cg[2] * dzv[0];
-      acc[6] += (C)ta.cl[0] * dxv[0] + (C)ta.cg2[1] * dyv[1] + (C)ta.cl[2] * dzv[2];
-      acc[7] += (C)ta.cg[1] * dyv[2] + (C)ta.cg[2] * dzv[1];
-      acc[8] += (C)ta.cl[0] * dxv[0] + (C)ta.cl[1] * dyv[1] + (C)ta.cg2[2] * dzv[2];
+      for (int q = 0; q < 9; q++) acc[q] = (C)odf[q] * kA;
+      acc[0] += kcv[0] * dxv[3] + kcv[1] * dyv[3] + kcv[2] * dzv[3];
+      acc[1] += kcv[0] * dxv[4] + kcv[1] * dyv[4] + kcv[2] * dzv[4];
+      acc[2] += kcv[0] * dxv[5] + kcv[1] * dyv[5] + kcv[2] * dzv[5];
+      acc[3] += kcg2[0] * dxv[0] + kcl[1] * dyv[1] + kcl[2] * dzv[2];
+      acc[4] += kcg[0] * dxv[1] + kcg[1] * dyv[0];
+      acc[5] += kcg[0] * dxv[2] + kcg[2] * dzv[0];
+      acc[6] += kcl[0] * dxv[0] + kcg2[1] * dyv[1] + kcl[2] * dzv[2];
+      acc[7] += kcg[1] * dyv[2] + kcg[2] * dzv[1];
+      acc[8] += kcl[0] * dxv[0] + kcl[1] * dyv[1] + kcg2[2] * dzv[2];
       C fo[9];
 #pragma unroll
       for (int q = 0; q < 6; q++) fo[q] = (C)win[q][r];
@@ -1293,31 +1396,37 @@ __global__ void __launch_bounds__(64 * TY) bulk_tiled3d(Tiled3D ta, const R* __r
 #pragma unroll
       for (int e = 0; e < 6; e++) {
         if (!ta.face_active[e]) continue;
-        const int de = e >> 1;
-        if (c3[de] != ((e & 1) ? ta.hi[de] - 1 : ta.lo[de])) continue;
-        const double* Mx = facemat + e * 81;
-#pragma unroll
-        for (int q = 0; q < 9; q++) {
-          C sq = 0.;
-#pragma unroll
-          for (int p = 0; p < 9; p++) sq += (C)Mx[q * 9 + p] * fo[p];
-          acc[q] += sq;
-        }
+        constexpr int sid[3][3] = {{3, 4, 5}, {4, 6, 7}, {5, 7, 8}};  // stress index of (row, col)
+        const int d = e >> 1, a = d == 0 ? 1 : 0, b = d == 2 ? 1 : 2;
+        if (c3[d] != ((e & 1) ? ta.hi[d] - 1 : ta.lo[d])) continue;
+        // read in the loop from device memory: kernel arguments would be
+        // hoisted out of the loop as 96 live registers
+        const C* m = reinterpret_cast<const C*>(ta.fcoef) + e * 16;
+        const C vd = fo[d], sdd = fo[sid[d][d]];
+        acc[d] += (C)m[0] * vd + (C)m[1] * sdd;
+        acc[sid[d][d]] += (C)m[2] * vd + (C)m[3] * sdd;
+        acc[sid[a][a]] += (C)m[4] * vd + (C)m[5] * sdd;
+        acc[sid[b][b]] += (C)m[6] * vd + (C)m[7] * sdd;
+        const C va = fo[a], sda = fo[sid[d][a]], vb = fo[b], sdb = fo[sid[d][b]];
+        acc[a] += (C)m[8] * va + (C)m[9] * sda;
+        acc[sid[d][a]] += (C)m[10] * va + (C)m[11] * sda;
+        acc[b] += (C)m[12] * vb + (C)m[13] * sdb;
+        acc[sid[d][b]] += (C)m[14] * vb + (C)m[15] * sdb;
       }
       // interface SAT terms written by fault_kernel (interface.cpp:826-962)
       for (int e = 0; e < ta.nif; e++) {
         if (c3[ta.if_dir[e]] != ta.if_fixed[e]) continue;
-        const long long o = (long long)(c3[ta.if_a[e]] - ta.if_orga[e]) * ta.if_n1[e] + (c3[ta.if_b[e]] - ta.if_orgb[e]);
+        const unsigned o = (unsigned)(((c3[ta.if_a[e]] - ta.if_orga[e]) * ta.if_n1[e] + (c3[ta.if_b[e]] - ta.if_orgb[e])) * 8);
         const long long n = ta.if_n[e];
         const double* sn = ta.if_sat[e] + (long long)(ta.if_side[e] * 2 + 0) * 9 * n;
-        const double* sv = ta.if_sat[e] + (long long)(ta.if_side[e] * 2 + 1) * 9 * n;
+        const double* sv = sn + 9 * n;
 #pragma unroll
-        for (int q = 0; q < 9; q++) acc[q] -= (C)(sn[q * n + o] + sv[q * n + o]);
+        for (int q = 0; q < 9; q++) acc[q] -= (C)(sn[q * n + o / 8] + sv[q * n + o / 8]);
       }
 #pragma unroll
       for (int q = 0; q < 9; q++) {
-        df[q * N + id] = (R)acc[q];
-        C v = fo[q] + (C)ta.B * acc[q];
+        bst<R>(rd, (R)acc[q], colb, q * fb + plb);
+        C v = fo[q] + kB * acc[q];
         if (q >= 3 && ta.roundtrip) {  // set_stress / remove_stress rounding (FP64 only)
           double w = (double)v;
           w += ta.s0[q - 3];
@@ -1326,7 +1435,7 @@ __global__ void __launch_bounds__(64 * TY) bulk_tiled3d(Tiled3D ta, const R* __r
           if (ta.hetstress) w -= hs[(q - 3) * N + id];
           v = (C)w;
         }
-        fn[q * N + id] = (R)v;
+        bst<R>(rn, (R)v, colb, q * fb + plb);
       }
     }
     lds_barrier();  // every wave is done reading this plane's tile

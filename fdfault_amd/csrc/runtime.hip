@@ -94,8 +94,8 @@ class Solver : public SolverBase {
   void do_stage(int istep, int st, bool full_step);
   void launch_fault(int i, const FaultStageDev& fs);
   void launch_bulk(int b, const StageDev& sd);
-  void launch_tiled(Tiled3D ta, int ty, const double* fm);
-  std::vector<DBuf<double>> facemats;  // per block [6][9][9] external SAT maps
+  void launch_tiled(Tiled3D ta, int ty);
+  std::vector<DBuf<R>> fcoefs;  // per block [6][16] external SAT entries
   int tile_ty = 4;
   void launch_plastic(int b);
   void write_outputs(int tstep);
@@ -137,13 +137,45 @@ class Solver : public SolverBase {
   int graph_flip = 0;
 };
 
+// The external SAT of a face normal to axis d, as the 9x9 map M[row][col] of
+// a point's fields, only couples (v_d, s_dd) into v_d, s_dd, s_aa, s_bb and
+// (v_c, s_dc) into v_c, s_dc for the two other axes c.  Extract those 16
+// entries; false if any other entry is not negligible (non-Cartesian face).
+static bool compact_face_sat(const double* M, int d, double* fc) {
+  static const int sid[3][3] = {{3, 4, 5}, {4, 6, 7}, {5, 7, 8}};
+  const int a = d == 0 ? 1 : 0, b = d == 2 ? 1 : 2;
+  bool used[81] = {};
+  int n = 0;
+  auto take = [&](int row, int col) {
+    used[row * 9 + col] = true;
+    fc[n++] = M[row * 9 + col];
+  };
+  for (int row : {d, sid[d][d], sid[a][a], sid[b][b]}) {
+    take(row, d);
+    take(row, sid[d][d]);
+  }
+  for (int c : {a, b})
+    for (int row : {c, sid[d][c]}) {
+      take(row, c);
+      take(row, sid[d][c]);
+    }
+  double mx = 0.;
+  for (int q = 0; q < 81; q++) mx = fmax(mx, fabs(M[q]));
+  for (int q = 0; q < 81; q++)
+    if (!used[q] && fabs(M[q]) > 1e-9 * mx) return false;
+  return true;
+}
+
 template <typename R>
 Solver<R>::Solver(const std::string& path, const fdf_opts& o) {
   opts = o;
   compile_problem(path, S, o.write_outputs != 0, o.quiet != 0);
   HIPCHK(hipSetDevice(o.device));
+  // measured best tile heights on a 401x201x201 block: 4 rows (FP64), 8 (FP32)
+  const int ty_default = sizeof(R) == 8 ? 4 : 8;
+  tile_ty = ty_default;
   if (const char* e = getenv("FDF_TILE_TY")) tile_ty = atoi(e);
-  if (tile_ty != 2 && tile_ty != 4 && tile_ty != 8) tile_ty = 4;
+  if (tile_ty != 2 && tile_ty != 4 && tile_ty != 8) tile_ty = ty_default;
   HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
   for (int e = 0; e < 2 * NEV; e++) HIPCHK(hipEventCreate(&evs[e]));
   const size_t N = S.N;
@@ -237,7 +269,9 @@ Solver<R>::Solver(const std::string& path, const fdf_opts& o) {
     }
     bdev.push_back(D);
     // rectilinear interior fast path (3-D, homogeneous, no dissipation)
-    bool ok = S.ndim == 3 && !S.hetmat && !(S.cdiss > 0.) && !(sizeof(R) == 8 && o.exact_fp64);
+    // (the tiled kernel addresses each 9-field array with 32-bit buffer offsets)
+    bool ok = S.ndim == 3 && !S.hetmat && !(S.cdiss > 0.) && !(sizeof(R) == 8 && o.exact_fp64) &&
+              9. * (double)N * sizeof(R) < 4294967296.;
     for (int d = 0; d < 3 && ok; d++) ok = B.mrb[d] > B.mc[d] && B.nx[d] >= 3 * (S.sbporder - 1);
     Tiled3D ta{};
     if (ok) {
@@ -267,8 +301,9 @@ Solver<R>::Solver(const std::string& path, const fdf_opts& o) {
         ta.Ny = S.nx[1];
         ta.Nz = S.nx[2];
         ta.N = (long long)N;
-        ta.cx = 16;
-        if (const char* e = getenv("FDF_TILE_CX")) ta.cx = atoi(e) > 0 ? atoi(e) : 16;
+        const int cx_default = 4;  // x planes per tile (short chunks: more workgroups in flight)
+        ta.cx = cx_default;
+        if (const char* e = getenv("FDF_TILE_CX")) ta.cx = atoi(e) > 0 ? atoi(e) : cx_default;
         ta.nzt = (ta.hi[2] - ta.lo[2] + 63) / 64;
         ta.nyt = (ta.hi[1] - ta.lo[1] + 3) / 4;
         ta.nxc = (ta.hi[0] - ta.lo[0] + ta.cx - 1) / ta.cx;
@@ -278,6 +313,10 @@ Solver<R>::Solver(const std::string& path, const fdf_opts& o) {
           ta.cg2[d] = B.K[d][1] * mdiag[d];
           ta.cg[d] = B.K[d][2] * mdiag[d];
           ta.cl[d] = B.K[d][3] * mdiag[d];
+          ta.fcv[d] = (float)ta.cv[d];
+          ta.fcg2[d] = (float)ta.cg2[d];
+          ta.fcg[d] = (float)ta.cg[d];
+          ta.fcl[d] = (float)ta.cl[d];
         }
         ta.hetstress = S.hetstress;
         for (int q = 0; q < 6; q++) ta.s0[q] = S.s0[q];
@@ -286,7 +325,6 @@ Solver<R>::Solver(const std::string& path, const fdf_opts& o) {
     }
     tiled_ok.push_back(ok);
     tiles.push_back(ta);
-    facemats.emplace_back();
   }
   // interfaces
   ibufs.resize(S.ifaces.size());
@@ -388,8 +426,10 @@ Solver<R>::Solver(const std::string& path, const fdf_opts& o) {
   {
     std::vector<double> unit(81, 0.);
     for (int q = 0; q < 9; q++) unit[q * 9 + q] = 1.;  // field q of point q
-    DBuf<double> dunit;
+    DBuf<double> dunit, fm;
     dunit.upload(unit);
+    fm.alloc(81);
+    fcoefs.resize(bdev.size());
     for (size_t b = 0; b < bdev.size(); b++) {
       if (!tiled_ok[b]) continue;
       const BlockDev& D = bdev[b];
@@ -408,16 +448,22 @@ Solver<R>::Solver(const std::string& path, const fdf_opts& o) {
         ta.if_side[e] = X.side;
         ta.if_sat[e] = X.sat;
       }
-      facemats[b].alloc(6 * 81);
-      facemats[b].zero(stream);
-      for (int e = 0; e < 6; e++) {
+      double fc[6][16] = {};
+      for (int e = 0; e < 6 && tiled_ok[b]; e++) {
         if (!D.face[e].active) continue;
         hipLaunchKernelGGL(face_matrix_kernel, dim3(1), dim3(64), 0, stream, D.face[e], G, S.dt, D.face[e].n0 / 2,
-                           D.face[e].n1 / 2, dunit.p, facemats[b].p + e * 81);
+                           D.face[e].n1 / 2, dunit.p, fm.p);
         HIPCHK(hipGetLastError());
+        double M[81];
+        HIPCHK(hipMemcpyAsync(M, fm.p, sizeof(M), hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        if (!compact_face_sat(M, e >> 1, fc[e])) tiled_ok[b] = false;
       }
+      std::vector<R> fcr(6 * 16);
+      for (int q = 0; q < 6 * 16; q++) fcr[q] = (R)fc[q / 16][q % 16];
+      fcoefs[b].upload(fcr);
+      ta.fcoef = fcoefs[b].p;
     }
-    HIPCHK(hipStreamSynchronize(stream));
   }
   HIPCHK(hipStreamSynchronize(stream));
   // t = 0: interface solve without SAT (domain.cpp:141-143) on f = 0
@@ -593,21 +639,23 @@ static Boxes make_boxes(const BlockDev& D, bool shell) {
 // interior kernel launch; the compute type is FP64 for FP64 storage and
 // FP32 for FP32 storage
 template <typename R, int TY>
-static void launch_tiled_ty(const Tiled3D& ta, int s, const R* f, R* fn, R* df, const double* hs, const double* fm,
-                            hipStream_t st) {
+static void launch_tiled_ty(const Tiled3D& ta, int s, const R* f, R* fn, R* df, const double* hs, hipStream_t st) {
   const dim3 grid((unsigned)(((ta.ntiles + 7) / 8) * 8)), blk(64 * TY);
-  if (s == 2) hipLaunchKernelGGL((bulk_tiled3d<R, R, 2, TY>), grid, blk, 0, st, ta, f, fn, df, hs, fm);
-  else if (s == 3) hipLaunchKernelGGL((bulk_tiled3d<R, R, 3, TY>), grid, blk, 0, st, ta, f, fn, df, hs, fm);
-  else hipLaunchKernelGGL((bulk_tiled3d<R, R, 4, TY>), grid, blk, 0, st, ta, f, fn, df, hs, fm);
+  // waves per SIMD the register allocation must allow: 4 (<= 128 VGPRs)
+  // for FP32; FP64 holds twice the registers per value and runs at 2
+  constexpr int WPE = sizeof(R) == 4 ? 4 : 2;
+  if (s == 2) hipLaunchKernelGGL((bulk_tiled3d<R, R, 2, TY, WPE>), grid, blk, 0, st, ta, f, fn, df, hs);
+  else if (s == 3) hipLaunchKernelGGL((bulk_tiled3d<R, R, 3, TY, WPE>), grid, blk, 0, st, ta, f, fn, df, hs);
+  else hipLaunchKernelGGL((bulk_tiled3d<R, R, 4, TY, WPE>), grid, blk, 0, st, ta, f, fn, df, hs);
 }
 
 template <typename R>
-void Solver<R>::launch_tiled(Tiled3D ta, int ty, const double* fm) {
+void Solver<R>::launch_tiled(Tiled3D ta, int ty) {
   ta.nyt = (ta.hi[1] - ta.lo[1] + ty - 1) / ty;
   ta.ntiles = ta.nzt * ta.nyt * ta.nxc;
-  if (ty == 2) launch_tiled_ty<R, 2>(ta, S.sbporder, f[cur].p, f[cur ^ 1].p, df.p, hs.p, fm, stream);
-  else if (ty == 4) launch_tiled_ty<R, 4>(ta, S.sbporder, f[cur].p, f[cur ^ 1].p, df.p, hs.p, fm, stream);
-  else launch_tiled_ty<R, 8>(ta, S.sbporder, f[cur].p, f[cur ^ 1].p, df.p, hs.p, fm, stream);
+  if (ty == 2) launch_tiled_ty<R, 2>(ta, S.sbporder, f[cur].p, f[cur ^ 1].p, df.p, hs.p, stream);
+  else if (ty == 4) launch_tiled_ty<R, 4>(ta, S.sbporder, f[cur].p, f[cur ^ 1].p, df.p, hs.p, stream);
+  else launch_tiled_ty<R, 8>(ta, S.sbporder, f[cur].p, f[cur ^ 1].p, df.p, hs.p, stream);
 }
 
 template <typename R>
@@ -619,8 +667,10 @@ void Solver<R>::launch_bulk(int b, const StageDev& sd) {
     Tiled3D ta = tiles[b];
     ta.A = sd.A;
     ta.B = sd.B;
+    ta.fA = (float)sd.A;
+    ta.fB = (float)sd.B;
     ta.roundtrip = sd.roundtrip;
-    launch_tiled(ta, tile_ty, facemats[b].p);
+    launch_tiled(ta, tile_ty);
     HIPCHK(hipGetLastError());
     return;
   }
