@@ -78,7 +78,7 @@ def main():
     ap.add_argument("--ny", type=int, default=201)
     ap.add_argument("--nz", type=int, default=201)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=5)
+    ap.add_argument("--cpu-steps", type=int, default=20)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU only (the whole grid fits one MI355X); "
@@ -101,6 +101,7 @@ def main():
         npts = g.N
         nst = g.nstages
         nblocks = g.info.nblocks
+        ntiled = g.info.tiled_blocks
         g.close()
         mpts = npts * nst * args.steps / t / 1e6
         B = bytes_per_point(9, w)
@@ -117,18 +118,23 @@ def main():
             cpu = {"value": round(cv, 3), "unit": "Mpt-updates/s", "cores": threads, "kind": "port",
                    "sample": "same 3-D workload at 2x%dx%dx%d points (%d pts), %d steps, FP64 CPU oracle, "
                              "OpenMP" % (snx, sny, snz, cn, args.cpu_steps)}
+    workload = "3d_planar_fault_slipweak_2x%dx%dx%d" % (args.nx, args.ny, args.nz)
+    kernel = "bulk_tiled3d" if ntiled == nblocks else "bulk_kernel"
+    from fdfault_amd.profiling import lookup_traffic
+    tr = lookup_traffic("%s/%s/sbp%d" % (workload, args.precision, args.sbporder))
     line = {
         "metric": "Mpt-updates/s per RK stage & % HBM peak; FP32 vs FP64 at <=1e-4 slip err",
         "value": round(mpts, 2), "unit": "Mpt-updates/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(t / args.steps * 1e3, 3), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64" if w == 8 else "f32",
         "data": "synthetic",
-        "config": {"workload": "3d_planar_fault_slipweak_2x%dx%dx%d" % (args.nx, args.ny, args.nz),
+        "config": {"workload": workload,
                    "points": npts, "rk_stages": nst, "sbporder": args.sbporder,
                    "precision": args.precision},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "bulk_kernel", "avg_launch_ms": round(kms, 4), "launches": nlaunch,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": tr["traffic"] if tr else None,
+                     "traffic_source": tr["summary"] if tr else None,
+                     "kernel": kernel, "avg_launch_ms": round(kms, 4), "launches": nlaunch,
                      "bytes_per_point": B, "points_per_launch": int(pts_per_launch)},
         "cpu_baseline": cpu,
     }

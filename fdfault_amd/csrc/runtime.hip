@@ -72,6 +72,7 @@ class SolverBase {
   SetupH S;
   fdf_opts opts{};
   int cur_step = 0;
+  int ntiled = 0;  // blocks on the rectilinear tiled kernel
 };
 
 template <typename R>
@@ -466,6 +467,7 @@ Solver<R>::Solver(const std::string& path, const fdf_opts& o) {
     }
   }
   HIPCHK(hipStreamSynchronize(stream));
+  for (size_t b = 0; b < tiled_ok.size(); b++) ntiled += tiled_ok[b] ? 1 : 0;
   // t = 0: interface solve without SAT (domain.cpp:141-143) on f = 0
   {
     FaultStageDev fs{0., 0., 0., 0., 1, -1., nullptr, 0., 0};
@@ -1139,6 +1141,7 @@ int fdf_get_info(fdf_t* h, fdf_info* info) {
   info->ttot = S.ttot;
   info->cfl = S.cfl;
   info->t = h->solver->cur_step * S.dt;
+  info->tiled_blocks = h->solver->ntiled;
   return 0;
 }
 

@@ -43,6 +43,7 @@ typedef struct fdf_info {
   int nfields_total; /* including plastic fields                                */
   int nifaces, sbporder, plastic, nblocks, step, precision;
   double dt, ttot, cfl, t;
+  int tiled_blocks;  /* blocks updated by the rectilinear tiled kernel         */
 } fdf_info;
 
 /* Replaces problem::problem(const char*) (src/problem.cpp:16-84): parses the
