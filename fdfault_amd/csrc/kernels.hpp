@@ -596,11 +596,10 @@ __device__ double stz_newton(double xmin, double xmax, const double* p, int* err
 // friction laws of friction.cpp:259-435, slipweak.cpp:206-308 and
 // stz.cpp:263-356, then friction::calc_df/update (friction.cpp:465-505).
 template <int M, typename R>
-__global__ void __launch_bounds__(256) fault_kernel(FaultDev F, GridDev g, FaultStageDev fs, const R* __restrict__ f,
-                                                    const R* __restrict__ mat, const double* __restrict__ hs,
-                                                    int* __restrict__ err, int* __restrict__ newton_iters) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= F.n) return;
+__device__ __forceinline__ void fault_point(const FaultDev& F, const GridDev& g, const FaultStageDev& fs,
+                                            const R* __restrict__ f, const R* __restrict__ mat,
+                                            const double* __restrict__ hs, int* __restrict__ err,
+                                            int* __restrict__ newton_iters, int p) {
   const int ii = p / F.n1, jj = p % F.n1;
   const long long N = g.N;
   const long long stride[3] = {(long long)g.Ny * g.Nz, (long long)g.Nz, 1};
@@ -890,6 +889,15 @@ __global__ void __launch_bounds__(256) fault_kernel(FaultDev F, GridDev g, Fault
     sv1[p] = cs1 * h1 * o1.v3; sv1[n + p] = cs1 * h1 * o1.s13; sv1[2 * n + p] = cs1 * h1 * o1.s23;
     sv2[p] = cs2 * h2 * o2.v3; sv2[n + p] = cs2 * h2 * o2.s13; sv2[2 * n + p] = cs2 * h2 * o2.s23;
   }
+}
+
+template <int M, typename R>
+__global__ void __launch_bounds__(256) fault_kernel(FaultDev F, GridDev g, FaultStageDev fs, const R* __restrict__ f,
+                                                    const R* __restrict__ mat, const double* __restrict__ hs,
+                                                    int* __restrict__ err, int* __restrict__ newton_iters) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= F.n) return;
+  fault_point<M, R>(F, g, fs, f, mat, hs, err, newton_iters, p);
 }
 
 // ---------------------------------------------------------------------------
@@ -1447,6 +1455,276 @@ __global__ void __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(WP
   }
 }
 
+// ---------------------------------------------------------------------------
+// Rectilinear 2-D mode II block kernel (fields vx vy sxx sxy syy).
+//
+// On a Cartesian block the update of block.cpp:1048-1278 reduces to
+// per-direction derivatives scaled by block constants.  A workgroup of TW
+// threads owns TW consecutive y points (y is the contiguous axis in 2-D) and
+// marches along x over a chunk of planes:
+//   * x derivatives of vx vy sxx sxy from a register window over 2r+1 planes;
+//   * y derivatives of vx vy sxy syy from an LDS row with a halo of 3(s-1)
+//     points on each side, wide enough that every one-sided SBP row of the
+//     block reads its taps from LDS;
+//   * external SAT as the face's 10 non-zero linear-map entries, interface
+//     SAT from the fault buffer, df = A*df + ..., f_new = f + B*df.
+// The per-tile routine is shared by the single-block launch and the batched
+// (ensemble) launch, so both give bit-identical results for one instance.
+struct Tiled2D {
+  int lo[2], hi[2];   // block box (x, y), global indices
+  int mc[2], mrb[2];  // interior rows [mc, mrb) per direction
+  int Ny;             // y size of the grid (row stride)
+  long long N;        // grid points
+  int cx;             // x planes per chunk
+  int nyt, nxc, ntiles;
+  double cv[2], cg2[2], cl[2], cg[2], A, B;
+  float fcv[2], fcg2[2], fcl[2], fcg[2], fA, fB;
+  int roundtrip, hetstress;
+  double s0[3];       // sxx sxy syy
+  int face_active[4];
+  const void* fcoef;  // [4 faces][10] external SAT entries in the bulk precision
+  int nif;
+  int if_dir[4], if_fixed[4], if_a[4], if_orga[4], if_n[4], if_side[4];
+  const double* if_sat[4];
+  // arrays of this grid (the batched launch gives each instance its own)
+  const void* f[2];   // f, indexed by the parity of the stage
+  void* df;
+  const double* hs;
+};
+
+template <typename R, typename C, int S, int TW>
+__device__ __forceinline__ void tile2d(const Tiled2D& ta, int tile_id, int par) {
+#pragma clang fp contract(fast)
+  constexpr int r = S - 1, W = 2 * r + 1;
+  constexpr int NRB = 2 * (S - 1), NTB = 3 * (S - 1);
+  constexpr int H = NTB;            // LDS halo per side
+  constexpr int LW = TW + 2 * H;
+  __shared__ R row[4][LW];          // 0 vx 1 vy 2 sxy 3 syy
+  __shared__ C sfd[NRB][NTB];
+  const R* __restrict__ f = reinterpret_cast<const R*>(ta.f[par]);
+  R* __restrict__ fn = const_cast<R*>(reinterpret_cast<const R*>(ta.f[par ^ 1]));
+  R* __restrict__ df = reinterpret_cast<R*>(ta.df);
+  const int t = threadIdx.x;
+  if (t < NRB * NTB) sfd[t / NTB][t % NTB] = (C)c_fd[1 + t / NTB][t % NTB];
+  const int yt = tile_id % ta.nyt, xc = tile_id / ta.nyt;
+  const int jb = ta.lo[1] + yt * TW, j = jb + t;
+  const bool active = j < ta.hi[1];
+  const int ib = ta.lo[0] + xc * ta.cx, ie = min(ib + ta.cx, ta.hi[0]);
+  const long long N = ta.N;
+  const int jc = min(j, ta.hi[1] - 1);
+  const unsigned colb = (unsigned)jc * (unsigned)sizeof(R);
+  const unsigned psb = (unsigned)ta.Ny * (unsigned)sizeof(R);
+  const unsigned fb = (unsigned)(N * (long long)sizeof(R));
+  const rsrc_t rf = buf_rsrc(f, 5 * N), rd = buf_rsrc(df, 5 * N), rn = buf_rsrc(fn, 5 * N);
+  const bool yin = j >= ta.mc[1] && j < ta.mrb[1];
+  C kcv[2], kcg2[2], kcl[2], kcg[2], kA, kB;
+#pragma unroll
+  for (int d = 0; d < 2; d++) {
+    if constexpr (sizeof(C) == 4) {
+      kcv[d] = ta.fcv[d]; kcg2[d] = ta.fcg2[d]; kcl[d] = ta.fcl[d]; kcg[d] = ta.fcg[d];
+    } else {
+      kcv[d] = ta.cv[d]; kcg2[d] = ta.cg2[d]; kcl[d] = ta.cl[d]; kcg[d] = ta.cg[d];
+    }
+  }
+  if constexpr (sizeof(C) == 4) { kA = ta.fA; kB = ta.fB; } else { kA = ta.A; kB = ta.B; }
+  C cd[r + 1];
+#pragma unroll
+  for (int n = 0; n <= r; n++) cd[n] = (C)c_fd[0][r + n];
+  auto xplane = [&](int i) { return (unsigned)min(max(i, ta.lo[0]), ta.hi[0] - 1) * psb; };
+  // halo entry of this thread (threads 0..2H-1): row position and y index
+  const bool hal = t < 2 * H;
+  const int hpos = t < H ? t : TW + t;  // position in the LDS row
+  const int hj = min(max(jb - H + hpos, ta.lo[1]), ta.hi[1] - 1);
+  const unsigned hoff = (unsigned)hj * (unsigned)sizeof(R);
+  // x window: vx vy sxx sxy at planes i-r .. i+r (clamped)
+  R win[4][W];
+#pragma unroll
+  for (int n = 0; n < W - 1; n++) {
+    const unsigned pw = xplane(ib - r + n);
+#pragma unroll
+    for (int q = 0; q < 4; q++) win[q][n] = bld<R>(rf, colb, q * fb + pw);
+  }
+  constexpr int tf[4] = {0, 1, 3, 4};  // f index of each LDS row field
+  R nwin[4], nown, nhal[4];
+  auto issue = [&](int ip) {
+    const unsigned pw = xplane(ip + r), pp = (unsigned)ip * psb;
+#pragma unroll
+    for (int q = 0; q < 4; q++) nwin[q] = bld<R>(rf, colb, q * fb + pw);
+    nown = bld<R>(rf, colb, 4 * fb + pp);
+    if (hal) {
+#pragma unroll
+      for (int q = 0; q < 4; q++) nhal[q] = bld<R>(rf, hoff, tf[q] * fb + pp);
+    }
+  };
+  issue(ib);
+  for (int i = ib; i < ie; i++) {
+    const unsigned plb = (unsigned)i * psb;
+#pragma unroll
+    for (int q = 0; q < 4; q++) win[q][W - 1] = nwin[q];
+    const R own = nown;
+    row[0][H + t] = win[0][r];
+    row[1][H + t] = win[1][r];
+    row[2][H + t] = win[3][r];
+    row[3][H + t] = own;
+    if (hal) {
+#pragma unroll
+      for (int q = 0; q < 4; q++) row[q][hpos] = nhal[q];
+    }
+    lds_barrier();
+    if (i + 1 < ie) issue(i + 1);
+    R odf[5];
+#pragma unroll
+    for (int q = 0; q < 5; q++) odf[q] = bld<R>(rd, colb, q * fb + plb);
+    if (active) {
+      C dxv[4], dyv[4];  // d/dx of vx vy sxx sxy; d/dy of vx vy sxy syy
+      if (i >= ta.mc[0] && i < ta.mrb[0]) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) dxv[q] = central<C, r>(cd, [&](int n) { return (C)win[q][n]; });
+      } else {  // one-sided row along x: taps are whole planes (wave-uniform)
+        const bool left = i - ta.lo[0] < NRB;
+        const int rw = left ? i - ta.lo[0] + 1 : ta.hi[0] - i;
+#pragma unroll
+        for (int n = 0; n < NTB; n++) {
+          const int tap = left ? ta.lo[0] + n : ta.hi[0] - 1 - n;
+          const C c = (C)c_fd[rw][n];
+#pragma unroll
+          for (int q = 0; q < 4; q++) {
+            const C v = (C)bld<R>(rf, colb, q * fb + (unsigned)tap * psb);
+            dxv[q] = n == 0 ? c * v : dxv[q] + c * v;
+          }
+        }
+        if (!left)
+#pragma unroll
+          for (int q = 0; q < 4; q++) dxv[q] = -dxv[q];
+      }
+      if (yin) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) dyv[q] = central<C, r>(cd, [&](int n) { return (C)row[q][t + H - r + n]; });
+      } else {  // one-sided row along y: taps in the LDS row
+        const bool left = j - ta.lo[1] < NRB;
+        const int rw = left ? j - ta.lo[1] + 1 : ta.hi[1] - j;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          C sy = 0.;
+#pragma unroll
+          for (int n = 0; n < NTB; n++) {
+            const int tj = left ? ta.lo[1] + n : ta.hi[1] - 1 - n;
+            sy += sfd[rw - 1][n] * (C)row[q][tj - jb + H];
+          }
+          dyv[q] = left ? sy : -sy;
+        }
+      }
+      C acc[5];
+#pragma unroll
+      for (int q = 0; q < 5; q++) acc[q] = (C)odf[q] * kA;
+      acc[0] += kcv[0] * dxv[2] + kcv[1] * dyv[2];  // vx: d/dx sxx + d/dy sxy
+      acc[1] += kcv[0] * dxv[3] + kcv[1] * dyv[3];  // vy: d/dx sxy + d/dy syy
+      acc[2] += kcg2[0] * dxv[0] + kcl[1] * dyv[1];
+      acc[3] += kcg[0] * dxv[1] + kcg[1] * dyv[0];
+      acc[4] += kcl[0] * dxv[0] + kcg2[1] * dyv[1];
+      C fo[5];
+#pragma unroll
+      for (int q = 0; q < 4; q++) fo[q] = (C)win[q][r];
+      fo[4] = (C)own;
+      const int c2[3] = {i, j, 0};
+      // external SAT: rows (v_d, s_dd, s_aa) x cols (v_d, s_dd), rows (v_a, sxy) x cols (v_a, sxy)
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        if (!ta.face_active[e]) continue;
+        const int d = e >> 1, a = 1 - d;
+        if (c2[d] != ((e & 1) ? ta.hi[d] - 1 : ta.lo[d])) continue;
+        const int sdd = d == 0 ? 2 : 4, saa = d == 0 ? 4 : 2;
+        const C* m = reinterpret_cast<const C*>(ta.fcoef) + e * 10;
+        const C vd = fo[d], sd = fo[sdd], va = fo[a], sa = fo[3];
+        acc[d] += m[0] * vd + m[1] * sd;
+        acc[sdd] += m[2] * vd + m[3] * sd;
+        acc[saa] += m[4] * vd + m[5] * sd;
+        acc[a] += m[6] * va + m[7] * sa;
+        acc[3] += m[8] * va + m[9] * sa;
+      }
+      // interface SAT terms written by the fault kernel (interface.cpp:826-962)
+      for (int e = 0; e < ta.nif; e++) {
+        if (c2[ta.if_dir[e]] != ta.if_fixed[e]) continue;
+        const long long o = c2[ta.if_a[e]] - ta.if_orga[e];
+        const long long n = ta.if_n[e];
+        const double* sn = ta.if_sat[e] + (long long)(ta.if_side[e] * 2 + 0) * 5 * n;
+        const double* sv = sn + 5 * n;
+#pragma unroll
+        for (int q = 0; q < 5; q++) acc[q] -= (C)(sn[q * n + o] + sv[q * n + o]);
+      }
+      constexpr int s0i[5] = {-1, -1, 0, 1, 2};
+#pragma unroll
+      for (int q = 0; q < 5; q++) {
+        bst<R>(rd, (R)acc[q], colb, q * fb + plb);
+        C v = fo[q] + kB * acc[q];
+        if (q >= 2 && ta.roundtrip) {  // set_stress / remove_stress rounding (FP64 only)
+          const long long id = (long long)i * ta.Ny + j;
+          double w = (double)v;
+          w += ta.s0[s0i[q]];
+          if (ta.hetstress) w += ta.hs[(q - 2) * N + id];
+          w -= ta.s0[s0i[q]];
+          if (ta.hetstress) w -= ta.hs[(q - 2) * N + id];
+          v = (C)w;
+        }
+        bst<R>(rn, (R)v, colb, q * fb + plb);
+      }
+    }
+    lds_barrier();  // every wave is done reading this plane's row
+#pragma unroll
+    for (int n = 0; n < W - 1; n++)
+#pragma unroll
+      for (int q = 0; q < 4; q++) win[q][n] = win[q][n + 1];
+  }
+}
+
+// one block: workgroup b is tile b in XCD-aware order
+template <typename R, typename C, int S, int TW>
+__global__ void __launch_bounds__(TW) bulk_tiled2d(Tiled2D ta, int par) {
+  const int nxcd = 8;
+  const int per = (ta.ntiles + nxcd - 1) / nxcd;
+  const int tile_id = (blockIdx.x % nxcd) * per + blockIdx.x / nxcd;
+  if (tile_id >= ta.ntiles) return;
+  tile2d<R, C, S, TW>(ta, tile_id, par);
+}
+
+// ensemble: a persistent grid walks the work list (block descriptor, tile)
+// of every instance; uneven instance sizes only change the list length
+template <typename R, typename C, int S, int TW>
+__global__ void __launch_bounds__(TW) bulk_tiled2d_batch(const Tiled2D* __restrict__ descs,
+                                                        const int2* __restrict__ work, int nwork, int par) {
+  for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
+    const int2 it = work[w];
+    tile2d<R, C, S, TW>(descs[it.x], it.y, par);
+  }
+}
+
+// ensemble fault update: work item = (instance, first fault point of a
+// chunk of 256); the f arrays of the stage parity come with the instance
+struct FaultBatchDev {
+  FaultDev F;
+  GridDev g;
+  const void* f[2];
+  const void* mat;
+  const double* hs;
+  double dt;  // this instance's time step (stage times come from the step counter)
+};
+
+template <int M, typename R>
+__global__ void __launch_bounds__(256) fault_batch_kernel(const FaultBatchDev* __restrict__ inst,
+                                                          const int2* __restrict__ work, int nwork,
+                                                          FaultStageDev fs, int par, int* __restrict__ err) {
+  for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
+    const int2 it = work[w];
+    const FaultBatchDev& I = inst[it.x];
+    const int p = it.y + threadIdx.x;
+    FaultStageDev fsi = fs;
+    fsi.dt = I.dt;
+    if (p < I.F.n)
+      fault_point<M, R>(I.F, I.g, fsi, reinterpret_cast<const R*>(I.f[par]), reinterpret_cast<const R*>(I.mat), I.hs,
+                        err, nullptr, p);
+  }
+}
+
 struct FaceList {
   int start[7];  // prefix offsets of the face point counts, face order
 };
@@ -1454,16 +1732,18 @@ struct FaceList {
 // Linear map of the external SAT on one face point: column q is ext_sat
 // applied to the unit state e_q (9 fields), so M . f reproduces the penalty
 // added to df for state f at that point.  Run once at set-up.
+template <int M>
 __global__ void face_matrix_kernel(FaceDev F, GridDev g, double dt, int u, int v, const double* __restrict__ unit,
                                    double* __restrict__ out) {
+  constexpr int NF = Nf<M>::v;
   const int q = threadIdx.x;
-  if (q >= 9) return;
+  if (q >= NF) return;
   GridDev gg = g;
-  gg.N = 9;
+  gg.N = NF;
   gg.hetmat = 0;
   double acc[10] = {0., 0., 0., 0., 0., 0., 0., 0., 0., 0.};
-  ext_sat<2, double, double>(acc, F, unit, nullptr, gg, q, u, v, dt);
-  for (int p = 0; p < 9; p++) out[p * 9 + q] = acc[p];
+  ext_sat<M, double, double>(acc, F, unit, nullptr, gg, q, u, v, dt);
+  for (int p = 0; p < NF; p++) out[p * NF + q] = acc[p];
 }
 
 // SAT terms of the face points of a block whose bulk update came from

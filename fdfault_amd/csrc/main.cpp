@@ -1,7 +1,10 @@
 // Drop-in replacement of the reference executable: `fdfault <file.in>`
 // (src/main.cpp:8-72).  Same argument, same progress lines, same output and
 // front files; errors print the reference's message and exit non-zero.
-// FDFAULT_PRECISION=fp32 selects the FP32-bulk / FP64-fault path.
+// FDFAULT_PRECISION=fp32 selects the FP32-bulk / FP64-fault path;
+// FDFAULT_EXACT=1 keeps the reference's arithmetic order everywhere (FP64
+// output files byte-identical to the reference algorithm) instead of the
+// rectilinear fast kernels (within 1e-10).
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -21,6 +24,8 @@ int main(int argc, char** argv) {
   o.quiet = 0;
   const char* p = getenv("FDFAULT_PRECISION");
   if (p && strcmp(p, "fp32") == 0) o.precision = FDF_FP32;
+  const char* ex = getenv("FDFAULT_EXACT");
+  if (ex && atoi(ex) != 0) o.exact_fp64 = 1;
   const char* dv = getenv("FDFAULT_DEVICE");
   if (dv) o.device = atoi(dv);
   fdf_t* h = nullptr;

@@ -76,7 +76,12 @@ class SolverBase {
 };
 
 template <typename R>
+class Batch;
+
+template <typename R>
 class Solver : public SolverBase {
+  friend class Batch<R>;
+
  public:
   explicit Solver(const std::string& path, const fdf_opts& o);
   ~Solver() override;
@@ -96,6 +101,8 @@ class Solver : public SolverBase {
   void launch_fault(int i, const FaultStageDev& fs);
   void launch_bulk(int b, const StageDev& sd);
   void launch_tiled(Tiled3D ta, int ty);
+  void launch_tiled2(Tiled2D t2, int tw, int par, hipStream_t st);
+  Tiled2D stage_tile2(int b, const StageDev& sd) const;
   std::vector<DBuf<R>> fcoefs;  // per block [6][16] external SAT entries
   int tile_ty = 4;
   void launch_plastic(int b);
@@ -116,6 +123,8 @@ class Solver : public SolverBase {
   std::vector<BlockDev> bdev;
   std::vector<bool> tiled_ok;
   std::vector<Tiled3D> tiles;
+  std::vector<Tiled2D> tiles2;
+  int tile2_tw = 256, tile2_cx = 8;  // 2-D tile: y points per workgroup, x planes per chunk
   struct FaceBufs {
     DBuf<double> nrm, dl;
   };
@@ -167,6 +176,31 @@ static bool compact_face_sat(const double* M, int d, double* fc) {
   return true;
 }
 
+// The same for a 2-D mode II face normal to axis d (fields vx vy sxx sxy
+// syy): (v_d, s_dd) couple into v_d, s_dd, s_aa; (v_a, sxy) into v_a, sxy.
+static bool compact_face_sat2(const double* M, int d, double* fc) {
+  const int a = 1 - d, sdd = d == 0 ? 2 : 4, saa = d == 0 ? 4 : 2;
+  bool used[25] = {};
+  int n = 0;
+  auto take = [&](int row, int col) {
+    used[row * 5 + col] = true;
+    fc[n++] = M[row * 5 + col];
+  };
+  for (int row : {d, sdd, saa}) {
+    take(row, d);
+    take(row, sdd);
+  }
+  for (int row : {a, 3}) {
+    take(row, a);
+    take(row, 3);
+  }
+  double mx = 0.;
+  for (int q = 0; q < 25; q++) mx = fmax(mx, fabs(M[q]));
+  for (int q = 0; q < 25; q++)
+    if (!used[q] && fabs(M[q]) > 1e-9 * mx) return false;
+  return true;
+}
+
 template <typename R>
 Solver<R>::Solver(const std::string& path, const fdf_opts& o) {
   opts = o;
@@ -177,6 +211,9 @@ Solver<R>::Solver(const std::string& path, const fdf_opts& o) {
   tile_ty = ty_default;
   if (const char* e = getenv("FDF_TILE_TY")) tile_ty = atoi(e);
   if (tile_ty != 2 && tile_ty != 4 && tile_ty != 8) tile_ty = ty_default;
+  if (const char* e = getenv("FDF_TILE2_TW")) tile2_tw = atoi(e);
+  if (tile2_tw != 64 && tile2_tw != 128 && tile2_tw != 256) tile2_tw = 256;
+  if (const char* e = getenv("FDF_TILE2_CX")) tile2_cx = atoi(e) > 0 ? atoi(e) : 8;
   HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
   for (int e = 0; e < 2 * NEV; e++) HIPCHK(hipEventCreate(&evs[e]));
   const size_t N = S.N;
@@ -269,63 +306,97 @@ Solver<R>::Solver(const std::string& path, const fdf_opts& o) {
       FD.dl = fb.dl.p;
     }
     bdev.push_back(D);
-    // rectilinear interior fast path (3-D, homogeneous, no dissipation)
-    // (the tiled kernel addresses each 9-field array with 32-bit buffer offsets)
-    bool ok = S.ndim == 3 && !S.hetmat && !(S.cdiss > 0.) && !(sizeof(R) == 8 && o.exact_fp64) &&
-              9. * (double)N * sizeof(R) < 4294967296.;
-    for (int d = 0; d < 3 && ok; d++) ok = B.mrb[d] > B.mc[d] && B.nx[d] >= 3 * (S.sbporder - 1);
-    Tiled3D ta{};
+    // rectilinear fast paths: 3-D, and 2-D mode II without plasticity;
+    // homogeneous material, no dissipation, not the bit-faithful FP64 mode
+    // (the tiled kernels address each field array with 32-bit buffer offsets)
+    const bool ok3 = nd == 3, ok2 = nd == 2 && S.mode == 2 && !S.plastic;
+    bool ok = (ok3 || ok2) && !S.hetmat && !(S.cdiss > 0.) && !(sizeof(R) == 8 && o.exact_fp64) &&
+              (double)S.nf * (double)N * sizeof(R) < 4294967296.;
+    for (int d = 0; d < nd && ok; d++) ok = B.mrb[d] > B.mc[d] && B.nx[d] >= 3 * (S.sbporder - 1);
+    double mdiag[3] = {0., 0., 0.};
     if (ok) {
       // metric diagonal and uniform, Jacobian uniform, to 1e-9 relative (the
       // SBP derivative of TFI coordinates leaves ~1e-12 rounding noise)
       const size_t gc = S.idx((B.mlb[0] + B.prb[0]) / 2, (B.mlb[1] + B.prb[1]) / 2, (B.mlb[2] + B.prb[2]) / 2);
-      double mdiag[3], jc = S.jac[gc];
-      for (int d = 0; d < 3; d++) mdiag[d] = S.metric[(size_t)(d * 3 + d) * N + gc];
+      const double jc = S.jac[gc];
+      for (int d = 0; d < nd; d++) mdiag[d] = S.metric[(size_t)(d * nd + d) * N + gc];
       for (int i = B.mlb[0]; i < B.prb[0] && ok; i++)
         for (int j = B.mlb[1]; j < B.prb[1] && ok; j++)
           for (int k = B.mlb[2]; k < B.prb[2] && ok; k++) {
             const size_t g = S.idx(i, j, k);
             if (fabs(S.jac[g] - jc) > 1e-9 * fabs(jc)) ok = false;
-            for (int l = 0; l < 3 && ok; l++)
-              for (int m = 0; m < 3 && ok; m++) {
-                const double v = S.metric[(size_t)(l * 3 + m) * N + g];
+            for (int l = 0; l < nd && ok; l++)
+              for (int m = 0; m < nd && ok; m++) {
+                const double v = S.metric[(size_t)(l * nd + m) * N + g];
                 if (l == m ? fabs(v - mdiag[l]) > 1e-9 * fabs(mdiag[l]) : fabs(v) > 1e-9 * fabs(mdiag[l])) ok = false;
               }
           }
-      if (ok) {
-        for (int d = 0; d < 3; d++) {
-          ta.lo[d] = B.mlb[d];
-          ta.hi[d] = B.prb[d];
-          ta.mc[d] = B.mc[d];
-          ta.mrb[d] = B.mrb[d];
-        }
-        ta.Ny = S.nx[1];
-        ta.Nz = S.nx[2];
-        ta.N = (long long)N;
-        const int cx_default = 4;  // x planes per tile (short chunks: more workgroups in flight)
-        ta.cx = cx_default;
-        if (const char* e = getenv("FDF_TILE_CX")) ta.cx = atoi(e) > 0 ? atoi(e) : cx_default;
-        ta.nzt = (ta.hi[2] - ta.lo[2] + 63) / 64;
-        ta.nyt = (ta.hi[1] - ta.lo[1] + 3) / 4;
-        ta.nxc = (ta.hi[0] - ta.lo[0] + ta.cx - 1) / ta.cx;
-        ta.ntiles = ta.nzt * ta.nyt * ta.nxc;
-        for (int d = 0; d < 3; d++) {
-          ta.cv[d] = B.K[d][0] * mdiag[d];
-          ta.cg2[d] = B.K[d][1] * mdiag[d];
-          ta.cg[d] = B.K[d][2] * mdiag[d];
-          ta.cl[d] = B.K[d][3] * mdiag[d];
-          ta.fcv[d] = (float)ta.cv[d];
-          ta.fcg2[d] = (float)ta.cg2[d];
-          ta.fcg[d] = (float)ta.cg[d];
-          ta.fcl[d] = (float)ta.cl[d];
-        }
-        ta.hetstress = S.hetstress;
-        for (int q = 0; q < 6; q++) ta.s0[q] = S.s0[q];
-        for (int e = 0; e < 6; e++) ta.face_active[e] = D.face[e].active;
+    }
+    Tiled3D ta{};
+    Tiled2D t2{};
+    if (ok && ok3) {
+      for (int d = 0; d < 3; d++) {
+        ta.lo[d] = B.mlb[d];
+        ta.hi[d] = B.prb[d];
+        ta.mc[d] = B.mc[d];
+        ta.mrb[d] = B.mrb[d];
       }
+      ta.Ny = S.nx[1];
+      ta.Nz = S.nx[2];
+      ta.N = (long long)N;
+      const int cx_default = 4;  // x planes per tile (short chunks: more workgroups in flight)
+      ta.cx = cx_default;
+      if (const char* e = getenv("FDF_TILE_CX")) ta.cx = atoi(e) > 0 ? atoi(e) : cx_default;
+      ta.nzt = (ta.hi[2] - ta.lo[2] + 63) / 64;
+      ta.nyt = (ta.hi[1] - ta.lo[1] + 3) / 4;
+      ta.nxc = (ta.hi[0] - ta.lo[0] + ta.cx - 1) / ta.cx;
+      ta.ntiles = ta.nzt * ta.nyt * ta.nxc;
+      for (int d = 0; d < 3; d++) {
+        ta.cv[d] = B.K[d][0] * mdiag[d];
+        ta.cg2[d] = B.K[d][1] * mdiag[d];
+        ta.cg[d] = B.K[d][2] * mdiag[d];
+        ta.cl[d] = B.K[d][3] * mdiag[d];
+        ta.fcv[d] = (float)ta.cv[d];
+        ta.fcg2[d] = (float)ta.cg2[d];
+        ta.fcg[d] = (float)ta.cg[d];
+        ta.fcl[d] = (float)ta.cl[d];
+      }
+      ta.hetstress = S.hetstress;
+      for (int q = 0; q < 6; q++) ta.s0[q] = S.s0[q];
+      for (int e = 0; e < 6; e++) ta.face_active[e] = D.face[e].active;
+    }
+    if (ok && ok2) {
+      for (int d = 0; d < 2; d++) {
+        t2.lo[d] = B.mlb[d];
+        t2.hi[d] = B.prb[d];
+        t2.mc[d] = B.mc[d];
+        t2.mrb[d] = B.mrb[d];
+      }
+      t2.Ny = S.nx[1];
+      t2.N = (long long)N;
+      t2.cx = tile2_cx;
+      t2.nyt = (t2.hi[1] - t2.lo[1] + tile2_tw - 1) / tile2_tw;
+      t2.nxc = (t2.hi[0] - t2.lo[0] + t2.cx - 1) / t2.cx;
+      t2.ntiles = t2.nyt * t2.nxc;
+      for (int d = 0; d < 2; d++) {
+        t2.cv[d] = B.K[d][0] * mdiag[d];
+        t2.cg2[d] = B.K[d][1] * mdiag[d];
+        t2.cg[d] = B.K[d][2] * mdiag[d];
+        t2.cl[d] = B.K[d][3] * mdiag[d];
+        t2.fcv[d] = (float)t2.cv[d];
+        t2.fcg2[d] = (float)t2.cg2[d];
+        t2.fcg[d] = (float)t2.cg[d];
+        t2.fcl[d] = (float)t2.cl[d];
+      }
+      t2.hetstress = S.hetstress;
+      t2.s0[0] = S.s0[0];
+      t2.s0[1] = S.s0[1];
+      t2.s0[2] = S.s0[3];
+      for (int e = 0; e < 4; e++) t2.face_active[e] = D.face[e].active;
     }
     tiled_ok.push_back(ok);
     tiles.push_back(ta);
+    tiles2.push_back(t2);
   }
   // interfaces
   ibufs.resize(S.ifaces.size());
@@ -423,10 +494,11 @@ Solver<R>::Solver(const std::string& path, const fdf_opts& o) {
       X.sat = ib.sat.p;
     }
   }
-  // rectilinear blocks: interface faces and external SAT maps for the tiled kernel
+  // rectilinear blocks: interface faces and external SAT maps for the tiled kernels
   {
-    std::vector<double> unit(81, 0.);
-    for (int q = 0; q < 9; q++) unit[q * 9 + q] = 1.;  // field q of point q
+    const int nfu = S.nf;
+    std::vector<double> unit((size_t)nfu * nfu, 0.);
+    for (int q = 0; q < nfu; q++) unit[q * nfu + q] = 1.;  // field q of point q
     DBuf<double> dunit, fm;
     dunit.upload(unit);
     fm.alloc(81);
@@ -434,36 +506,60 @@ Solver<R>::Solver(const std::string& path, const fdf_opts& o) {
     for (size_t b = 0; b < bdev.size(); b++) {
       if (!tiled_ok[b]) continue;
       const BlockDev& D = bdev[b];
-      Tiled3D& ta = tiles[b];
-      ta.nif = D.nif;
-      for (int e = 0; e < D.nif; e++) {
-        const IfFaceDev& X = D.ifc[e];
-        ta.if_dir[e] = X.dir;
-        ta.if_fixed[e] = X.fixed;
-        ta.if_a[e] = X.a;
-        ta.if_b[e] = X.b;
-        ta.if_orga[e] = X.org_a;
-        ta.if_orgb[e] = X.org_b;
-        ta.if_n[e] = X.n;
-        ta.if_n1[e] = X.n1;
-        ta.if_side[e] = X.side;
-        ta.if_sat[e] = X.sat;
-      }
+      const int nfc = 2 * nd, ne = nd == 3 ? 16 : 10;  // faces, compact entries per face
       double fc[6][16] = {};
-      for (int e = 0; e < 6 && tiled_ok[b]; e++) {
+      for (int e = 0; e < nfc && tiled_ok[b]; e++) {
         if (!D.face[e].active) continue;
-        hipLaunchKernelGGL(face_matrix_kernel, dim3(1), dim3(64), 0, stream, D.face[e], G, S.dt, D.face[e].n0 / 2,
-                           D.face[e].n1 / 2, dunit.p, fm.p);
+        if (nd == 3)
+          hipLaunchKernelGGL(face_matrix_kernel<2>, dim3(1), dim3(64), 0, stream, D.face[e], G, S.dt,
+                             D.face[e].n0 / 2, D.face[e].n1 / 2, dunit.p, fm.p);
+        else
+          hipLaunchKernelGGL(face_matrix_kernel<0>, dim3(1), dim3(64), 0, stream, D.face[e], G, S.dt,
+                             D.face[e].n0 / 2, D.face[e].n1 / 2, dunit.p, fm.p);
         HIPCHK(hipGetLastError());
         double M[81];
-        HIPCHK(hipMemcpyAsync(M, fm.p, sizeof(M), hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipMemcpyAsync(M, fm.p, sizeof(double) * nfu * nfu, hipMemcpyDeviceToHost, stream));
         HIPCHK(hipStreamSynchronize(stream));
-        if (!compact_face_sat(M, e >> 1, fc[e])) tiled_ok[b] = false;
+        const bool good = nd == 3 ? compact_face_sat(M, e >> 1, fc[e]) : compact_face_sat2(M, e >> 1, fc[e]);
+        if (!good) tiled_ok[b] = false;
       }
-      std::vector<R> fcr(6 * 16);
-      for (int q = 0; q < 6 * 16; q++) fcr[q] = (R)fc[q / 16][q % 16];
+      if (!tiled_ok[b]) continue;
+      std::vector<R> fcr((size_t)nfc * ne);
+      for (int q = 0; q < nfc * ne; q++) fcr[q] = (R)fc[q / ne][q % ne];
       fcoefs[b].upload(fcr);
-      ta.fcoef = fcoefs[b].p;
+      if (nd == 3) {
+        Tiled3D& ta = tiles[b];
+        ta.fcoef = fcoefs[b].p;
+        ta.nif = D.nif;
+        for (int e = 0; e < D.nif; e++) {
+          const IfFaceDev& X = D.ifc[e];
+          ta.if_dir[e] = X.dir;
+          ta.if_fixed[e] = X.fixed;
+          ta.if_a[e] = X.a;
+          ta.if_b[e] = X.b;
+          ta.if_orga[e] = X.org_a;
+          ta.if_orgb[e] = X.org_b;
+          ta.if_n[e] = X.n;
+          ta.if_n1[e] = X.n1;
+          ta.if_side[e] = X.side;
+          ta.if_sat[e] = X.sat;
+        }
+      } else {
+        Tiled2D& t2 = tiles2[b];
+        t2.fcoef = fcoefs[b].p;
+        if (D.nif > 4) throw Error{"too many interfaces on one block"};
+        t2.nif = D.nif;
+        for (int e = 0; e < D.nif; e++) {
+          const IfFaceDev& X = D.ifc[e];
+          t2.if_dir[e] = X.dir;
+          t2.if_fixed[e] = X.fixed;
+          t2.if_a[e] = X.a;
+          t2.if_orga[e] = X.org_a;
+          t2.if_n[e] = X.n;
+          t2.if_side[e] = X.side;
+          t2.if_sat[e] = X.sat;
+        }
+      }
     }
   }
   HIPCHK(hipStreamSynchronize(stream));
@@ -651,6 +747,39 @@ static void launch_tiled_ty(const Tiled3D& ta, int s, const R* f, R* fn, R* df, 
   else hipLaunchKernelGGL((bulk_tiled3d<R, R, 4, TY, WPE>), grid, blk, 0, st, ta, f, fn, df, hs);
 }
 
+template <typename R, int TW>
+static void launch_tiled2_tw(const Tiled2D& t2, int s, int par, hipStream_t st) {
+  const dim3 grid((unsigned)(((t2.ntiles + 7) / 8) * 8)), blk(TW);
+  if (s == 2) hipLaunchKernelGGL((bulk_tiled2d<R, R, 2, TW>), grid, blk, 0, st, t2, par);
+  else if (s == 3) hipLaunchKernelGGL((bulk_tiled2d<R, R, 3, TW>), grid, blk, 0, st, t2, par);
+  else hipLaunchKernelGGL((bulk_tiled2d<R, R, 4, TW>), grid, blk, 0, st, t2, par);
+}
+
+// stage descriptor of a 2-D rectilinear block: RK coefficients and arrays
+template <typename R>
+Tiled2D Solver<R>::stage_tile2(int b, const StageDev& sd) const {
+  Tiled2D t2 = tiles2[b];
+  t2.A = sd.A;
+  t2.B = sd.B;
+  t2.fA = (float)sd.A;
+  t2.fB = (float)sd.B;
+  t2.roundtrip = sd.roundtrip;
+  t2.f[0] = f[0].p;
+  t2.f[1] = f[1].p;
+  t2.df = df.p;
+  t2.hs = hs.p;
+  return t2;
+}
+
+template <typename R>
+void Solver<R>::launch_tiled2(Tiled2D t2, int tw, int par, hipStream_t st) {
+  t2.nyt = (t2.hi[1] - t2.lo[1] + tw - 1) / tw;
+  t2.ntiles = t2.nyt * t2.nxc;
+  if (tw == 64) launch_tiled2_tw<R, 64>(t2, S.sbporder, par, st);
+  else if (tw == 128) launch_tiled2_tw<R, 128>(t2, S.sbporder, par, st);
+  else launch_tiled2_tw<R, 256>(t2, S.sbporder, par, st);
+}
+
 template <typename R>
 void Solver<R>::launch_tiled(Tiled3D ta, int ty) {
   ta.nyt = (ta.hi[1] - ta.lo[1] + ty - 1) / ty;
@@ -665,6 +794,12 @@ void Solver<R>::launch_bulk(int b, const StageDev& sd) {
   const BlockDev& D = bdev[b];
   const bool diss = S.cdiss > 0., het = S.hetmat;
   const bool tiled = tiled_ok[b];
+  if (tiled && S.ndim == 2) {
+    Tiled2D t2 = stage_tile2(b, sd);
+    launch_tiled2(t2, tile2_tw, cur, stream);
+    HIPCHK(hipGetLastError());
+    return;
+  }
   if (tiled) {
     Tiled3D ta = tiles[b];
     ta.A = sd.A;
@@ -1022,10 +1157,375 @@ double Solver<R>::bulk_ms(int* n) {
   return avg;
 }
 
+// ---------------------------------------------------------------------------
+// Ensemble of independent 2-D mode II ruptures stepped together.  Each
+// instance is set up exactly like a single run (parse, grid, t = 0 solve,
+// outputs); the batch then replaces the per-instance stage launches by two
+// launches per RK stage over every instance: fault_batch_kernel (all fault
+// points) and the persistent bulk_tiled2d_batch (all tiles of all blocks).
+// Stage descriptors are precomputed per RK stage, so one time step is a fixed
+// launch sequence that replays from a hipGraph.  Tile width and x-chunk come
+// from a timing autotuner run at open (cached per device and ensemble).
+class BatchBase {
+ public:
+  virtual ~BatchBase() {}
+  virtual void step(int n) = 0;
+  virtual SolverBase* instance(int i) = 0;
+  virtual int size() const = 0;
+  virtual void sync() = 0;
+  virtual void finish() = 0;
+  int tw = 0, cx = 0;
+  double tune_ms = 0.;
+  int tuned_from_cache = 0;
+};
+
+template <typename R>
+class Batch : public BatchBase {
+ public:
+  Batch(const std::vector<std::string>& paths, const fdf_opts& o);
+  ~Batch() override;
+  void step(int n) override;
+  SolverBase* instance(int i) override { return inst.at(i).get(); }
+  int size() const override { return (int)inst.size(); }
+  void sync() override;
+  void finish() override;
+
+ private:
+  void build(int tw_, int cx_);
+  void launch_stage(int st);
+  double time_config(int tw_, int cx_);
+  void autotune();
+  template <int TW>
+  void launch_bulk_tw(int st);
+
+  fdf_opts opts{};
+  std::vector<std::unique_ptr<Solver<R>>> inst;
+  hipStream_t stream = nullptr;
+  int nst = 0, sbp = 0, rko = 0, cur = 0, cur_step = 0, nt = 0;
+  LowStorageRK rk;
+  std::vector<DBuf<Tiled2D>> descs;  // per RK stage, all blocks of all instances
+  DBuf<int2> work;
+  int nwork = 0, grid = 0;
+  DBuf<FaultBatchDev> fdescs;
+  DBuf<int2> fwork;
+  int nfwork = 0;
+  DBuf<int> err, dstep;
+  hipGraphExec_t graphs[2] = {nullptr, nullptr};
+  bool dstep_valid = false;
+};
+
+template <typename R>
+Batch<R>::Batch(const std::vector<std::string>& paths, const fdf_opts& o) : opts(o) {
+  if (paths.empty()) throw Error{"fdf_open_batch: no instances"};
+  for (size_t i = 0; i < paths.size(); i++) {
+    fdf_opts oi = o;
+    oi.use_graph = 0;
+    inst.emplace_back(new Solver<R>(paths[i], oi));
+    const Solver<R>& I = *inst.back();
+    const SetupH& S = I.S;
+    if (S.ndim != 2 || S.mode != 2 || S.plastic)
+      throw Error{"fdf_open_batch: instance " + std::to_string(i) + " is not a 2-D mode II elastic problem"};
+    for (size_t b = 0; b < S.blocks.size(); b++)
+      if (!I.tiled_ok[b])
+        throw Error{"fdf_open_batch: instance " + std::to_string(i) +
+                    " has a block that is not rectilinear and homogeneous (or uses dissipation)"};
+    if (i == 0) {
+      nst = S.rk.nstages;
+      sbp = S.sbporder;
+      rko = S.rkorder;
+      rk = S.rk;
+      nt = S.nt;
+    } else if (S.rkorder != rko || S.sbporder != sbp) {
+      throw Error{"fdf_open_batch: instances must share sbporder and rkorder"};
+    }
+  }
+  HIPCHK(hipSetDevice(o.device));
+  HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  for (auto& I : inst) HIPCHK(hipStreamSynchronize(I->stream));
+  err.alloc(1);
+  err.zero(stream);
+  dstep.alloc(1);
+  dstep.zero(stream);
+  // fault work list: (instance interface, first point of a chunk of 256)
+  std::vector<FaultBatchDev> fd;
+  std::vector<int2> fw;
+  for (auto& I : inst)
+    for (size_t k = 0; k < I->fdev.size(); k++) {
+      FaultBatchDev d{};
+      d.F = I->fdev[k];
+      d.g = I->G;
+      d.f[0] = I->f[0].p;
+      d.f[1] = I->f[1].p;
+      d.mat = I->mat.p;
+      d.hs = I->hs.p;
+      d.dt = I->S.dt;
+      for (int p0 = 0; p0 < d.F.n; p0 += 256) fw.push_back(make_int2((int)fd.size(), p0));
+      fd.push_back(d);
+    }
+  fdescs.upload(fd);
+  fwork.upload(fw);
+  nfwork = (int)fw.size();
+  descs.resize(nst);
+  autotune();
+  HIPCHK(hipStreamSynchronize(stream));
+}
+
+template <typename R>
+Batch<R>::~Batch() {
+  if (stream) (void)hipStreamSynchronize(stream);
+  for (int q = 0; q < 2; q++)
+    if (graphs[q]) (void)hipGraphExecDestroy(graphs[q]);
+  if (stream) (void)hipStreamDestroy(stream);
+}
+
+// per-stage descriptors and the tile work list for tile width tw_, chunk cx_
+template <typename R>
+void Batch<R>::build(int tw_, int cx_) {
+  tw = tw_;
+  cx = cx_;
+  std::vector<int2> wl;
+  for (int st = 0; st < nst; st++) {
+    std::vector<Tiled2D> d;
+    for (auto& I : inst) {
+      const StageDev sd{rk.A[st], rk.B[st], I->S.dt, (st + 1 == nst && sizeof(R) == 8) ? 1 : 0};
+      for (size_t b = 0; b < I->S.blocks.size(); b++) {
+        Tiled2D t2 = I->stage_tile2((int)b, sd);
+        t2.cx = cx;
+        t2.nyt = (t2.hi[1] - t2.lo[1] + tw - 1) / tw;
+        t2.nxc = (t2.hi[0] - t2.lo[0] + cx - 1) / cx;
+        t2.ntiles = t2.nyt * t2.nxc;
+        if (st == 0)
+          for (int tl = 0; tl < t2.ntiles; tl++) wl.push_back(make_int2((int)d.size(), tl));
+        d.push_back(t2);
+      }
+    }
+    descs[st].upload(d);
+  }
+  work.upload(wl);
+  nwork = (int)wl.size();
+  // persistent grid: as many workgroups as the device keeps resident
+  int dev = 0, ncu = 0, occ = 0;
+  HIPCHK(hipGetDevice(&dev));
+  HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  if (tw == 64) HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, bulk_tiled2d_batch<R, R, 3, 64>, 64, 0));
+  else if (tw == 128) HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, bulk_tiled2d_batch<R, R, 3, 128>, 128, 0));
+  else HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, bulk_tiled2d_batch<R, R, 3, 256>, 256, 0));
+  grid = std::max(1, std::min(nwork, ncu * std::max(occ, 1)));
+  for (int q = 0; q < 2; q++)
+    if (graphs[q]) {
+      HIPCHK(hipGraphExecDestroy(graphs[q]));
+      graphs[q] = nullptr;
+    }
+}
+
+template <typename R>
+template <int TW>
+void Batch<R>::launch_bulk_tw(int st) {
+  const Tiled2D* d = descs[st].p;
+  if (sbp == 2) hipLaunchKernelGGL((bulk_tiled2d_batch<R, R, 2, TW>), dim3(grid), dim3(TW), 0, stream, d, work.p, nwork, cur);
+  else if (sbp == 3) hipLaunchKernelGGL((bulk_tiled2d_batch<R, R, 3, TW>), dim3(grid), dim3(TW), 0, stream, d, work.p, nwork, cur);
+  else hipLaunchKernelGGL((bulk_tiled2d_batch<R, R, 4, TW>), dim3(grid), dim3(TW), 0, stream, d, work.p, nwork, cur);
+}
+
+// one RK stage of every instance: the stage of domain::do_rk_stage
+// (domain.cpp:228-305) with the fault update first, as in Solver::do_stage.
+// Stage and front times come from the device step counter with each
+// instance's own dt (t = step*dt + C*dt, front (step+1)*dt), the same
+// expressions as a single run.
+template <typename R>
+void Batch<R>::launch_stage(int st) {
+  const bool last = st + 1 == nst;
+  const FaultStageDev fs{rk.A[st], rk.B[st], 0., 0., 0, -1., dstep.p, rk.C[st], last ? 1 : 0};
+  const int fgrid = std::max(1, std::min(nfwork, 4096));
+  if (nfwork)
+    hipLaunchKernelGGL((fault_batch_kernel<0, R>), dim3(fgrid), dim3(256), 0, stream, fdescs.p, fwork.p, nfwork, fs, cur,
+                       err.p);
+  if (tw == 64) launch_bulk_tw<64>(st);
+  else if (tw == 128) launch_bulk_tw<128>(st);
+  else launch_bulk_tw<256>(st);
+  HIPCHK(hipGetLastError());
+  cur ^= 1;
+}
+
+// time of one stage's bulk launch for a configuration (inputs unchanged:
+// df is saved and restored around the trial launches, f_new is overwritten
+// by the next stage anyway)
+template <typename R>
+double Batch<R>::time_config(int tw_, int cx_) {
+  build(tw_, cx_);
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  const int save = cur;
+  if (tw == 64) launch_bulk_tw<64>(0);
+  else if (tw == 128) launch_bulk_tw<128>(0);
+  else launch_bulk_tw<256>(0);
+  const int reps = 3;
+  HIPCHK(hipEventRecord(e0, stream));
+  for (int k = 0; k < reps; k++) {
+    if (tw == 64) launch_bulk_tw<64>(0);
+    else if (tw == 128) launch_bulk_tw<128>(0);
+    else launch_bulk_tw<256>(0);
+  }
+  HIPCHK(hipEventRecord(e1, stream));
+  HIPCHK(hipEventSynchronize(e1));
+  HIPCHK(hipGetLastError());
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+  HIPCHK(hipEventDestroy(e0));
+  HIPCHK(hipEventDestroy(e1));
+  cur = save;
+  return ms / reps;
+}
+
+template <typename R>
+void Batch<R>::autotune() {
+  // key: device, precision, order, and every instance's grid shape
+  hipDeviceProp_t prop;
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  HIPCHK(hipGetDeviceProperties(&prop, dev));
+  unsigned long long h = 1469598103934665603ULL;
+  auto mix = [&](unsigned long long v) { h = (h ^ v) * 1099511628211ULL; };
+  for (auto& I : inst) {
+    mix((unsigned long long)I->S.nx[0]);
+    mix((unsigned long long)I->S.nx[1]);
+    mix((unsigned long long)I->S.blocks.size());
+  }
+  char key[256];
+  snprintf(key, sizeof(key), "%s:%s:s%d:n%zu:%016llx", prop.gcnArchName, sizeof(R) == 4 ? "fp32" : "fp64", sbp,
+           inst.size(), h);
+  std::string cache;
+  if (const char* e = getenv("FDF_TUNE_CACHE")) cache = e;
+  else if (const char* home = getenv("HOME")) cache = std::string(home) + "/.cache/fdfault_amd_tune.txt";
+  int ftw = 0, fcx = 0;
+  if (const char* e = getenv("FDF_BATCH_TW")) ftw = atoi(e);
+  if (const char* e = getenv("FDF_BATCH_CX")) fcx = atoi(e);
+  if (ftw && fcx) {  // fixed by the caller
+    build(ftw, fcx);
+    return;
+  }
+  if (!cache.empty()) {
+    if (FILE* fh = fopen(cache.c_str(), "r")) {
+      char k[256];
+      int a = 0, c = 0;
+      double ms = 0.;
+      while (fscanf(fh, "%255s %d %d %lf", k, &a, &c, &ms) == 4)
+        if (!strcmp(k, key) && (a == 64 || a == 128 || a == 256) && c > 0) {
+          ftw = a;
+          fcx = c;
+          tune_ms = ms;
+        }
+      fclose(fh);
+    }
+  }
+  if (ftw) {
+    tuned_from_cache = 1;
+    build(ftw, fcx);
+    return;
+  }
+  // save df of every instance, time the candidates, restore
+  std::vector<DBuf<R>> saved(inst.size());
+  for (size_t i = 0; i < inst.size(); i++) {
+    saved[i].alloc(inst[i]->df.n);
+    HIPCHK(hipMemcpyAsync(saved[i].p, inst[i]->df.p, inst[i]->df.n * sizeof(R), hipMemcpyDeviceToDevice, stream));
+  }
+  double best = 1e300;
+  int btw = 256, bcx = 8;
+  for (int a : {64, 128, 256})
+    for (int c : {2, 4, 8, 16, 32}) {
+      const double ms = time_config(a, c);
+      if (ms < best) {
+        best = ms;
+        btw = a;
+        bcx = c;
+      }
+    }
+  for (size_t i = 0; i < inst.size(); i++)
+    HIPCHK(hipMemcpyAsync(inst[i]->df.p, saved[i].p, inst[i]->df.n * sizeof(R), hipMemcpyDeviceToDevice, stream));
+  HIPCHK(hipStreamSynchronize(stream));
+  tune_ms = best;
+  build(btw, bcx);
+  if (!cache.empty())
+    if (FILE* fh = fopen(cache.c_str(), "a")) {
+      fprintf(fh, "%s %d %d %.6f\n", key, btw, bcx, best);
+      fclose(fh);
+    }
+}
+
+template <typename R>
+void Batch<R>::step(int n) {
+  for (int k = 0; k < n; k++) {
+    if (opts.use_graph) {
+      const int par = cur;
+      if (!graphs[par]) {
+        HIPCHK(hipStreamSynchronize(stream));
+        HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        for (int st = 0; st < nst; st++) launch_stage(st);
+        hipLaunchKernelGGL(advance_step, dim3(1), dim3(1), 0, stream, dstep.p);
+        hipGraph_t g;
+        HIPCHK(hipStreamEndCapture(stream, &g));
+        HIPCHK(hipGraphInstantiate(&graphs[par], g, nullptr, nullptr, 0));
+        HIPCHK(hipGraphDestroy(g));
+        cur = par;
+      }
+      if (!dstep_valid) {
+        HIPCHK(hipStreamSynchronize(stream));
+        HIPCHK(hipMemcpy(dstep.p, &cur_step, sizeof(int), hipMemcpyHostToDevice));
+        dstep_valid = true;
+      }
+      HIPCHK(hipGraphLaunch(graphs[par], stream));
+      cur ^= (nst & 1);
+    } else {
+      if (!dstep_valid) {
+        HIPCHK(hipStreamSynchronize(stream));
+        HIPCHK(hipMemcpy(dstep.p, &cur_step, sizeof(int), hipMemcpyHostToDevice));
+        dstep_valid = true;
+      }
+      for (int st = 0; st < nst; st++) launch_stage(st);
+      hipLaunchKernelGGL(advance_step, dim3(1), dim3(1), 0, stream, dstep.p);
+    }
+    cur_step++;
+    if (opts.write_outputs) {
+      sync();
+      for (auto& I : inst) I->write_outputs(cur_step);
+    }
+  }
+  sync();
+}
+
+// wait for the batch stream, check the error flags, and hand the current
+// buffer parity and step to every instance (reads go through them)
+template <typename R>
+void Batch<R>::sync() {
+  HIPCHK(hipStreamSynchronize(stream));
+  int e = 0;
+  HIPCHK(hipMemcpy(&e, err.p, sizeof(int), hipMemcpyDeviceToHost));
+  for (auto& I : inst) {
+    I->cur = cur;
+    I->cur_step = cur_step;
+  }
+  if (e) {
+    int z = 0;
+    HIPCHK(hipMemcpy(err.p, &z, sizeof(int), hipMemcpyHostToDevice));
+    throw Error{"fault solve error flag " + std::to_string(e)};
+  }
+}
+
+template <typename R>
+void Batch<R>::finish() {
+  sync();
+  for (auto& I : inst) I->finish();
+}
+
 }  // namespace fdf
 
 struct fdf_s {
   std::unique_ptr<fdf::SolverBase> solver;
+};
+
+struct fdf_batch_s {
+  std::unique_ptr<fdf::BatchBase> b;
 };
 
 using fdf::Error;
@@ -1121,9 +1621,8 @@ int fdf_finish(fdf_t* h) {
   return guarded([&] { h->solver->finish(); });
 }
 
-int fdf_get_info(fdf_t* h, fdf_info* info) {
-  if (!h || !info) return -2;
-  const fdf::SetupH& S = h->solver->S;
+static void fill_info(const fdf::SolverBase* sv, fdf_info* info) {
+  const fdf::SetupH& S = sv->S;
   info->nt = S.nt;
   info->nstages = S.rk.nstages;
   info->ndim = S.ndim;
@@ -1135,14 +1634,94 @@ int fdf_get_info(fdf_t* h, fdf_info* info) {
   info->sbporder = S.sbporder;
   info->plastic = S.plastic;
   info->nblocks = (int)S.blocks.size();
-  info->step = h->solver->cur_step;
-  info->precision = h->solver->opts.precision;
+  info->step = sv->cur_step;
+  info->precision = sv->opts.precision;
   info->dt = S.dt;
   info->ttot = S.ttot;
   info->cfl = S.cfl;
-  info->t = h->solver->cur_step * S.dt;
-  info->tiled_blocks = h->solver->ntiled;
+  info->t = sv->cur_step * S.dt;
+  info->tiled_blocks = sv->ntiled;
+}
+
+int fdf_get_info(fdf_t* h, fdf_info* info) {
+  if (!h || !info) return -2;
+  fill_info(h->solver.get(), info);
   return 0;
+}
+
+int fdf_open_batch(const char* const* paths, int n, const fdf_opts* opts, fdf_batch_t** out) {
+  if (!paths || n <= 0 || !out) {
+    fdf::g_last_error = "fdf_open_batch: null argument or empty batch";
+    return -2;
+  }
+  *out = nullptr;
+  fdf_opts o{};
+  o.quiet = 1;
+  if (opts) o = *opts;
+  return guarded([&] {
+    std::vector<std::string> v;
+    for (int i = 0; i < n; i++) {
+      if (!paths[i]) throw Error{"fdf_open_batch: null path"};
+      v.push_back(paths[i]);
+    }
+    std::unique_ptr<fdf_batch_s> h(new fdf_batch_s());
+    if (o.precision == FDF_FP32) h->b.reset(new fdf::Batch<float>(v, o));
+    else if (o.precision == FDF_FP64) h->b.reset(new fdf::Batch<double>(v, o));
+    else throw Error{"fdf_open_batch: unknown precision"};
+    *out = h.release();
+  });
+}
+
+void fdf_batch_close(fdf_batch_t* h) { delete h; }
+
+int fdf_batch_size(fdf_batch_t* h) { return h ? h->b->size() : -2; }
+
+int fdf_batch_step(fdf_batch_t* h, int nsteps) {
+  return guarded([&] { h->b->step(nsteps); });
+}
+
+int fdf_batch_get_info(fdf_batch_t* h, int inst, fdf_info* info) {
+  if (!h || !info || inst < 0 || inst >= h->b->size()) return -2;
+  fill_info(h->b->instance(inst), info);
+  return 0;
+}
+
+long fdf_batch_read_iface(fdf_batch_t* h, int inst, int iface, int which, double* host, long n) {
+  long r = -1;
+  int rc = guarded([&] {
+    if (inst < 0 || inst >= h->b->size()) throw Error{"fdf_batch_read_iface: bad instance"};
+    r = h->b->instance(inst)->read_iface(iface, which, host, n);
+  });
+  return rc ? -1 : r;
+}
+
+long fdf_batch_read_front(fdf_batch_t* h, int inst, int iface, double* host, long n) {
+  long r = -1;
+  int rc = guarded([&] {
+    if (inst < 0 || inst >= h->b->size()) throw Error{"fdf_batch_read_front: bad instance"};
+    r = h->b->instance(inst)->read_front(iface, host, n);
+  });
+  return rc ? -1 : r;
+}
+
+int fdf_batch_read_array(fdf_batch_t* h, int inst, int which, double* host, long n) {
+  return guarded([&] {
+    if (inst < 0 || inst >= h->b->size()) throw Error{"fdf_batch_read_array: bad instance"};
+    h->b->instance(inst)->read_array(which, host, n);
+  });
+}
+
+int fdf_batch_tuning(fdf_batch_t* h, int* tile_width, int* chunk, double* stage_ms, int* from_cache) {
+  if (!h) return -2;
+  if (tile_width) *tile_width = h->b->tw;
+  if (chunk) *chunk = h->b->cx;
+  if (stage_ms) *stage_ms = h->b->tune_ms;
+  if (from_cache) *from_cache = h->b->tuned_from_cache;
+  return 0;
+}
+
+int fdf_batch_finish(fdf_batch_t* h) {
+  return guarded([&] { h->b->finish(); });
 }
 
 double fdf_bulk_kernel_ms(fdf_t* h, int* nlaunch) { return h->solver->bulk_ms(nlaunch); }

@@ -43,7 +43,10 @@ class Info(C.Structure):
 
 EXPORTS = ["fdf_open", "fdf_step", "fdf_stage", "fdf_run", "fdf_read_field", "fdf_read_array",
            "fdf_write_array", "fdf_read_iface", "fdf_read_front", "fdf_finish", "fdf_get_info",
-           "fdf_host_setup", "fdf_last_error", "fdf_close", "fdf_bulk_kernel_ms"]
+           "fdf_host_setup", "fdf_last_error", "fdf_close", "fdf_bulk_kernel_ms",
+           "fdf_open_batch", "fdf_batch_step", "fdf_batch_size", "fdf_batch_get_info", "fdf_batch_read_iface",
+           "fdf_batch_read_front", "fdf_batch_read_array", "fdf_batch_tuning", "fdf_batch_finish",
+           "fdf_batch_close"]
 
 _lib = None
 
@@ -76,6 +79,18 @@ def load():
     L.fdf_close.argtypes = [C.c_void_p]
     L.fdf_bulk_kernel_ms.restype = C.c_double
     L.fdf_bulk_kernel_ms.argtypes = [C.c_void_p, ip]
+    L.fdf_open_batch.argtypes = [C.POINTER(C.c_char_p), C.c_int, C.POINTER(Opts), C.POINTER(C.c_void_p)]
+    L.fdf_batch_step.argtypes = [C.c_void_p, C.c_int]
+    L.fdf_batch_size.argtypes = [C.c_void_p]
+    L.fdf_batch_get_info.argtypes = [C.c_void_p, C.c_int, C.POINTER(Info)]
+    L.fdf_batch_read_iface.restype = C.c_long
+    L.fdf_batch_read_iface.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, dp, C.c_long]
+    L.fdf_batch_read_front.restype = C.c_long
+    L.fdf_batch_read_front.argtypes = [C.c_void_p, C.c_int, C.c_int, dp, C.c_long]
+    L.fdf_batch_read_array.argtypes = [C.c_void_p, C.c_int, C.c_int, dp, C.c_long]
+    L.fdf_batch_tuning.argtypes = [C.c_void_p, ip, ip, dp, ip]
+    L.fdf_batch_finish.argtypes = [C.c_void_p]
+    L.fdf_batch_close.argtypes = [C.c_void_p]
     _lib = L
     return L
 
@@ -207,3 +222,84 @@ def run(inpath: str, precision: int = FP64, quiet: bool = True, exact: bool = Fa
     o = Opts(precision, 0, 1, 1 if quiet else 0, 0, 1 if exact else 0)
     if load().fdf_run(inpath.encode(), C.byref(o)) != 0:
         raise _err()
+
+
+class Batch:
+    """An ensemble of independent 2-D mode II problems stepped together
+    (one fault launch and one persistent bulk launch per RK stage for all
+    instances).  Per-instance readback mirrors :class:`Solver`."""
+
+    def __init__(self, inpaths, precision: int = FP32, device: int = 0, write_outputs: bool = False,
+                 quiet: bool = True, graph: bool = True):
+        L = load()
+        self.opts = Opts(precision, device, 1 if write_outputs else 0, 1 if quiet else 0, 1 if graph else 0, 0)
+        arr = (C.c_char_p * len(inpaths))(*[p.encode() for p in inpaths])
+        h = C.c_void_p()
+        if L.fdf_open_batch(arr, len(inpaths), C.byref(self.opts), C.byref(h)) != 0:
+            raise _err()
+        self.h = h
+        self.n = L.fdf_batch_size(self.h)
+        self.infos = []
+        for i in range(self.n):
+            info = Info()
+            L.fdf_batch_get_info(self.h, i, C.byref(info))
+            self.infos.append(info)
+        self.dt = self.infos[0].dt
+        self.nstages = self.infos[0].nstages
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().fdf_batch_close(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step(self, n: int = 1):
+        if load().fdf_batch_step(self.h, n) != 0:
+            raise _err()
+
+    def points(self, i: int):
+        nx = self.infos[i].nx
+        return nx[0] * nx[1] * nx[2]
+
+    def tuning(self):
+        tw, cx, fc = C.c_int(), C.c_int(), C.c_int()
+        ms = C.c_double()
+        load().fdf_batch_tuning(self.h, C.byref(tw), C.byref(cx), C.byref(ms), C.byref(fc))
+        return {"tile_width": tw.value, "chunk": cx.value, "stage_ms": ms.value, "from_cache": bool(fc.value)}
+
+    def iface(self, inst: int, i: int, which: str):
+        L = load()
+        n = L.fdf_batch_read_iface(self.h, inst, i, IFACE_FIELDS[which], None, 0)
+        if n < 0:
+            raise _err()
+        out = np.empty(n)
+        if L.fdf_batch_read_iface(self.h, inst, i, IFACE_FIELDS[which], _dp(out), n) < 0:
+            raise _err()
+        return out
+
+    def front(self, inst: int, i: int):
+        L = load()
+        n = L.fdf_batch_read_front(self.h, inst, i, None, 0)
+        if n < 0:
+            return None
+        out = np.empty(n)
+        L.fdf_batch_read_front(self.h, inst, i, _dp(out), n)
+        return out
+
+    def array(self, inst: int, which: str):
+        i = self.infos[inst]
+        n = {"f": i.nfields_total, "df": i.nfields, "x": i.ndim, "metric": i.ndim * i.ndim, "jac": 1}[which]
+        N = self.points(inst)
+        out = np.empty(n * N)
+        if load().fdf_batch_read_array(self.h, inst, ARRAYS[which], _dp(out), out.size) != 0:
+            raise _err()
+        return out.reshape((n,) + tuple(i.nx))
+
+    def finish(self):
+        if load().fdf_batch_finish(self.h) != 0:
+            raise _err()

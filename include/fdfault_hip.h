@@ -106,6 +106,36 @@ void fdf_close(fdf_t* h);
  * in the reference. */
 double fdf_bulk_kernel_ms(fdf_t* h, int* nlaunch);
 
+/* ---- Ensembles ------------------------------------------------------------
+ * No counterpart in the reference, which runs problem::solve
+ * (src/problem.cpp:136-192) once per .in file, one process each.  A batch
+ * steps n independent 2-D mode II elastic problems together on one GPU: each
+ * instance is set up as fdf_open would (same outputs and front files), then
+ * every RK stage of every instance runs as one fault launch and one
+ * persistent bulk launch.  Instances must share sbporder, rkorder and dt;
+ * their grid sizes may differ.  The bulk kernel's tile width and x-chunk are
+ * chosen at open by timing candidates (cached in $FDF_TUNE_CACHE or
+ * ~/.cache/fdfault_amd_tune.txt; FDF_BATCH_TW/FDF_BATCH_CX fix them). */
+typedef struct fdf_batch_s fdf_batch_t;
+
+int fdf_open_batch(const char* const* in_paths, int n, const fdf_opts* opts, fdf_batch_t** out);
+/* nsteps time steps of every instance (problem::solve's loop body, per
+ * instance: stages, outputs and fronts). */
+int fdf_batch_step(fdf_batch_t* h, int nsteps);
+int fdf_batch_size(fdf_batch_t* h);
+/* Per-instance readback, with fdf_get_info / fdf_read_iface /
+ * fdf_read_front / fdf_read_array semantics for instance `inst`. */
+int fdf_batch_get_info(fdf_batch_t* h, int inst, fdf_info* info);
+long fdf_batch_read_iface(fdf_batch_t* h, int inst, int iface, int which, double* host, long n);
+long fdf_batch_read_front(fdf_batch_t* h, int inst, int iface, double* host, long n);
+int fdf_batch_read_array(fdf_batch_t* h, int inst, int which, double* host, long n);
+/* The autotuner's choice: tile width (y points per workgroup), x planes per
+ * chunk, the timed bulk-stage time (ms) and whether it came from the cache. */
+int fdf_batch_tuning(fdf_batch_t* h, int* tile_width, int* chunk, double* stage_ms, int* from_cache);
+/* fdf_finish for every instance. */
+int fdf_batch_finish(fdf_batch_t* h);
+void fdf_batch_close(fdf_batch_t* h);
+
 #ifdef __cplusplus
 }
 #endif

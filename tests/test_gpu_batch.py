@@ -1,0 +1,167 @@
+"""2-D rectilinear tiled kernel and the ensemble (batch) path on the GPU.
+
+* The 2-D tiled kernel (register window along x, LDS row along y, compact
+  face SAT) against the bit-faithful FP64 path and the oracle: within 1e-10
+  over a long rupture, 1e-12 for one stage from a random state.
+* Tile shape changes only the work split: results are bit-identical for any
+  tile width / x-chunk.
+* A batch of instances of different sizes gives, per instance, bit-identical
+  results to single runs (same kernels, same arithmetic per point), in FP32
+  and FP64, eager and graph-replayed; the autotuner's cache is reused.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from cases import write_case
+from fdfault_amd.problems import Pert
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def _nuc(nx, dx):
+    Lx = (nx - 1) * dx
+    return [Pert("constant", 0., 0., 0., 0., 0., (-120., 70., 0.)),
+            Pert("boxcar", 0., 0.5 * Lx, 0.6, 0., 0., (0., 11.6, 0.))]
+
+
+def _case(tmp, name, nx, ny, nt=40, dx=0.1, **kw):
+    kw.setdefault("s0", (0., 0., 0., 0., 0., 0.))
+    return write_case(str(tmp), name, nx=nx, ny=ny, dx=dx, nt=nt, loads=_nuc(nx, dx), front=("V", 0.001), **kw)
+
+
+def test_mode2_fast_path_drift_long_run(tmp_path):
+    """2-D tiled kernel (block-constant metric, fused multiply-adds) against
+    the bit-faithful FP64 path over 300 steps of a rupture: within 1e-10."""
+    from fdfault_amd import lib
+    p = _case(tmp_path, "drift2", 121, 61, nt=300)
+    a = lib.Solver(p, exact=True)
+    b = lib.Solver(p, exact=False)
+    assert b.info.tiled_blocks == 2 and a.info.tiled_blocks == 0
+    a.step(300)
+    b.step(300)
+    for w in ("U", "V", "S"):
+        r = _rel(b.iface(0, w), a.iface(0, w))
+        print(w, "fast vs exact", r, flush=True)
+        assert r <= 1e-10, (w, r)
+    assert _rel(b.array("f"), a.array("f")) <= 1e-10
+    assert np.abs(a.front(0) - b.front(0)).max() <= a.dt * 1e-6
+    a.close()
+    b.close()
+
+
+@pytest.mark.parametrize("sbp", [2, 3, 4])
+def test_kernel_level_random_state_2d_tiled(tmp_path, sbp):
+    """Every branch of the 2-D tiled kernel at once (interior, one-sided rows
+    in x and y, four external faces, the interface) from a random state."""
+    from fdfault_amd import lib
+    p = write_case(str(tmp_path), "rnd2", nx=77, ny=45, dx=0.1, nt=2, sbporder=sbp)
+    o = O.Oracle(p)
+    g = lib.Solver(p)
+    assert g.info.tiled_blocks == 2
+    rng = np.random.default_rng(5)
+    f = rng.normal(size=o.array("f").shape)
+    df = rng.normal(size=o.array("df").shape)
+    for h in (o, g):
+        h.set_array("f", f)
+        h.set_array("df", df)
+    for st in range(o.nstages):
+        o.stage(0, st)
+        g.stage(0, st)
+        assert _rel(g.array("df"), o.array("df")) <= 1e-12
+        assert _rel(g.array("f"), o.array("f")) <= 1e-12
+    g.close()
+    o.close()
+
+
+def test_tile_shape_does_not_change_results(tmp_path, monkeypatch):
+    from fdfault_amd import lib
+    p = _case(tmp_path, "shape", 97, 53, nt=20)
+    res = []
+    for tw, cx in ((64, 1), (128, 3), (256, 8)):
+        monkeypatch.setenv("FDF_TILE2_TW", str(tw))
+        monkeypatch.setenv("FDF_TILE2_CX", str(cx))
+        g = lib.Solver(p, precision=lib.FP32)
+        g.step(20)
+        res.append((g.array("f"), g.iface(0, "U"), g.front(0)))
+        g.close()
+    for r in res[1:]:
+        assert np.array_equal(r[0], res[0][0])
+        assert np.array_equal(r[1], res[0][1])
+        assert np.array_equal(r[2], res[0][2])
+
+
+SIZES = [(61, 31), (97, 45), (45, 27), (130, 70)]
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+@pytest.mark.parametrize("graph", [False, True])
+def test_batch_matches_single_runs(tmp_path, monkeypatch, precision, graph):
+    from fdfault_amd import lib
+    prec = lib.FP32 if precision == "fp32" else lib.FP64
+    monkeypatch.setenv("FDF_TUNE_CACHE", str(tmp_path / "tune.txt"))
+    paths = [_case(tmp_path, "b%d" % i, nx, ny, nt=30) for i, (nx, ny) in enumerate(SIZES)]
+    print("batch", precision, "graph", graph, flush=True)
+    b = lib.Batch(paths, precision=prec, graph=graph)
+    t = b.tuning()
+    print("tuning", t, flush=True)
+    assert t["tile_width"] in (64, 128, 256) and t["chunk"] > 0
+    for k in range(3):
+        b.step(10)
+    monkeypatch.setenv("FDF_TILE2_TW", str(t["tile_width"]))
+    monkeypatch.setenv("FDF_TILE2_CX", str(t["chunk"]))
+    for i, p in enumerate(paths):
+        s = lib.Solver(p, precision=prec)
+        s.step(30)
+        assert np.array_equal(b.array(i, "f"), s.array("f")), i
+        for w in ("U", "V", "S", "Sn"):
+            assert np.array_equal(b.iface(i, 0, w), s.iface(0, w)), (i, w)
+        assert np.array_equal(b.front(i, 0), s.front(0)), i
+        assert b.infos[i].nx[0] == SIZES[i][0]
+        s.close()
+    b.close()
+
+
+def test_batch_fp64_instance_matches_oracle(tmp_path, monkeypatch):
+    from fdfault_amd import lib
+    monkeypatch.setenv("FDF_TUNE_CACHE", str(tmp_path / "tune.txt"))
+    paths = [_case(tmp_path, "o%d" % i, nx, ny, nt=25) for i, (nx, ny) in enumerate(SIZES[:2])]
+    b = lib.Batch(paths, precision=lib.FP64, graph=True)
+    b.step(25)
+    for i, p in enumerate(paths):
+        o = O.Oracle(p)
+        o.step(25)
+        for w in ("U", "V", "S"):
+            assert _rel(b.iface(i, 0, w), o.iface(0, w)) <= 1e-10, (i, w)
+        assert _rel(b.array(i, "f"), o.array("f")) <= 1e-10
+        o.close()
+    b.close()
+
+
+def test_batch_autotune_cache_reused(tmp_path, monkeypatch):
+    from fdfault_amd import lib
+    monkeypatch.setenv("FDF_TUNE_CACHE", str(tmp_path / "tune.txt"))
+    paths = [_case(tmp_path, "c%d" % i, nx, ny, nt=5) for i, (nx, ny) in enumerate(SIZES[:3])]
+    a = lib.Batch(paths, precision=lib.FP32)
+    ta = a.tuning()
+    a.close()
+    assert not ta["from_cache"]
+    b = lib.Batch(paths, precision=lib.FP32)
+    tb = b.tuning()
+    b.close()
+    assert tb["from_cache"] and (tb["tile_width"], tb["chunk"]) == (ta["tile_width"], ta["chunk"])
+    assert os.path.getsize(str(tmp_path / "tune.txt")) > 0
+
+
+def test_batch_rejects_unsupported_instance(tmp_path):
+    from fdfault_amd import lib
+    good = _case(tmp_path, "g", 61, 31, nt=5)
+    diss = _case(tmp_path, "d", 61, 31, nt=5, cdiss=0.1)
+    with pytest.raises(lib.FdfaultError, match="not rectilinear"):
+        lib.Batch([good, diss], precision=lib.FP32)
