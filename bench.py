@@ -67,6 +67,72 @@ def cpu_baseline(sample_in, steps, threads):
     return mpts, n
 
 
+def ensemble(args):
+    """Batched ensemble of uneven 2-D ruptures: ruptures/hour, whole-ensemble
+    Mpt-updates/s and, with --check-error, each instance's final-slip and
+    rupture-front error against a bit-faithful FP64 single run."""
+    from fdfault_amd import lib
+    from fdfault_amd.ensemble import ensemble_workload
+    prec = lib.FP64 if args.precision == "fp64" else lib.FP32
+    w = 8 if prec == lib.FP64 else 4
+    with tempfile.TemporaryDirectory() as d:
+        paths, shapes = ensemble_workload(d, n=args.instances, nt=args.ens_nt)
+        print("ensemble written: %d instances, %d points" % (len(paths), sum(a * b for a, b in shapes)), flush=True)
+        t0 = time.perf_counter()
+        b = lib.Batch(paths, precision=prec, graph=True)
+        topen = time.perf_counter() - t0
+        tun = b.tuning()
+        print("opened in %.1f s, tuning %s" % (topen, tun), flush=True)
+        b.step(args.warmup)
+        t0 = time.perf_counter()
+        b.step(args.steps)
+        t = time.perf_counter() - t0
+        npts = sum(b.points(i) for i in range(b.n))
+        nst = b.nstages
+        stage_ms = t / args.steps / nst * 1e3
+        B = bytes_per_point(5, w)
+        achieved = B * npts / (stage_ms * 1e-3) / 1e9
+        rph = b.n / (t / args.steps * args.ens_nt) * 3600.
+        err = None
+        if args.check_error:
+            b.step(args.err_steps - args.warmup - args.steps)
+            worst_slip, worst_front, unmatched = 0., 0., 0
+            per = []
+            for i, p in enumerate(paths):
+                s = lib.Solver(p, precision=lib.FP64, exact=True)
+                s.step(args.err_steps)
+                u64, u32 = s.iface(0, "U"), b.iface(i, 0, "U")
+                es = float(np.abs(u32 - u64).max() / max(np.abs(u64).max(), 1e-300))
+                f64, f32 = s.front(0), b.front(i, 0)
+                both = (f64 >= 0) & (f32 >= 0)
+                ef = float(np.abs(f64[both] - f32[both]).max() / s.dt) if both.any() else 0.
+                unmatched += int(((f64 >= 0) != (f32 >= 0)).sum())
+                per.append({"slip_rel": es, "front_steps": ef})
+                print("instance %d: slip err %.3g, front err %.3g steps" % (i, es, ef), flush=True)
+                worst_slip, worst_front = max(worst_slip, es), max(worst_front, ef)
+                s.close()
+            err = {"steps": args.err_steps, "max_final_slip_rel_err": worst_slip,
+                   "max_front_err_steps": worst_front, "front_points_ruptured_in_one_only": unmatched,
+                   "reference": "FP64 bit-faithful single runs", "per_instance": per}
+        b.close()
+    line = {
+        "metric": "ruptures/hour (batched ensemble of uneven 2-D ruptures)",
+        "value": round(rph, 1), "unit": "ruptures/hour", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(t / args.steps * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64" if w == 8 else "f32", "data": "synthetic",
+        "config": {"workload": "ensemble_2d_slipweak_%d_instances" % len(paths), "instances": len(paths),
+                   "points": npts, "nt_per_rupture": args.ens_nt, "sbporder": 3, "precision": args.precision,
+                   "tile_width": tun["tile_width"], "x_chunk": tun["chunk"], "tuned_from_cache": tun["from_cache"]},
+        "mpt_updates_per_s": round(npts * nst * args.steps / t / 1e6, 1),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "bulk_tiled2d_batch + fault_batch_kernel (whole stage)",
+                     "stage_ms": round(stage_ms, 4), "bytes_per_point": B},
+        "error_vs_fp64": err,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -79,11 +145,21 @@ def main():
     ap.add_argument("--nz", type=int, default=201)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--workload", choices=["planar3d", "ensemble"], default="planar3d")
+    ap.add_argument("--instances", type=int, default=64)
+    ap.add_argument("--ens-nt", type=int, default=2000, help="time steps of one rupture (ensemble)")
+    ap.add_argument("--check-error", action="store_true", help="ensemble: compare with FP64 single runs")
+    ap.add_argument("--err-steps", type=int, default=2000)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU only (the whole grid fits one MI355X); "
               "multi-GPU slab decomposition is designed but not built", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "ensemble":
+        if "--precision" not in sys.argv:
+            args.precision = "fp32"
+        ensemble(args)
+        return
     from fdfault_amd import lib
     prec = lib.FP32 if args.precision == "fp32" else lib.FP64
     w = 4 if prec == lib.FP32 else 8
