@@ -19,6 +19,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "mlp.hpp"
+
 namespace fdf {
 
 // SBP tables in constant memory: row 0 interior, rows 1.. boundary rows
@@ -495,6 +497,14 @@ struct FaultDev {
   double* front;           // [n] or null
   double front_value;
   int front_field;
+  // STZ root: learned proposal (MLP parameters, null = plain solve only),
+  // solve log rows (phi, sn, chi, eta, log V) and counters
+  // [solves, Newton iterations, proposals used, fallbacks to the plain solve]
+  const float* mlp;
+  double* log;
+  unsigned long long* log_n;
+  long long log_cap;
+  unsigned long long* stats;
 };
 
 struct FaultStageDev {
@@ -540,31 +550,14 @@ __device__ __forceinline__ double stz_df(double mu, const double* p) {
   return p[1] * d - p[2];
 }
 
-// bracketed Newton with bisection fallback, utilities.cpp:89-180; err codes:
-// 1 zero derivative, 2 no convergence, 4 overflow repair did not terminate
-__device__ double stz_newton(double xmin, double xmax, const double* p, int* err, int* iters) {
+// the safeguarded Newton iteration of utilities.cpp:89-180 from x with
+// bracket [xl, xh] (f(xl) < 0 < f(xh)): a Newton step unless it leaves the
+// bracket or converges too slowly, else bisection; stops when |dx| < 2e-16
+__device__ __forceinline__ double stz_rtsafe(double x, double xl, double xh, const double* p, int* err, int* iters) {
   int i = 0;
   const int nmax = 100;
   const double tol = 2.e-16;
-  double x = 0.5 * (xmin + xmax), xh = xmax, xl = xmin;
   double func = stz_f(x, p);
-  const double fh = stz_f(xh, p);
-  if (isinf(fh)) {
-    int guard = 0;
-    while (true) {
-      if (!isinf(func)) {
-        if (func > 0.) { xh = x; break; }
-        xl = x;
-      } else {
-        xh = x;
-      }
-      x = 0.5 * (xh + xl);
-      func = stz_f(x, p);
-      if (++guard > 4000) { *err |= 4; break; }
-    }
-  }
-  x = 0.5 * (xh + xl);
-  func = stz_f(x, p);
   double der = stz_df(x, p);
   double dx = fabs(xh - xl), dxold = dx;
   while (i < nmax) {
@@ -589,6 +582,48 @@ __device__ double stz_newton(double xmin, double xmax, const double* p, int* err
   }
   if (i == nmax) *err |= 2;
   *iters = i;
+  return x;
+}
+
+// bracketed Newton with bisection fallback, utilities.cpp:89-180; err codes:
+// 1 zero derivative, 2 no convergence, 4 overflow repair did not terminate
+__device__ double stz_newton(double xmin, double xmax, const double* p, int* err, int* iters) {
+  double x = 0.5 * (xmin + xmax), xh = xmax, xl = xmin;
+  double func = stz_f(x, p);
+  const double fh = stz_f(xh, p);
+  if (isinf(fh)) {
+    int guard = 0;
+    while (true) {
+      if (!isinf(func)) {
+        if (func > 0.) { xh = x; break; }
+        xl = x;
+      } else {
+        xh = x;
+      }
+      x = 0.5 * (xh + xl);
+      func = stz_f(x, p);
+      if (++guard > 4000) { *err |= 4; break; }
+    }
+  }
+  x = 0.5 * (xh + xl);
+  return stz_rtsafe(x, xl, xh, p, err, iters);
+}
+
+// The same root from a proposal x0 inside the bracket: the bracket shrinks to
+// the side of x0 that keeps the sign change, then the same iteration runs.
+// Accepted only if the result stays finite, inside the bracket, and has a
+// residual |f| <= 1e-12 |phi|; otherwise *err is set and the caller runs the
+// plain solve.
+__device__ double stz_newton_from(double xmin, double xmax, double x0, const double* p, int* err, int* iters) {
+  *iters = 0;
+  if (!(x0 > xmin && x0 < xmax)) { *err |= 32; return x0; }
+  const double f0 = stz_f(x0, p);
+  if (!isfinite(f0)) { *err |= 32; return x0; }
+  double xl = xmin, xh = xmax;
+  if (f0 < 0.) xl = x0; else xh = x0;
+  int e = 0;
+  const double x = stz_rtsafe(x0, xl, xh, p, &e, iters);
+  if (e || !isfinite(x) || x < xmin || x > xmax || !(fabs(stz_f(x, p)) <= 1e-12 * fabs(p[0]))) *err |= 32;
   return x;
 }
 
@@ -758,10 +793,39 @@ __device__ __forceinline__ void fault_point(const FaultDev& F, const GridDev& g,
           if (pr[7] >= -phi / snc) {
             mu = -phi / snc;
           } else {
-            int e = 0, it = 0;
-            mu = stz_newton(pr[7], -phi / snc, pr, &e, &it);
+            int e = 0, it = 0, used = 0, fell = 0;
+            if (F.mlp) {  // learned proposal for log V, then mu = (phi - eta V)/|sn|
+              const double lv = mlp_logv(F.mlp, phi, snc, pr[3], eta);
+              const double mu0 = (phi - eta * exp(lv)) / (-snc);
+              int e1 = 0;
+              mu = stz_newton_from(pr[7], -phi / snc, mu0, pr, &e1, &it);
+              used = !e1;
+              fell = e1 ? 1 : 0;
+            }
+            if (!used) {
+              int it2 = 0;
+              mu = stz_newton(pr[7], -phi / snc, pr, &e, &it2);
+              it += it2;
+            }
             if (e) atomicOr(err, e);
             if (newton_iters) atomicAdd(newton_iters, it);
+            if (F.stats) {
+              atomicAdd(&F.stats[0], 1ULL);
+              atomicAdd(&F.stats[1], (unsigned long long)it);
+              if (used) atomicAdd(&F.stats[2], 1ULL);
+              if (fell) atomicAdd(&F.stats[3], 1ULL);
+            }
+            if (F.log) {  // training pair of the plain method's root
+              const unsigned long long r = atomicAdd(F.log_n, 1ULL);
+              if ((long long)r < F.log_cap) {
+                double* row = F.log + 5 * r;
+                row[0] = phi;
+                row[1] = snc;
+                row[2] = pr[3];
+                row[3] = eta;
+                row[4] = log(stz_vpl(mu, pr));
+              }
+            }
           }
         }
         if (mu * fabs(snc) > phi) { V = 0.; S = phi; }

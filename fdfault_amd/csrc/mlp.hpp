@@ -1,0 +1,198 @@
+// Learned proposal for the STZ slip-rate root: a 4 -> 64 -> 64 -> 1 tanh MLP
+// (FP32) that maps (locked shear traction phi, normal stress sn, effective
+// temperature chi, radiation-damping impedance eta) to log(V), V the slip
+// rate of the root of stz::calc_mu (stz.cpp:263-356).  Inference runs inside
+// the fault kernel (one point per thread); training runs on the GPU with the
+// kernels below (per-sample forward and deltas, weight gradients as
+// reductions over the batch, Adam).  Nothing here exists in the reference.
+//
+// Parameter vector (floats), in this order:
+//   W1[64][4] b1[64] W2[64][64] b2[64] W3[64] b3
+//   in_mean[4] in_std[4] out_mean out_std          (normalisation)
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace fdf {
+
+constexpr int MLP_IN = 4, MLP_H = 64;
+constexpr int MLP_W1 = 0, MLP_B1 = MLP_W1 + MLP_H * MLP_IN, MLP_W2 = MLP_B1 + MLP_H, MLP_B2 = MLP_W2 + MLP_H * MLP_H,
+              MLP_W3 = MLP_B2 + MLP_H, MLP_B3 = MLP_W3 + MLP_H, MLP_NW = MLP_B3 + 1;  // 4545 trained parameters
+constexpr int MLP_NORM = MLP_NW, MLP_NP = MLP_NORM + 2 * MLP_IN + 2;      // + normalisation
+
+// normalised network output for one normalised input (FP32)
+__device__ __forceinline__ float mlp_eval(const float* __restrict__ P, const float* x) {
+  float h1[MLP_H];
+#pragma unroll
+  for (int j = 0; j < MLP_H; j++) {
+    float s = P[MLP_B1 + j];
+#pragma unroll
+    for (int i = 0; i < MLP_IN; i++) s += P[MLP_W1 + j * MLP_IN + i] * x[i];
+    h1[j] = tanhf(s);
+  }
+  float y = P[MLP_B3];
+  for (int j = 0; j < MLP_H; j++) {
+    float s = P[MLP_B2 + j];
+#pragma unroll
+    for (int k = 0; k < MLP_H; k++) s += P[MLP_W2 + j * MLP_H + k] * h1[k];
+    y += P[MLP_W3 + j] * tanhf(s);
+  }
+  return y;
+}
+
+// log(V) proposed for raw inputs (phi, sn, chi, eta)
+__device__ __forceinline__ double mlp_logv(const float* __restrict__ P, double phi, double sn, double chi, double eta) {
+  const double raw[MLP_IN] = {phi, sn, chi, eta};
+  float x[MLP_IN];
+#pragma unroll
+  for (int i = 0; i < MLP_IN; i++)
+    x[i] = (float)((raw[i] - (double)P[MLP_NORM + i]) / (double)P[MLP_NORM + MLP_IN + i]);
+  const float y = mlp_eval(P, x);
+  return (double)y * (double)P[MLP_NORM + 2 * MLP_IN + 1] + (double)P[MLP_NORM + 2 * MLP_IN];
+}
+
+// ---- training ---------------------------------------------------------------
+// Buffers (B = batch size): X [n][4] normalised inputs and T [n] normalised
+// targets (a batch picks rows through idx), H1/H2 [B][64] activations and
+// D1/D2 [B][64] deltas of the pre-activations (sample-major: at a fixed
+// sample the 64 lanes of a wave read 64 consecutive floats), DY [B] output
+// delta.  Loss = mean over the batch of (y - t)^2 / 2.
+
+// forward and backward deltas, one sample per thread
+__global__ void __launch_bounds__(256) mlp_fwd_bwd_kernel(const float* __restrict__ P, const float* __restrict__ X,
+                                                          const float* __restrict__ T, const int* __restrict__ idx,
+                                                          int B, float* __restrict__ H1, float* __restrict__ H2,
+                                                          float* __restrict__ D1, float* __restrict__ D2,
+                                                          float* __restrict__ DY, float* __restrict__ loss) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  float l = 0.f;
+  if (b < B) {
+    const int s = idx ? idx[b] : b;
+    float x[MLP_IN];
+#pragma unroll
+    for (int i = 0; i < MLP_IN; i++) x[i] = X[(long long)s * MLP_IN + i];
+    float h1[MLP_H], h2[MLP_H];
+#pragma unroll
+    for (int j = 0; j < MLP_H; j++) {
+      float a = P[MLP_B1 + j];
+#pragma unroll
+      for (int i = 0; i < MLP_IN; i++) a += P[MLP_W1 + j * MLP_IN + i] * x[i];
+      h1[j] = tanhf(a);
+      H1[(long long)b * MLP_H + j] = h1[j];
+    }
+    float y = P[MLP_B3];
+    for (int j = 0; j < MLP_H; j++) {
+      float a = P[MLP_B2 + j];
+#pragma unroll
+      for (int k = 0; k < MLP_H; k++) a += P[MLP_W2 + j * MLP_H + k] * h1[k];
+      h2[j] = tanhf(a);
+      H2[(long long)b * MLP_H + j] = h2[j];
+      y += P[MLP_W3 + j] * h2[j];
+    }
+    const float r = y - T[s];
+    l = 0.5f * r * r;
+    const float dy = r / (float)B;
+    DY[b] = dy;
+    float d2[MLP_H];
+#pragma unroll
+    for (int j = 0; j < MLP_H; j++) {
+      d2[j] = dy * P[MLP_W3 + j] * (1.f - h2[j] * h2[j]);
+      D2[(long long)b * MLP_H + j] = d2[j];
+    }
+    for (int k = 0; k < MLP_H; k++) {
+      float g = 0.f;
+#pragma unroll
+      for (int j = 0; j < MLP_H; j++) g += d2[j] * P[MLP_W2 + j * MLP_H + k];
+      D1[(long long)b * MLP_H + k] = g * (1.f - h1[k] * h1[k]);
+    }
+  }
+  // batch loss (sum; the host divides by B)
+  __shared__ float red[256];
+  red[threadIdx.x] = l;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && loss) atomicAdd(loss, red[0]);
+}
+
+// weight gradients: one workgroup per output group, reduction over the batch
+//   G[W2 j k] = sum_b D2[b][j] H1[b][k]    G[b2 j] = sum_b D2[b][j]
+//   G[W1 j i] = sum_b D1[b][j] X[b][i]     G[b1 j] = sum_b D1[b][j]
+//   G[W3 j]   = sum_b DY[b] H2[b][j]       G[b3]   = sum_b DY[b]
+// workgroup g < 64: row j = g of W2 (64 outputs + b2); g == 64: W1, b1, W3, b3
+__global__ void __launch_bounds__(256) mlp_grad_kernel(const float* __restrict__ X, const int* __restrict__ idx, int B,
+                                                       const float* __restrict__ H1, const float* __restrict__ H2,
+                                                       const float* __restrict__ D1, const float* __restrict__ D2,
+                                                       const float* __restrict__ DY, float* __restrict__ G) {
+  const int g = blockIdx.x, t = threadIdx.x;
+  __shared__ float part[4][MLP_H + 1];
+  if (g < MLP_H) {
+    // 4 waves split the batch; lane k of each wave owns output (j, k)
+    const int k = t & 63, wv = t >> 6;
+    float s = 0.f, sb = 0.f;
+    for (int b = wv; b < B; b += 4) {
+      const float d = D2[(long long)b * MLP_H + g];
+      s += d * H1[(long long)b * MLP_H + k];
+      sb += d;
+    }
+    part[wv][k] = s;
+    if (k == 0) part[wv][MLP_H] = sb;
+    __syncthreads();
+    if (t < MLP_H) G[MLP_W2 + g * MLP_H + t] = part[0][t] + part[1][t] + part[2][t] + part[3][t];
+    if (t == 0) G[MLP_B2 + g] = part[0][MLP_H] + part[1][MLP_H] + part[2][MLP_H] + part[3][MLP_H];
+  } else {
+    // thread t < 64: W1 row t and b1 t; 64 <= t < 128: W3 (t-64); t == 128: b3
+    if (t < MLP_H) {
+      float w[MLP_IN] = {0.f, 0.f, 0.f, 0.f}, sb = 0.f;
+      for (int b = 0; b < B; b++) {
+        const float d = D1[(long long)b * MLP_H + t];
+        const int s = idx ? idx[b] : b;
+#pragma unroll
+        for (int i = 0; i < MLP_IN; i++) w[i] += d * X[(long long)s * MLP_IN + i];
+        sb += d;
+      }
+#pragma unroll
+      for (int i = 0; i < MLP_IN; i++) G[MLP_W1 + t * MLP_IN + i] = w[i];
+      G[MLP_B1 + t] = sb;
+    } else if (t < 2 * MLP_H) {
+      const int j = t - MLP_H;
+      float s = 0.f;
+      for (int b = 0; b < B; b++) s += DY[b] * H2[(long long)b * MLP_H + j];
+      G[MLP_W3 + j] = s;
+    } else if (t == 2 * MLP_H) {
+      float s = 0.f;
+      for (int b = 0; b < B; b++) s += DY[b];
+      G[MLP_B3] = s;
+    }
+  }
+}
+
+// Adam (Kingma & Ba 2015) on the trained parameters; step counts from 1
+__global__ void mlp_adam_kernel(float* __restrict__ P, const float* __restrict__ G, float* __restrict__ M,
+                                float* __restrict__ V, float lr, float b1, float b2, float eps, int step) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= MLP_NW) return;
+  const float g = G[i];
+  const float m = b1 * M[i] + (1.f - b1) * g;
+  const float v = b2 * V[i] + (1.f - b2) * g * g;
+  M[i] = m;
+  V[i] = v;
+  const float mh = m / (1.f - powf(b1, (float)step));
+  const float vh = v / (1.f - powf(b2, (float)step));
+  P[i] -= lr * mh / (sqrtf(vh) + eps);
+}
+
+// normalised predictions for a set of normalised inputs
+__global__ void mlp_predict_kernel(const float* __restrict__ P, const float* __restrict__ X, int n,
+                                   float* __restrict__ Y) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= n) return;
+  float x[MLP_IN];
+#pragma unroll
+  for (int i = 0; i < MLP_IN; i++) x[i] = X[(long long)b * MLP_IN + i];
+  Y[b] = mlp_eval(P, x);
+}
+
+}  // namespace fdf

@@ -7,8 +7,10 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <algorithm>
 #include <ctime>
 #include <memory>
+#include <random>
 #include <type_traits>
 #include <string>
 #include <vector>
@@ -135,6 +137,17 @@ class Solver : public SolverBase {
   };
   std::vector<IfBufs> ibufs;
   std::vector<FaultDev> fdev;
+  // STZ learned proposal, solve log and counters (shared by all STZ interfaces)
+  DBuf<float> stz_model;
+  DBuf<double> stz_log;
+  DBuf<unsigned long long> stz_cnt;  // [log rows, solves, iterations, proposals used, fallbacks]
+  void stz_refresh();
+ public:
+  void stz_set_model(const float* params, int n);
+  void stz_enable_log(long cap);
+  long stz_read_log(double* out, long cap);
+  void stz_stats(unsigned long long out[4]);
+ private:
   DBuf<int> err;
   DBuf<double> stage_buf;
   std::vector<OutFile> outs;
@@ -475,6 +488,13 @@ Solver<R>::Solver(const std::string& path, const fdf_opts& o) {
       F.front = ib.front.p;
       F.front_value = S.front_value;
       F.front_field = S.front_field;
+    }
+    if (I.law == LAW_STZ) {
+      if (!stz_cnt.p) {
+        stz_cnt.alloc(5);
+        stz_cnt.zero(stream);
+      }
+      F.stats = stz_cnt.p + 1;
     }
     fdev.push_back(F);
     // register this interface on the two blocks' face lists (interface order)
@@ -1518,6 +1538,178 @@ void Batch<R>::finish() {
   for (auto& I : inst) I->finish();
 }
 
+// ---- STZ learned proposal: model, log, counters --------------------------
+template <typename R>
+void Solver<R>::stz_refresh() {
+  for (size_t i = 0; i < fdev.size(); i++) {
+    if (S.ifaces[i].law != LAW_STZ) continue;
+    fdev[i].mlp = stz_model.p;
+    fdev[i].log = stz_log.p;
+    fdev[i].log_n = stz_log.p ? stz_cnt.p : nullptr;
+    fdev[i].log_cap = (long long)(stz_log.n / 5);
+  }
+  for (int q = 0; q < 2; q++)  // captured launches hold the old arguments
+    if (graphs[q]) {
+      HIPCHK(hipGraphExecDestroy(graphs[q]));
+      graphs[q] = nullptr;
+    }
+}
+
+template <typename R>
+void Solver<R>::stz_set_model(const float* params, int n) {
+  HIPCHK(hipStreamSynchronize(stream));
+  if (!params) {
+    stz_model.free();
+  } else {
+    if (n != MLP_NP) throw Error{"stz model: expected " + std::to_string(MLP_NP) + " parameters"};
+    stz_model.upload(params, (size_t)n);
+  }
+  stz_refresh();
+}
+
+template <typename R>
+void Solver<R>::stz_enable_log(long cap) {
+  HIPCHK(hipStreamSynchronize(stream));
+  if (!stz_cnt.p) throw Error{"stz log: no STZ interface"};
+  if (cap <= 0) stz_log.free();
+  else stz_log.alloc((size_t)cap * 5);
+  const unsigned long long z = 0;
+  HIPCHK(hipMemcpy(stz_cnt.p, &z, sizeof(z), hipMemcpyHostToDevice));
+  stz_refresh();
+}
+
+template <typename R>
+long Solver<R>::stz_read_log(double* out, long cap) {
+  HIPCHK(hipStreamSynchronize(stream));
+  if (!stz_cnt.p || !stz_log.p) return 0;
+  unsigned long long n = 0;
+  HIPCHK(hipMemcpy(&n, stz_cnt.p, sizeof(n), hipMemcpyDeviceToHost));
+  long rows = (long)std::min<unsigned long long>(n, (unsigned long long)(stz_log.n / 5));
+  if (out) {
+    rows = std::min(rows, cap);
+    HIPCHK(hipMemcpy(out, stz_log.p, (size_t)rows * 5 * sizeof(double), hipMemcpyDeviceToHost));
+  }
+  return rows;
+}
+
+// counters since the last call: solves, Newton iterations, proposals used,
+// fallbacks to the plain solve
+template <typename R>
+void Solver<R>::stz_stats(unsigned long long out[4]) {
+  HIPCHK(hipStreamSynchronize(stream));
+  for (int q = 0; q < 4; q++) out[q] = 0;
+  if (!stz_cnt.p) return;
+  HIPCHK(hipMemcpy(out, stz_cnt.p + 1, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+  const unsigned long long z[4] = {0, 0, 0, 0};
+  HIPCHK(hipMemcpy(stz_cnt.p + 1, z, sizeof(z), hipMemcpyHostToDevice));
+}
+
+// ---- MLP training on the GPU ---------------------------------------------
+struct MlpBufs {
+  DBuf<float> P, G, M, V, X, T, H1, H2, D1, D2, DY, loss;
+  DBuf<int> idx;
+};
+
+static void mlp_alloc(MlpBufs& b, long n, int B) {
+  b.G.alloc(MLP_NW);
+  b.H1.alloc((size_t)B * MLP_H);
+  b.H2.alloc((size_t)B * MLP_H);
+  b.D1.alloc((size_t)B * MLP_H);
+  b.D2.alloc((size_t)B * MLP_H);
+  b.DY.alloc((size_t)B);
+  b.loss.alloc(1);
+  (void)n;
+}
+
+// gradient of the batch loss (mean of (y-t)^2/2) for B rows of X/T selected
+// by idx (or rows 0..B-1); the loss sum is added to b.loss
+static void mlp_batch_grad(MlpBufs& b, const int* idx, int B, hipStream_t st) {
+  hipLaunchKernelGGL(mlp_fwd_bwd_kernel, dim3((B + 255) / 256), dim3(256), 0, st, b.P.p, b.X.p, b.T.p, idx, B, b.H1.p,
+                     b.H2.p, b.D1.p, b.D2.p, b.DY.p, b.loss.p);
+  hipLaunchKernelGGL(mlp_grad_kernel, dim3(MLP_H + 1), dim3(256), 0, st, b.X.p, idx, B, b.H1.p, b.H2.p, b.D1.p,
+                     b.D2.p, b.DY.p, b.G.p);
+  HIPCHK(hipGetLastError());
+}
+
+// Train the proposal network on log rows (phi, sn, chi, eta, log V): inputs
+// and target standardised with the data's mean and deviation, Glorot-uniform
+// initial weights, Adam over shuffled minibatches with the learning rate
+// decaying geometrically to lr/10 over the run.  Returns the parameter
+// vector (with normalisation) and the mean loss of each epoch.
+static void mlp_train(const double* rows, long n, int epochs, int B, float lr, unsigned seed, float* params,
+                      float* epoch_loss) {
+  if (n < 1 || epochs < 1 || B < 1) throw Error{"mlp_train: empty data or bad settings"};
+  if (B > n) B = (int)n;
+  std::vector<float> P(MLP_NP, 0.f);
+  double mean[5] = {0, 0, 0, 0, 0}, sd[5] = {0, 0, 0, 0, 0};
+  for (long r = 0; r < n; r++)
+    for (int c = 0; c < 5; c++) mean[c] += rows[r * 5 + c];
+  for (int c = 0; c < 5; c++) mean[c] /= (double)n;
+  for (long r = 0; r < n; r++)
+    for (int c = 0; c < 5; c++) sd[c] += (rows[r * 5 + c] - mean[c]) * (rows[r * 5 + c] - mean[c]);
+  for (int c = 0; c < 5; c++) {
+    sd[c] = sqrt(sd[c] / (double)n);
+    if (!(sd[c] > 1e-12 * (fabs(mean[c]) + 1e-300))) sd[c] = 1.;  // constant column
+  }
+  for (int c = 0; c < MLP_IN; c++) {
+    P[MLP_NORM + c] = (float)mean[c];
+    P[MLP_NORM + MLP_IN + c] = (float)sd[c];
+  }
+  P[MLP_NORM + 2 * MLP_IN] = (float)mean[4];
+  P[MLP_NORM + 2 * MLP_IN + 1] = (float)sd[4];
+  std::vector<float> X((size_t)n * MLP_IN), T((size_t)n);
+  for (long r = 0; r < n; r++) {
+    for (int c = 0; c < MLP_IN; c++) X[r * MLP_IN + c] = (float)((rows[r * 5 + c] - mean[c]) / sd[c]);
+    T[r] = (float)((rows[r * 5 + 4] - mean[4]) / sd[4]);
+  }
+  std::mt19937 rng(seed);
+  auto glorot = [&](int off, int fan_in, int fan_out, int count) {
+    const float a = sqrtf(6.f / (float)(fan_in + fan_out));
+    std::uniform_real_distribution<float> u(-a, a);
+    for (int i = 0; i < count; i++) P[off + i] = u(rng);
+  };
+  glorot(MLP_W1, MLP_IN, MLP_H, MLP_H * MLP_IN);
+  glorot(MLP_W2, MLP_H, MLP_H, MLP_H * MLP_H);
+  glorot(MLP_W3, MLP_H, 1, MLP_H);
+  MlpBufs b;
+  b.P.upload(P);
+  b.X.upload(X);
+  b.T.upload(T);
+  mlp_alloc(b, n, B);
+  b.M.alloc(MLP_NW);
+  b.V.alloc(MLP_NW);
+  hipStream_t st;
+  HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  b.M.zero(st);
+  b.V.zero(st);
+  std::vector<int> perm((size_t)n);
+  for (long r = 0; r < n; r++) perm[r] = (int)r;
+  DBuf<int> didx;
+  didx.alloc((size_t)n);
+  const long nb = n / B;  // full batches per epoch
+  int stepc = 0;
+  for (int ep = 0; ep < epochs; ep++) {
+    std::shuffle(perm.begin(), perm.end(), rng);
+    HIPCHK(hipMemcpyAsync(didx.p, perm.data(), (size_t)n * sizeof(int), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemsetAsync(b.loss.p, 0, sizeof(float), st));
+    const float lre = lr * powf(0.1f, (float)ep / (float)epochs);
+    for (long k = 0; k < nb; k++) {
+      mlp_batch_grad(b, didx.p + k * B, B, st);
+      hipLaunchKernelGGL(mlp_adam_kernel, dim3((MLP_NW + 255) / 256), dim3(256), 0, st, b.P.p, b.G.p, b.M.p, b.V.p,
+                         lre, 0.9f, 0.999f, 1e-8f, ++stepc);
+    }
+    HIPCHK(hipGetLastError());
+    float ls = 0.f;
+    HIPCHK(hipMemcpyAsync(&ls, b.loss.p, sizeof(float), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (epoch_loss) epoch_loss[ep] = ls / (float)(nb * B);
+  }
+  HIPCHK(hipMemcpyAsync(P.data(), b.P.p, MLP_NW * sizeof(float), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  HIPCHK(hipStreamDestroy(st));
+  std::copy(P.begin(), P.end(), params);
+}
+
 }  // namespace fdf
 
 struct fdf_s {
@@ -1723,6 +1915,82 @@ int fdf_batch_tuning(fdf_batch_t* h, int* tile_width, int* chunk, double* stage_
 int fdf_batch_finish(fdf_batch_t* h) {
   return guarded([&] { h->b->finish(); });
 }
+
+int fdf_mlp_grad(const float* params, const float* X, const float* T, long n, float* grad, float* loss) {
+  return guarded([&] {
+    if (!params || !X || !T || n < 1 || n > (1L << 24)) throw Error{"fdf_mlp_grad: bad arguments"};
+    fdf::MlpBufs b;
+    b.P.upload(params, fdf::MLP_NP);
+    b.X.upload(X, (size_t)n * fdf::MLP_IN);
+    b.T.upload(T, (size_t)n);
+    fdf::mlp_alloc(b, n, (int)n);
+    HIPCHK(hipMemset(b.loss.p, 0, sizeof(float)));
+    fdf::mlp_batch_grad(b, nullptr, (int)n, nullptr);
+    HIPCHK(hipDeviceSynchronize());
+    if (grad) HIPCHK(hipMemcpy(grad, b.G.p, fdf::MLP_NW * sizeof(float), hipMemcpyDeviceToHost));
+    if (loss) {
+      float l = 0.f;
+      HIPCHK(hipMemcpy(&l, b.loss.p, sizeof(float), hipMemcpyDeviceToHost));
+      *loss = l / (float)n;
+    }
+  });
+}
+
+int fdf_mlp_predict(const float* params, const float* X, long n, float* Y) {
+  return guarded([&] {
+    if (!params || !X || !Y || n < 1) throw Error{"fdf_mlp_predict: bad arguments"};
+    fdf::DBuf<float> P, dx, dy;
+    P.upload(params, fdf::MLP_NP);
+    dx.upload(X, (size_t)n * fdf::MLP_IN);
+    dy.alloc((size_t)n);
+    hipLaunchKernelGGL(fdf::mlp_predict_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, nullptr, P.p, dx.p,
+                       (int)n, dy.p);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpy(Y, dy.p, (size_t)n * sizeof(float), hipMemcpyDeviceToHost));
+  });
+}
+
+int fdf_mlp_train(const double* rows, long n, int epochs, int batch, float lr, unsigned seed, float* params,
+                  float* epoch_loss) {
+  return guarded([&] {
+    if (!rows || !params) throw Error{"fdf_mlp_train: null argument"};
+    fdf::mlp_train(rows, n, epochs, batch, lr, seed, params, epoch_loss);
+  });
+}
+
+int fdf_mlp_param_count(void) { return fdf::MLP_NP; }
+
+int fdf_stz_set_model(fdf_t* h, const float* params, int n) {
+  return guarded([&] {
+    if (auto* s = dynamic_cast<fdf::Solver<double>*>(h->solver.get())) s->stz_set_model(params, n);
+    else if (auto* s32 = dynamic_cast<fdf::Solver<float>*>(h->solver.get())) s32->stz_set_model(params, n);
+  });
+}
+
+int fdf_stz_log(fdf_t* h, long capacity) {
+  return guarded([&] {
+    if (auto* s = dynamic_cast<fdf::Solver<double>*>(h->solver.get())) s->stz_enable_log(capacity);
+    else if (auto* s32 = dynamic_cast<fdf::Solver<float>*>(h->solver.get())) s32->stz_enable_log(capacity);
+  });
+}
+
+long fdf_stz_log_read(fdf_t* h, double* out, long cap) {
+  long r = -1;
+  int rc = guarded([&] {
+    if (auto* s = dynamic_cast<fdf::Solver<double>*>(h->solver.get())) r = s->stz_read_log(out, cap);
+    else if (auto* s32 = dynamic_cast<fdf::Solver<float>*>(h->solver.get())) r = s32->stz_read_log(out, cap);
+  });
+  return rc ? -1 : r;
+}
+
+int fdf_stz_stats(fdf_t* h, unsigned long long* out4) {
+  return guarded([&] {
+    if (!out4) throw Error{"fdf_stz_stats: null argument"};
+    if (auto* s = dynamic_cast<fdf::Solver<double>*>(h->solver.get())) s->stz_stats(out4);
+    else if (auto* s32 = dynamic_cast<fdf::Solver<float>*>(h->solver.get())) s32->stz_stats(out4);
+  });
+}
+
 
 double fdf_bulk_kernel_ms(fdf_t* h, int* nlaunch) { return h->solver->bulk_ms(nlaunch); }
 

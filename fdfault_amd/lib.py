@@ -46,7 +46,8 @@ EXPORTS = ["fdf_open", "fdf_step", "fdf_stage", "fdf_run", "fdf_read_field", "fd
            "fdf_host_setup", "fdf_last_error", "fdf_close", "fdf_bulk_kernel_ms",
            "fdf_open_batch", "fdf_batch_step", "fdf_batch_size", "fdf_batch_get_info", "fdf_batch_read_iface",
            "fdf_batch_read_front", "fdf_batch_read_array", "fdf_batch_tuning", "fdf_batch_finish",
-           "fdf_batch_close"]
+           "fdf_batch_close", "fdf_mlp_param_count", "fdf_mlp_train", "fdf_mlp_grad", "fdf_mlp_predict",
+           "fdf_stz_set_model", "fdf_stz_log", "fdf_stz_log_read", "fdf_stz_stats"]
 
 _lib = None
 
@@ -91,6 +92,15 @@ def load():
     L.fdf_batch_tuning.argtypes = [C.c_void_p, ip, ip, dp, ip]
     L.fdf_batch_finish.argtypes = [C.c_void_p]
     L.fdf_batch_close.argtypes = [C.c_void_p]
+    fp = C.POINTER(C.c_float)
+    L.fdf_mlp_train.argtypes = [dp, C.c_long, C.c_int, C.c_int, C.c_float, C.c_uint, fp, fp]
+    L.fdf_mlp_grad.argtypes = [fp, fp, fp, C.c_long, fp, fp]
+    L.fdf_mlp_predict.argtypes = [fp, fp, C.c_long, fp]
+    L.fdf_stz_set_model.argtypes = [C.c_void_p, fp, C.c_int]
+    L.fdf_stz_log.argtypes = [C.c_void_p, C.c_long]
+    L.fdf_stz_log_read.restype = C.c_long
+    L.fdf_stz_log_read.argtypes = [C.c_void_p, dp, C.c_long]
+    L.fdf_stz_stats.argtypes = [C.c_void_p, C.POINTER(C.c_ulonglong)]
     _lib = L
     return L
 
@@ -212,6 +222,37 @@ class Solver:
         if load().fdf_finish(self.h) != 0:
             raise _err()
 
+    def stz_set_model(self, params):
+        L = load()
+        if params is None:
+            rc = L.fdf_stz_set_model(self.h, None, 0)
+        else:
+            p = np.ascontiguousarray(params, np.float32)
+            rc = L.fdf_stz_set_model(self.h, _fp(p), p.size)
+        if rc != 0:
+            raise _err()
+
+    def stz_log(self, capacity):
+        if load().fdf_stz_log(self.h, int(capacity)) != 0:
+            raise _err()
+
+    def stz_log_read(self):
+        L = load()
+        n = L.fdf_stz_log_read(self.h, None, 0)
+        if n < 0:
+            raise _err()
+        out = np.empty((n, 5))
+        if n and L.fdf_stz_log_read(self.h, _dp(out), n) < 0:
+            raise _err()
+        return out
+
+    def stz_stats(self):
+        """(solves, Newton iterations, proposals used, fallbacks) since the last call."""
+        out = (C.c_ulonglong * 4)()
+        if load().fdf_stz_stats(self.h, out) != 0:
+            raise _err()
+        return tuple(int(v) for v in out)
+
     def bulk_kernel_ms(self):
         n = C.c_int(0)
         ms = load().fdf_bulk_kernel_ms(self.h, C.byref(n))
@@ -222,6 +263,46 @@ def run(inpath: str, precision: int = FP64, quiet: bool = True, exact: bool = Fa
     o = Opts(precision, 0, 1, 1 if quiet else 0, 0, 1 if exact else 0)
     if load().fdf_run(inpath.encode(), C.byref(o)) != 0:
         raise _err()
+
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def mlp_param_count():
+    return load().fdf_mlp_param_count()
+
+
+def mlp_train(rows, epochs=20, batch=1024, lr=3e-3, seed=0):
+    """Train the STZ proposal network on the GPU from log rows
+    (phi, sn, chi, eta, log V); returns (params, per-epoch mean loss)."""
+    rows = np.ascontiguousarray(rows, dtype=np.float64)
+    P = np.zeros(mlp_param_count(), np.float32)
+    losses = np.zeros(epochs, np.float32)
+    if load().fdf_mlp_train(_dp(rows), rows.shape[0], epochs, batch, lr, seed, _fp(P), _fp(losses)) != 0:
+        raise _err()
+    return P, losses
+
+
+def mlp_grad(P, X, T):
+    """GPU gradient and mean loss of one batch of normalised samples."""
+    P = np.ascontiguousarray(P, np.float32)
+    X = np.ascontiguousarray(X, np.float32)
+    T = np.ascontiguousarray(T, np.float32)
+    G = np.zeros(4545, np.float32)
+    loss = np.zeros(1, np.float32)
+    if load().fdf_mlp_grad(_fp(P), _fp(X), _fp(T), X.shape[0], _fp(G), _fp(loss)) != 0:
+        raise _err()
+    return G, float(loss[0])
+
+
+def mlp_predict(P, X):
+    P = np.ascontiguousarray(P, np.float32)
+    X = np.ascontiguousarray(X, np.float32)
+    Y = np.zeros(X.shape[0], np.float32)
+    if load().fdf_mlp_predict(_fp(P), _fp(X), X.shape[0], _fp(Y)) != 0:
+        raise _err()
+    return Y
 
 
 class Batch:

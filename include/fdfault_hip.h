@@ -31,9 +31,11 @@ typedef struct fdf_opts {
   int quiet;         /* 1: no stdout progress lines                             */
   int use_graph;     /* 1: capture the per-step launch sequence in a hipGraph   */
   int exact_fp64;    /* 1: every FP64 stencil keeps the reference's per-point
-                        association (bit-faithful); 0: interiors of Cartesian
-                        3-D blocks use the faster interior kernel (block-
-                        constant metric; within 1e-13 per stage)             */
+                        association (bit-faithful); 0: Cartesian homogeneous
+                        blocks without dissipation (3-D, and 2-D mode II
+                        elastic) run on the tiled kernels (block-constant
+                        metric, fused multiply-adds; within 1e-12 per stage
+                        and 1e-10 over long runs)                             */
 } fdf_opts;
 
 typedef struct fdf_info {
@@ -112,8 +114,8 @@ double fdf_bulk_kernel_ms(fdf_t* h, int* nlaunch);
  * steps n independent 2-D mode II elastic problems together on one GPU: each
  * instance is set up as fdf_open would (same outputs and front files), then
  * every RK stage of every instance runs as one fault launch and one
- * persistent bulk launch.  Instances must share sbporder, rkorder and dt;
- * their grid sizes may differ.  The bulk kernel's tile width and x-chunk are
+ * persistent bulk launch.  Instances must share sbporder and rkorder; grid
+ * sizes and time steps may differ.  The bulk kernel's tile width and x-chunk are
  * chosen at open by timing candidates (cached in $FDF_TUNE_CACHE or
  * ~/.cache/fdfault_amd_tune.txt; FDF_BATCH_TW/FDF_BATCH_CX fix them). */
 typedef struct fdf_batch_s fdf_batch_t;
@@ -135,6 +137,41 @@ int fdf_batch_tuning(fdf_batch_t* h, int* tile_width, int* chunk, double* stage_
 /* fdf_finish for every instance. */
 int fdf_batch_finish(fdf_batch_t* h);
 void fdf_batch_close(fdf_batch_t* h);
+
+/* ---- Learned proposal for the STZ root ------------------------------------
+ * No counterpart in the reference, whose STZ law solves for the friction
+ * coefficient with a bracketed Newton iteration from the bracket midpoint
+ * (stz::calc_mu, src/stz.cpp:263-356, and solve_newton,
+ * src/utilities.cpp:89-180).  A 4 -> 64 -> 64 -> 1 tanh network maps
+ * (phi, sn, chi, eta) -- locked shear traction, normal stress, effective
+ * temperature, impedance -- to log(V); the friction coefficient implied by
+ * that slip rate starts the same safeguarded Newton iteration, which must
+ * reach |f| <= 1e-12 |phi|, else the plain bracketed solve runs.  The
+ * parameter vector (fdf_mlp_param_count() floats) is
+ * W1[64][4] b1[64] W2[64][64] b2[64] W3[64] b3 in_mean[4] in_std[4] out_mean
+ * out_std. */
+int fdf_mlp_param_count(void);
+/* Train on n log rows (phi, sn, chi, eta, log V) with the GPU kernels
+ * (standardised data, Adam, minibatches of `batch`, learning rate decaying to
+ * lr/10); writes the parameters and each epoch's mean loss. */
+int fdf_mlp_train(const double* rows, long n, int epochs, int batch, float lr, unsigned seed, float* params,
+                  float* epoch_loss);
+/* Gradient (4545 floats) and mean loss of one batch of n normalised samples,
+ * X[n][4], T[n], computed by the GPU training kernels. */
+int fdf_mlp_grad(const float* params, const float* X, const float* T, long n, float* grad, float* loss);
+/* Normalised network outputs for n normalised inputs (GPU). */
+int fdf_mlp_predict(const float* params, const float* X, long n, float* Y);
+/* Use (params != NULL, n = fdf_mlp_param_count()) or stop using the network
+ * in this problem's STZ solves. */
+int fdf_stz_set_model(fdf_t* h, const float* params, int n);
+/* Log the next `capacity` STZ root solves as rows (phi, sn, chi, eta, log V);
+ * 0 stops logging.  fdf_stz_log_read copies up to cap rows and returns the
+ * number logged. */
+int fdf_stz_log(fdf_t* h, long capacity);
+long fdf_stz_log_read(fdf_t* h, double* out, long cap);
+/* Counters since the last call: root solves, Newton iterations, solves
+ * finished from the network's proposal, fallbacks to the plain solve. */
+int fdf_stz_stats(fdf_t* h, unsigned long long* out4);
 
 #ifdef __cplusplus
 }

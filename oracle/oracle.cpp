@@ -2399,4 +2399,60 @@ void orc_rotate(const double* b9, const double* nn, const double* t1, const doub
   memcpy(out9, &o, sizeof(BF));
 }
 
+// ---- proposal network (test infrastructure for the GPU training kernels) --
+// Layer by layer in double precision: h1 = tanh(W1 x + b1),
+// h2 = tanh(W2 h1 + b2), y = W3 . h2 + b3; the gradient is plain reverse-mode
+// differentiation of the mean squared error over the batch.
+static const int NI = 4, NH = 64;
+static const int OW1 = 0, OB1 = OW1 + NH * NI, OW2 = OB1 + NH, OB2 = OW2 + NH * NH, OW3 = OB2 + NH, OB3 = OW3 + NH,
+                 NWTS = OB3 + 1;
+
+static double mlp_one(const double* P, const double* x, double* h1, double* h2) {
+  for (int j = 0; j < NH; j++) {
+    double a = P[OB1 + j];
+    for (int i = 0; i < NI; i++) a += P[OW1 + j * NI + i] * x[i];
+    h1[j] = tanh(a);
+  }
+  double y = P[OB3];
+  for (int j = 0; j < NH; j++) {
+    double a = P[OB2 + j];
+    for (int k = 0; k < NH; k++) a += P[OW2 + j * NH + k] * h1[k];
+    h2[j] = tanh(a);
+    y += P[OW3 + j] * h2[j];
+  }
+  return y;
+}
+
+void orc_mlp_forward(const double* P, const double* X, long n, double* Y) {
+  double h1[NH], h2[NH];
+  for (long b = 0; b < n; b++) Y[b] = mlp_one(P, X + b * NI, h1, h2);
+}
+
+double orc_mlp_grad(const double* P, const double* X, const double* T, long n, double* G) {
+  for (int q = 0; q < NWTS; q++) G[q] = 0.;
+  double loss = 0.;
+  double h1[NH], h2[NH], d2[NH];
+  for (long b = 0; b < n; b++) {
+    const double* x = X + b * NI;
+    const double r = mlp_one(P, x, h1, h2) - T[b];
+    loss += 0.5 * r * r;
+    const double dy = r / (double)n;
+    G[OB3] += dy;
+    for (int j = 0; j < NH; j++) {
+      G[OW3 + j] += dy * h2[j];
+      d2[j] = dy * P[OW3 + j] * (1. - h2[j] * h2[j]);
+      G[OB2 + j] += d2[j];
+      for (int k = 0; k < NH; k++) G[OW2 + j * NH + k] += d2[j] * h1[k];
+    }
+    for (int k = 0; k < NH; k++) {
+      double g = 0.;
+      for (int j = 0; j < NH; j++) g += d2[j] * P[OW2 + j * NH + k];
+      const double d1 = g * (1. - h1[k] * h1[k]);
+      G[OB1 + k] += d1;
+      for (int i = 0; i < NI; i++) G[OW1 + k * NI + i] += d1 * x[i];
+    }
+  }
+  return loss / (double)n;
+}
+
 }  // extern "C"

@@ -67,6 +67,14 @@ void orc_solve_locked(double* io4, double z1, double z2);
 /* rotate xyz->nt and back (boundary.cpp:505-568); b9 = v1 v2 v3 s11 s12 s13 s22 s23 s33 */
 void orc_rotate(const double* b9, const double* nn, const double* t1, const double* t2, int to_nt, double* out9);
 
+/* The learned STZ proposal's network (no counterpart in the reference):
+ * 4 -> 64 -> 64 -> 1 with tanh, parameters laid out as
+ * W1[64][4] b1[64] W2[64][64] b2[64] W3[64] b3 (+ normalisation, unused
+ * here).  Forward of n normalised inputs X[n][4] into Y[n]; gradient G[4545]
+ * of the batch loss mean((y - t)^2 / 2), returned value = that loss. */
+void orc_mlp_forward(const double* P, const double* X, long n, double* Y);
+double orc_mlp_grad(const double* P, const double* X, const double* T, long n, double* G);
+
 #ifdef __cplusplus
 }
 #endif

@@ -61,6 +61,9 @@ def lib():
         L.orc_sw_mu.argtypes = [C.c_double] * 7
         L.orc_solve_locked.argtypes = [dp, C.c_double, C.c_double]
         L.orc_rotate.argtypes = [dp, dp, dp, dp, C.c_int, dp]
+        L.orc_mlp_forward.argtypes = [dp, dp, C.c_long, dp]
+        L.orc_mlp_grad.restype = C.c_double
+        L.orc_mlp_grad.argtypes = [dp, dp, dp, C.c_long, dp]
         _lib = L
     return _lib
 
@@ -152,3 +155,22 @@ class Oracle:
 
 def run(inpath: str, quiet=True) -> int:
     return lib().orc_run(inpath.encode(), 1 if quiet else 0)
+
+
+def mlp_forward(P, X):
+    """Oracle forward of the proposal network (double precision)."""
+    P = np.ascontiguousarray(P, np.float64)
+    X = np.ascontiguousarray(X, np.float64)
+    Y = np.empty(X.shape[0])
+    lib().orc_mlp_forward(_dp(P), _dp(X), X.shape[0], _dp(Y))
+    return Y
+
+
+def mlp_grad(P, X, T):
+    """Oracle gradient (4545) and mean loss of one batch (double precision)."""
+    P = np.ascontiguousarray(P, np.float64)
+    X = np.ascontiguousarray(X, np.float64)
+    T = np.ascontiguousarray(T, np.float64)
+    G = np.empty(4545)
+    loss = lib().orc_mlp_grad(_dp(P), _dp(X), _dp(T), X.shape[0], _dp(G))
+    return G, loss
