@@ -45,6 +45,96 @@ def workload_3d(d, nx=401, ny=201, nz=201, dx=0.1, sbporder=3, nt=1500, seed=123
     return p.write(os.path.join(d, "bench3d.in"))
 
 
+STZ_PARAMS_3D = (1.e-6, 0.3, 0.01, 0.1, 1.e4, 1.e-3, 0.1, 0.4, 1.e4)  # v0 f0 a muy c0 R beta chiw v1
+
+
+def workload_stz3d(d, seed, nx=301, ny=151, nz=151, dx=0.1, sbporder=3, nt=1000):
+    """3-D STZ rupture: two blocks nx x ny x nz, -120 MPa normal / 70 MPa shear
+    load, effective temperature seeded uniform on [0.06, 0.08] (state file)."""
+    os.makedirs(os.path.join(d, "data"), exist_ok=True)
+    chi = np.random.default_rng(seed).uniform(0.06, 0.08, size=(nx, nz))
+    sp = os.path.join(d, "state.bin")
+    chi.tofile(sp)
+    stz = dict(chi0=0., statefile=sp, perts=[Pert("constant", 0., 0., 0., 0., 0., STZ_PARAMS_3D)])
+    loads = [Pert("constant", 0., 0., 0., 0., 0., (-120., 70., 0.))]
+    p = two_block_fault(ndim=3, nx=nx, ny=ny, nz=nz, dx=dx, nt=nt, sbporder=sbporder, law="stz", stz=stz,
+                        loads=loads, datadir=os.path.join(d, "data") + "/", name="stz3d", front=("V", 0.001))
+    return p.write(os.path.join(d, "stz3d.in"))
+
+
+def stz_proposal(args):
+    """Learned STZ root proposal on the 3-D STZ workload: log roots from plain
+    runs (4 seeds, one step each at different times), train on the GPU, then
+    compare plain and proposal solves on a fresh seed."""
+    from fdfault_amd import lib
+    prec = lib.FP64 if args.precision == "fp64" else lib.FP32
+    rows = []
+    with tempfile.TemporaryDirectory() as d:
+        for k, seed in enumerate((11, 12, 13, 14)):
+            path = workload_stz3d(os.path.join(d, "s%d" % seed), seed, args.stz_nx, args.stz_ny, args.stz_nz)
+            s = lib.Solver(path, precision=prec)
+            s.step(8 * k)
+            s.stz_log(400000)
+            s.step(1)
+            rows.append(s.stz_log_read())
+            s.close()
+            print("logged seed %d: %d solves" % (seed, rows[-1].shape[0]), flush=True)
+        rows = np.concatenate(rows)
+        t0 = time.perf_counter()
+        P, losses = lib.mlp_train(rows, epochs=args.mlp_epochs, batch=1024, lr=3e-3, seed=0)
+        ttrain = time.perf_counter() - t0
+        print("trained on %d pairs in %.1f s, loss %.3g -> %.3g" % (rows.shape[0], ttrain, losses[0], losses[-1]),
+              flush=True)
+        path = workload_stz3d(os.path.join(d, "s99"), 99, args.stz_nx, args.stz_ny, args.stz_nz)
+        res = {}
+        for mode in ("plain", "exact_derivative_no_network", "proposal"):
+            if mode == "exact_derivative_no_network":
+                os.environ["FDF_STZ_NEWTON"] = "exact"
+            s = lib.Solver(path, precision=prec)
+            os.environ.pop("FDF_STZ_NEWTON", None)
+            if mode == "proposal":
+                s.stz_set_model(P)
+            s.step(args.warmup)
+            s.stz_stats()
+            s.fault_kernel_ms()
+            s.bulk_kernel_ms()
+            t0 = time.perf_counter()
+            s.step(args.steps)
+            t = time.perf_counter() - t0
+            st = s.stz_stats()
+            fms, _ = s.fault_kernel_ms()
+            bms, _ = s.bulk_kernel_ms()
+            res[mode] = {"solves": st[0], "mean_newton_iterations": round(st[1] / max(st[0], 1), 3),
+                         "proposals_used": st[2], "fallbacks": st[3], "fault_ms_per_stage": round(fms, 4),
+                         "bulk_ms_per_launch": round(bms, 4), "ms_per_step": round(t / args.steps * 1e3, 3),
+                         "U": s.iface(0, "U")}
+            s.close()
+            print(mode, {k: v for k, v in res[mode].items() if k != "U"}, flush=True)
+        du = float(np.abs(res["proposal"]["U"] - res["plain"]["U"]).max() / max(np.abs(res["plain"]["U"]).max(), 1e-300))
+        for m in res.values():
+            del m["U"]
+    npts = 2 * args.stz_nx * args.stz_ny * args.stz_nz
+    line = {
+        "metric": "STZ root solve: Newton iterations per solve, plain / learned proposal",
+        "value": round(res["plain"]["mean_newton_iterations"] / max(res["proposal"]["mean_newton_iterations"], 1e-9), 3),
+        "unit": "x fewer iterations", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "higher_is_better": True, "vs_baseline": None, "dtype": "f64" if prec == lib.FP64 else "f32",
+        "data": "synthetic",
+        "config": {"workload": "stz3d_2x%dx%dx%d" % (args.stz_nx, args.stz_ny, args.stz_nz), "points": npts,
+                   "stz": dict(zip(("v0", "f0", "a", "muy", "c0", "R", "beta", "chiw", "v1"), STZ_PARAMS_3D)),
+                   "chi0": "uniform [0.06, 0.08], seed 99", "training_pairs": int(rows.shape[0]),
+                   "training_seeds": [11, 12, 13, 14], "epochs": args.mlp_epochs, "train_s": round(ttrain, 2),
+                   "final_train_loss": float(losses[-1])},
+        "plain": res["plain"], "exact_derivative_no_network": res["exact_derivative_no_network"],
+        "proposal": res["proposal"],
+        "note": "plain = the reference's Newton (bracket midpoint start, the reference's df/dmu whose muy term has "
+                "the wrong sign, stz.cpp:479); exact_derivative_no_network = same start, exact df/dmu; proposal = "
+                "network start, exact df/dmu, fallback to plain",
+        "final_slip_rel_diff_proposal_vs_plain": du,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def bytes_per_point(nf, w, curvilinear=False, ndim=3, plastic_words=0, nstages=5):
     """Algorithmic bytes per point per RK stage: read f, read df, write df,
     write f (4 nf words) + geometry words (curvilinear) + plastic words/step."""
@@ -145,7 +235,11 @@ def main():
     ap.add_argument("--nz", type=int, default=201)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=20)
-    ap.add_argument("--workload", choices=["planar3d", "ensemble"], default="planar3d")
+    ap.add_argument("--workload", choices=["planar3d", "ensemble", "stz3d"], default="planar3d")
+    ap.add_argument("--stz-nx", type=int, default=301)
+    ap.add_argument("--stz-ny", type=int, default=151)
+    ap.add_argument("--stz-nz", type=int, default=151)
+    ap.add_argument("--mlp-epochs", type=int, default=20)
     ap.add_argument("--instances", type=int, default=64)
     ap.add_argument("--ens-nt", type=int, default=2000, help="time steps of one rupture (ensemble)")
     ap.add_argument("--check-error", action="store_true", help="ensemble: compare with FP64 single runs")
@@ -155,6 +249,11 @@ def main():
         print("bench.py: this build runs on one GPU only (the whole grid fits one MI355X); "
               "multi-GPU slab decomposition is designed but not built", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "stz3d":
+        if "--precision" not in sys.argv:
+            args.precision = "fp32"
+        stz_proposal(args)
+        return
     if args.workload == "ensemble":
         if "--precision" not in sys.argv:
             args.precision = "fp32"

@@ -505,6 +505,7 @@ struct FaultDev {
   unsigned long long* log_n;
   long long log_cap;
   unsigned long long* stats;
+  int stz_exact_newton;  // 1: plain solves use the exact derivative (comparison runs only)
 };
 
 struct FaultStageDev {
@@ -545,20 +546,31 @@ __device__ __forceinline__ double stz_vpl(double mu, const double* p) {
   return p[4] * exp(-p[5] + mu / p[6] - 1. / p[3]) * (1. - p[7] / mu);
 }
 __device__ __forceinline__ double stz_f(double mu, const double* p) { return p[1] * stz_vpl(mu, p) - p[2] * mu - p[0]; }
+// df/dmu as the reference computes it (stz.cpp:471-484): the derivative of
+// the factor (1 - muy/mu) enters with a minus sign where the product rule
+// gives a plus, so the plain Newton iteration converges linearly (not
+// quadratically) to the same root.  Kept for the plain solve, whose iterates
+// then match the reference's.
 __device__ __forceinline__ double stz_df(double mu, const double* p) {
   double d = (mu <= p[7]) ? 0. : stz_vpl(mu, p) * (1. / p[6] - p[7] / mu / (mu - p[7]));
+  return p[1] * d - p[2];
+}
+// the exact derivative, dV/dmu = V (1/a + muy / (mu (mu - muy)))
+__device__ __forceinline__ double stz_df_exact(double mu, const double* p) {
+  double d = (mu <= p[7]) ? 0. : stz_vpl(mu, p) * (1. / p[6] + p[7] / mu / (mu - p[7]));
   return p[1] * d - p[2];
 }
 
 // the safeguarded Newton iteration of utilities.cpp:89-180 from x with
 // bracket [xl, xh] (f(xl) < 0 < f(xh)): a Newton step unless it leaves the
 // bracket or converges too slowly, else bisection; stops when |dx| < 2e-16
+template <bool EXACT>
 __device__ __forceinline__ double stz_rtsafe(double x, double xl, double xh, const double* p, int* err, int* iters) {
   int i = 0;
   const int nmax = 100;
   const double tol = 2.e-16;
   double func = stz_f(x, p);
-  double der = stz_df(x, p);
+  double der = EXACT ? stz_df_exact(x, p) : stz_df(x, p);
   double dx = fabs(xh - xl), dxold = dx;
   while (i < nmax) {
     if (der == 0.) { *err |= 1; break; }
@@ -576,7 +588,7 @@ __device__ __forceinline__ double stz_rtsafe(double x, double xl, double xh, con
     }
     if (fabs(dx) < tol) break;
     func = stz_f(x, p);
-    der = stz_df(x, p);
+    der = EXACT ? stz_df_exact(x, p) : stz_df(x, p);
     if (func < 0.) xl = x; else xh = x;
     i++;
   }
@@ -586,7 +598,9 @@ __device__ __forceinline__ double stz_rtsafe(double x, double xl, double xh, con
 }
 
 // bracketed Newton with bisection fallback, utilities.cpp:89-180; err codes:
-// 1 zero derivative, 2 no convergence, 4 overflow repair did not terminate
+// 1 zero derivative, 2 no convergence, 4 overflow repair did not terminate.
+// EXACT = false: the reference's derivative (the plain solve).
+template <bool EXACT>
 __device__ double stz_newton(double xmin, double xmax, const double* p, int* err, int* iters) {
   double x = 0.5 * (xmin + xmax), xh = xmax, xl = xmin;
   double func = stz_f(x, p);
@@ -606,11 +620,12 @@ __device__ double stz_newton(double xmin, double xmax, const double* p, int* err
     }
   }
   x = 0.5 * (xh + xl);
-  return stz_rtsafe(x, xl, xh, p, err, iters);
+  return stz_rtsafe<EXACT>(x, xl, xh, p, err, iters);
 }
 
 // The same root from a proposal x0 inside the bracket: the bracket shrinks to
-// the side of x0 that keeps the sign change, then the same iteration runs.
+// the side of x0 that keeps the sign change, then the same safeguarded
+// iteration runs with the exact derivative (quadratic convergence).
 // Accepted only if the result stays finite, inside the bracket, and has a
 // residual |f| <= 1e-12 |phi|; otherwise *err is set and the caller runs the
 // plain solve.
@@ -622,7 +637,7 @@ __device__ double stz_newton_from(double xmin, double xmax, double x0, const dou
   double xl = xmin, xh = xmax;
   if (f0 < 0.) xl = x0; else xh = x0;
   int e = 0;
-  const double x = stz_rtsafe(x0, xl, xh, p, &e, iters);
+  const double x = stz_rtsafe<true>(x0, xl, xh, p, &e, iters);
   if (e || !isfinite(x) || x < xmin || x > xmax || !(fabs(stz_f(x, p)) <= 1e-12 * fabs(p[0]))) *err |= 32;
   return x;
 }
@@ -634,7 +649,7 @@ template <int M, typename R>
 __device__ __forceinline__ void fault_point(const FaultDev& F, const GridDev& g, const FaultStageDev& fs,
                                             const R* __restrict__ f, const R* __restrict__ mat,
                                             const double* __restrict__ hs, int* __restrict__ err,
-                                            int* __restrict__ newton_iters, int p) {
+                                            int* __restrict__ newton_iters, int p, const float* mlp) {
   const int ii = p / F.n1, jj = p % F.n1;
   const long long N = g.N;
   const long long stride[3] = {(long long)g.Ny * g.Nz, (long long)g.Nz, 1};
@@ -794,17 +809,30 @@ __device__ __forceinline__ void fault_point(const FaultDev& F, const GridDev& g,
             mu = -phi / snc;
           } else {
             int e = 0, it = 0, used = 0, fell = 0;
-            if (F.mlp) {  // learned proposal for log V, then mu = (phi - eta V)/|sn|
-              const double lv = mlp_logv(F.mlp, phi, snc, pr[3], eta);
-              const double mu0 = (phi - eta * exp(lv)) / (-snc);
+            if (mlp) {
+              // learned proposal for log V; the friction coefficient it implies,
+              // from the force balance mu = (phi - eta V)/|sn| or from the
+              // inverted flow law mu = a (log(V/v0) + f0 + 1/chi - log(1 - muy/mu))
+              // (stz.cpp:436-484), whichever leaves the smaller residual
+              const double lv = mlp_logv(mlp, phi, snc, pr[3], eta);
+              const double xmin = pr[7], xmax = -phi / snc;
+              const double ma = (phi - eta * exp(lv)) / (-snc);
+              const double lb = lv - log(pr[4]) + pr[5] + 1. / pr[3];
+              double mb = pr[6] * lb;
+              for (int q = 0; q < 2; q++)
+                if (mb > xmin) mb = pr[6] * (lb - log1p(-xmin / mb));
+              const bool ina = ma > xmin && ma < xmax, inb = mb > xmin && mb < xmax;
+              const double ra = ina ? fabs(stz_f(ma, pr)) : INFINITY, rb = inb ? fabs(stz_f(mb, pr)) : INFINITY;
+              const double mu0 = rb < ra ? mb : ma;
               int e1 = 0;
-              mu = stz_newton_from(pr[7], -phi / snc, mu0, pr, &e1, &it);
+              mu = stz_newton_from(xmin, xmax, mu0, pr, &e1, &it);
               used = !e1;
               fell = e1 ? 1 : 0;
             }
             if (!used) {
               int it2 = 0;
-              mu = stz_newton(pr[7], -phi / snc, pr, &e, &it2);
+              mu = F.stz_exact_newton ? stz_newton<true>(pr[7], -phi / snc, pr, &e, &it2)
+                                      : stz_newton<false>(pr[7], -phi / snc, pr, &e, &it2);
               it += it2;
             }
             if (e) atomicOr(err, e);
@@ -959,9 +987,15 @@ template <int M, typename R>
 __global__ void __launch_bounds__(256) fault_kernel(FaultDev F, GridDev g, FaultStageDev fs, const R* __restrict__ f,
                                                     const R* __restrict__ mat, const double* __restrict__ hs,
                                                     int* __restrict__ err, int* __restrict__ newton_iters) {
+  // the proposal network (if any) is read from LDS: one copy per workgroup
+  __shared__ float smlp[MLP_NP];
+  if (F.mlp) {
+    for (int i = threadIdx.x; i < MLP_NP; i += blockDim.x) smlp[i] = F.mlp[i];
+    __syncthreads();
+  }
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= F.n) return;
-  fault_point<M, R>(F, g, fs, f, mat, hs, err, newton_iters, p);
+  fault_point<M, R>(F, g, fs, f, mat, hs, err, newton_iters, p, F.mlp ? smlp : nullptr);
 }
 
 // ---------------------------------------------------------------------------
@@ -1785,7 +1819,7 @@ __global__ void __launch_bounds__(256) fault_batch_kernel(const FaultBatchDev* _
     fsi.dt = I.dt;
     if (p < I.F.n)
       fault_point<M, R>(I.F, I.g, fsi, reinterpret_cast<const R*>(I.f[par]), reinterpret_cast<const R*>(I.mat), I.hs,
-                        err, nullptr, p);
+                        err, nullptr, p, nullptr);
   }
 }
 

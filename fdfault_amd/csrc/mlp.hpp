@@ -20,7 +20,13 @@ constexpr int MLP_W1 = 0, MLP_B1 = MLP_W1 + MLP_H * MLP_IN, MLP_W2 = MLP_B1 + ML
               MLP_W3 = MLP_B2 + MLP_H, MLP_B3 = MLP_W3 + MLP_H, MLP_NW = MLP_B3 + 1;  // 4545 trained parameters
 constexpr int MLP_NORM = MLP_NW, MLP_NP = MLP_NORM + 2 * MLP_IN + 2;      // + normalisation
 
-// normalised network output for one normalised input (FP32)
+// tanh(x) = 1 - 2 / (exp(2x) + 1) with the hardware exponential and
+// reciprocal (relative error ~1e-6; exact limits +-1 for large |x|); training
+// and inference use the same function
+__device__ __forceinline__ float mlp_tanh(float x) { return 1.f - __fdividef(2.f, __expf(2.f * x) + 1.f); }
+
+// normalised network output for one normalised input (FP32); the second
+// layer runs four hidden units at a time (independent accumulation chains)
 __device__ __forceinline__ float mlp_eval(const float* __restrict__ P, const float* x) {
   float h1[MLP_H];
 #pragma unroll
@@ -28,14 +34,20 @@ __device__ __forceinline__ float mlp_eval(const float* __restrict__ P, const flo
     float s = P[MLP_B1 + j];
 #pragma unroll
     for (int i = 0; i < MLP_IN; i++) s += P[MLP_W1 + j * MLP_IN + i] * x[i];
-    h1[j] = tanhf(s);
+    h1[j] = mlp_tanh(s);
   }
   float y = P[MLP_B3];
-  for (int j = 0; j < MLP_H; j++) {
-    float s = P[MLP_B2 + j];
+  for (int j = 0; j < MLP_H; j += 4) {
+    float s0 = P[MLP_B2 + j], s1 = P[MLP_B2 + j + 1], s2 = P[MLP_B2 + j + 2], s3 = P[MLP_B2 + j + 3];
 #pragma unroll
-    for (int k = 0; k < MLP_H; k++) s += P[MLP_W2 + j * MLP_H + k] * h1[k];
-    y += P[MLP_W3 + j] * tanhf(s);
+    for (int k = 0; k < MLP_H; k++) {
+      s0 += P[MLP_W2 + j * MLP_H + k] * h1[k];
+      s1 += P[MLP_W2 + (j + 1) * MLP_H + k] * h1[k];
+      s2 += P[MLP_W2 + (j + 2) * MLP_H + k] * h1[k];
+      s3 += P[MLP_W2 + (j + 3) * MLP_H + k] * h1[k];
+    }
+    y += P[MLP_W3 + j] * mlp_tanh(s0) + P[MLP_W3 + j + 1] * mlp_tanh(s1) + P[MLP_W3 + j + 2] * mlp_tanh(s2) +
+         P[MLP_W3 + j + 3] * mlp_tanh(s3);
   }
   return y;
 }
@@ -77,7 +89,7 @@ __global__ void __launch_bounds__(256) mlp_fwd_bwd_kernel(const float* __restric
       float a = P[MLP_B1 + j];
 #pragma unroll
       for (int i = 0; i < MLP_IN; i++) a += P[MLP_W1 + j * MLP_IN + i] * x[i];
-      h1[j] = tanhf(a);
+      h1[j] = mlp_tanh(a);
       H1[(long long)b * MLP_H + j] = h1[j];
     }
     float y = P[MLP_B3];
@@ -85,7 +97,7 @@ __global__ void __launch_bounds__(256) mlp_fwd_bwd_kernel(const float* __restric
       float a = P[MLP_B2 + j];
 #pragma unroll
       for (int k = 0; k < MLP_H; k++) a += P[MLP_W2 + j * MLP_H + k] * h1[k];
-      h2[j] = tanhf(a);
+      h2[j] = mlp_tanh(a);
       H2[(long long)b * MLP_H + j] = h2[j];
       y += P[MLP_W3 + j] * h2[j];
     }

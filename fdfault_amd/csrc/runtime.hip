@@ -71,6 +71,7 @@ class SolverBase {
   virtual long read_front(int iface, double* h, long n) = 0;
   virtual void finish() = 0;
   virtual double bulk_ms(int* n) = 0;
+  virtual double fault_ms(int* n) = 0;
   SetupH S;
   fdf_opts opts{};
   int cur_step = 0;
@@ -96,6 +97,7 @@ class Solver : public SolverBase {
   long read_front(int iface, double* h, long n) override;
   void finish() override;
   double bulk_ms(int* n) override;
+  double fault_ms(int* n) override;
 
  private:
   int mode_id() const { return S.ndim == 3 ? 2 : (S.mode == 2 ? 0 : 1); }
@@ -115,8 +117,9 @@ class Solver : public SolverBase {
   static constexpr int NEV = 128;
   hipEvent_t evs[2 * NEV] = {};
   int nev = 0;  // recorded pairs not yet summed
-  double bulk_time_ms = 0.;
-  int bulk_count = 0;
+  double bulk_time_ms = 0., fault_time_ms = 0.;
+  int bulk_count = 0, fault_count = 0;
+  int evkind[NEV] = {};  // 0 bulk launch, 1 the stage's fault launches
   void flush_events();
   GridDev G{};
   DBuf<R> f[2], df, fp, metric, jac, mat;
@@ -490,6 +493,9 @@ Solver<R>::Solver(const std::string& path, const fdf_opts& o) {
       F.front_field = S.front_field;
     }
     if (I.law == LAW_STZ) {
+      // FDF_STZ_NEWTON=exact: plain solves with the exact derivative instead
+      // of the reference's (to separate the two effects in comparisons)
+      if (const char* e = getenv("FDF_STZ_NEWTON")) F.stz_exact_newton = !strcmp(e, "exact");
       if (!stz_cnt.p) {
         stz_cnt.alloc(5);
         stz_cnt.zero(stream);
@@ -862,7 +868,16 @@ void Solver<R>::do_stage(int istep, int st, bool full_step) {
   const double tfront = (double)(istep + 1) * S.dt;
   FaultStageDev fs{S.rk.A[st], S.rk.B[st], S.dt, tst, 0, (full_step && last && !S.plastic) ? tfront : -1.,
                    capturing ? graph_step_ptr : nullptr, S.rk.C[st], (full_step && last && !S.plastic) ? 1 : 0};
+  if (!capturing && !S.ifaces.empty()) {
+    if (nev == NEV) flush_events();
+    HIPCHK(hipEventRecord(evs[2 * nev], stream));
+    evkind[nev] = 1;
+  }
   for (size_t i = 0; i < S.ifaces.size(); i++) launch_fault((int)i, fs);
+  if (!capturing && !S.ifaces.empty()) {
+    HIPCHK(hipEventRecord(evs[2 * nev + 1], stream));
+    nev++;
+  }
   StageDev sd{S.rk.A[st], S.rk.B[st], S.dt, (full_step && last && !S.plastic && sizeof(R) == 8) ? 1 : 0};
   for (size_t b = 0; b < S.blocks.size(); b++) {
     if (capturing) {
@@ -871,6 +886,7 @@ void Solver<R>::do_stage(int istep, int st, bool full_step) {
     }
     if (nev == NEV) flush_events();
     HIPCHK(hipEventRecord(evs[2 * nev], stream));
+    evkind[nev] = 0;
     launch_bulk((int)b, sd);
     HIPCHK(hipEventRecord(evs[2 * nev + 1], stream));
     nev++;
@@ -891,8 +907,13 @@ void Solver<R>::flush_events() {
     HIPCHK(hipEventSynchronize(evs[2 * e + 1]));
     float ms = 0.f;
     HIPCHK(hipEventElapsedTime(&ms, evs[2 * e], evs[2 * e + 1]));
-    bulk_time_ms += ms;
-    bulk_count += 1;
+    if (evkind[e]) {
+      fault_time_ms += ms;
+      fault_count += 1;
+    } else {
+      bulk_time_ms += ms;
+      bulk_count += 1;
+    }
   }
   nev = 0;
 }
@@ -1165,6 +1186,16 @@ void Solver<R>::finish() {
     fprintf(py, "endian = '%c'\nnx = %d\nny = %d\n", endian, I.n0, I.n1);
     fclose(py);
   }
+}
+
+template <typename R>
+double Solver<R>::fault_ms(int* n) {
+  flush_events();
+  const double avg = fault_count ? fault_time_ms / fault_count : 0.;
+  if (n) *n = fault_count;
+  fault_time_ms = 0.;
+  fault_count = 0;
+  return avg;
 }
 
 template <typename R>
@@ -1993,6 +2024,7 @@ int fdf_stz_stats(fdf_t* h, unsigned long long* out4) {
 
 
 double fdf_bulk_kernel_ms(fdf_t* h, int* nlaunch) { return h->solver->bulk_ms(nlaunch); }
+double fdf_fault_kernel_ms(fdf_t* h, int* nstages) { return h->solver->fault_ms(nstages); }
 
 int fdf_host_setup(const char* in_path, fdf_info* info, double* x, double* metric, double* jac) {
   return guarded([&] {

@@ -43,7 +43,7 @@ class Info(C.Structure):
 
 EXPORTS = ["fdf_open", "fdf_step", "fdf_stage", "fdf_run", "fdf_read_field", "fdf_read_array",
            "fdf_write_array", "fdf_read_iface", "fdf_read_front", "fdf_finish", "fdf_get_info",
-           "fdf_host_setup", "fdf_last_error", "fdf_close", "fdf_bulk_kernel_ms",
+           "fdf_host_setup", "fdf_last_error", "fdf_close", "fdf_bulk_kernel_ms", "fdf_fault_kernel_ms",
            "fdf_open_batch", "fdf_batch_step", "fdf_batch_size", "fdf_batch_get_info", "fdf_batch_read_iface",
            "fdf_batch_read_front", "fdf_batch_read_array", "fdf_batch_tuning", "fdf_batch_finish",
            "fdf_batch_close", "fdf_mlp_param_count", "fdf_mlp_train", "fdf_mlp_grad", "fdf_mlp_predict",
@@ -80,6 +80,8 @@ def load():
     L.fdf_close.argtypes = [C.c_void_p]
     L.fdf_bulk_kernel_ms.restype = C.c_double
     L.fdf_bulk_kernel_ms.argtypes = [C.c_void_p, ip]
+    L.fdf_fault_kernel_ms.restype = C.c_double
+    L.fdf_fault_kernel_ms.argtypes = [C.c_void_p, ip]
     L.fdf_open_batch.argtypes = [C.POINTER(C.c_char_p), C.c_int, C.POINTER(Opts), C.POINTER(C.c_void_p)]
     L.fdf_batch_step.argtypes = [C.c_void_p, C.c_int]
     L.fdf_batch_size.argtypes = [C.c_void_p]
@@ -252,6 +254,11 @@ class Solver:
         if load().fdf_stz_stats(self.h, out) != 0:
             raise _err()
         return tuple(int(v) for v in out)
+
+    def fault_kernel_ms(self):
+        n = C.c_int()
+        ms = load().fdf_fault_kernel_ms(self.h, C.byref(n))
+        return ms, n.value
 
     def bulk_kernel_ms(self):
         n = C.c_int(0)

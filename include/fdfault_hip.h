@@ -107,6 +107,9 @@ void fdf_close(fdf_t* h);
  * launches in *nlaunch.  Measurement hook for the benchmark; no counterpart
  * in the reference. */
 double fdf_bulk_kernel_ms(fdf_t* h, int* nlaunch);
+/* The same for the fault (interface) launches of one stage, all interfaces
+ * together; count of stages in *nstages. */
+double fdf_fault_kernel_ms(fdf_t* h, int* nstages);
 
 /* ---- Ensembles ------------------------------------------------------------
  * No counterpart in the reference, which runs problem::solve
