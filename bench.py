@@ -45,6 +45,126 @@ def workload_3d(d, nx=401, ny=201, nz=201, dx=0.1, sbporder=3, nt=1500, seed=123
     return p.write(os.path.join(d, "bench3d.in"))
 
 
+def workload_2d(d, nx=801, ny=401, dx=0.04, sbporder=3, nt=2000):
+    """2-D mode II planar fault: two blocks nx x ny, -120 MPa normal / 70 MPa
+    shear, 3 km nucleation patch (+11.6 MPa) at x = 16 km, slip weakening."""
+    loads = [Pert("constant", 0., 0., 0., 0., 0., (-120., 70., 0.)),
+             Pert("boxcar", 0., 16.0, 1.5, 0., 0., (0., 11.6, 0.))]
+    p = two_block_fault(ndim=2, nx=nx, ny=ny, dx=dx, nt=nt, sbporder=sbporder, loads=loads,
+                        datadir=os.path.join(d, "data") + "/", name="planar2d", front=("V", 0.001))
+    os.makedirs(os.path.join(d, "data"), exist_ok=True)
+    return p.write(os.path.join(d, "planar2d.in"))
+
+
+def workload_rough2d(d, nx=1601, ny=801, dx=0.025, sbporder=4, nt=4000, seed=7):
+    """2-D mode II rough fault with Drucker-Prager plasticity: fault profile
+    self-similar (Hurst 1, amplitude/wavelength 1e-2, wavelengths 0.2-20 km),
+    curvilinear blocks nx x ny, s0 = (sxx -100, sxy 52, syy -120, szz -55) MPa
+    (szz = nu (sxx + syy) for plane strain), plastic mu 0.7, c 0, beta 0.35,
+    eta = 0.05 s x G, slip weakening, 3 km nucleation patch (+30 MPa shear) at
+    the middle of the fault."""
+    from fdfault_amd.generators import fault_surfaces, self_similar_profile
+    os.makedirs(os.path.join(d, "data"), exist_ok=True)
+    h = self_similar_profile(nx, dx, 1.0, 1e-2, 0.2, 20.0, seed)
+    surf = fault_surfaces(d, nx, ny, dx, h)
+    Lx = (nx - 1) * dx
+    loads = [Pert("boxcar", 0., 0.5 * Lx, 1.5, 0., 0., (0., 30., 0.))]
+    p = two_block_fault(ndim=2, nx=nx, ny=ny, dx=dx, nt=nt, sbporder=sbporder, loads=loads, surf=surf,
+                        material="plastic", plastic=dict(mu=0.7, c=0., beta=0.35, eta=0.05 * 32.04),
+                        s0=(-100., 52., 0., -120., 0., -55.), datadir=os.path.join(d, "data") + "/",
+                        name="rough2d", front=("V", 0.001))
+    return p.write(os.path.join(d, "rough2d.in"))
+
+
+def single_workload(args):
+    """Plain single-problem workloads on one GPU: 2-D planar (FP64 by default)
+    and 2-D rough-fault plastic (FP32 bulk / FP64 fault by default)."""
+    from fdfault_amd import lib
+    rough = args.workload == "rough2d"
+    if "--precision" not in sys.argv:
+        args.precision = "fp32" if rough else "fp64"
+    prec = lib.FP32 if args.precision == "fp32" else lib.FP64
+    w = 4 if prec == lib.FP32 else 8
+    with tempfile.TemporaryDirectory() as d:
+        path = workload_rough2d(d) if rough else workload_2d(d)
+        g = lib.Solver(path, precision=prec)
+        print("opened: N=%d dt=%.6g tiled blocks %d" % (g.N, g.dt, g.info.tiled_blocks), flush=True)
+        g.step(args.warmup)
+        g.bulk_kernel_ms()
+        g.fault_kernel_ms()
+        t0 = time.perf_counter()
+        g.step(args.steps)
+        t = time.perf_counter() - t0
+        kms, nl = g.bulk_kernel_ms()
+        fms, _ = g.fault_kernel_ms()
+        npts, nst, nb, nt_ = g.N, g.nstages, g.info.nblocks, g.nt
+        tiled = g.info.tiled_blocks == nb
+        g.close()
+        cpu = None
+        if not args.no_cpu:
+            threads = min(os.cpu_count() or 1, 16)
+            cv, cn = cpu_baseline(path, args.cpu_steps, threads)
+            cpu = {"value": round(cv, 3), "unit": "Mpt-updates/s", "cores": threads, "kind": "port",
+                   "sample": "the same problem (%d pts), %d steps, FP64 CPU oracle, OpenMP" % (cn, args.cpu_steps)}
+    nf = 6 if rough else 5
+    B = bytes_per_point(nf, w, curvilinear=rough, ndim=2, plastic_words=2 * (4 + 2) if rough else 0)
+    ppl = npts / nb
+    line = {
+        "metric": "Mpt-updates/s per RK stage (%s)" % ("2-D rough-fault plastic" if rough else "2-D planar fault"),
+        "value": round(npts * nst * args.steps / t / 1e6, 2), "unit": "Mpt-updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(t / args.steps * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64" if w == 8 else "f32",
+        "data": "synthetic",
+        "config": {"workload": ("rough2d_plastic_2x1601x801" if rough else "planar2d_2x801x401"), "points": npts,
+                   "rk_stages": nst, "sbporder": 4 if rough else 3, "precision": args.precision, "nt": nt_,
+                   "minutes_per_full_run": round(t / args.steps * nt_ / 60., 3)},
+        "roofline": {"bound": "hbm", "achieved": round(B * ppl / (kms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(B * ppl / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "bulk_tiled2d" if tiled else "bulk_kernel", "avg_launch_ms": round(kms, 4),
+                     "launches": nl, "bytes_per_point": B, "points_per_launch": int(ppl),
+                     "fault_ms_per_stage": round(fms, 4),
+                     "note": "working set %.0f MB: may be resident in the 256 MB Infinity Cache" % (
+                         npts * (4 * nf + (5 if rough else 0)) * w / 1e6)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def compare_precision(args):
+    """FP32 bulk / FP64 fault against FP64 on the 3-D planar workload: per-field
+    relative error of f, final slip and slip-rate error, front error in steps."""
+    from fdfault_amd import lib
+    res = {}
+    with tempfile.TemporaryDirectory() as d:
+        path = workload_3d(d, args.nx, args.ny, args.nz, sbporder=args.sbporder)
+        for prec in (lib.FP64, lib.FP32):
+            g = lib.Solver(path, precision=prec, graph=True)
+            done = 0
+            while done < args.compare_steps:
+                k = min(100, args.compare_steps - done)
+                g.step(k)
+                done += k
+                print("precision %d: %d steps" % (prec, done), flush=True)
+            res[prec] = {"f": g.array("f"), "U": g.iface(0, "U"), "V": g.iface(0, "V"), "front": g.front(0),
+                         "dt": g.dt}
+            g.close()
+    a, b = res[lib.FP32], res[lib.FP64]
+    names = ["vx", "vy", "vz", "sxx", "sxy", "sxz", "syy", "syz", "szz"]
+    per = {n: float(np.abs(a["f"][q] - b["f"][q]).max() / max(np.abs(b["f"][q]).max(), 1e-300))
+           for q, n in enumerate(names)}
+    both = (a["front"] >= 0) & (b["front"] >= 0)
+    line = {"metric": "FP32 bulk / FP64 fault vs FP64, 3-D planar fault",
+            "config": {"workload": "3d_planar_fault_slipweak_2x%dx%dx%d" % (args.nx, args.ny, args.nz),
+                       "steps": args.compare_steps},
+            "per_field_rel_err": per,
+            "final_slip_rel_err": float(np.abs(a["U"] - b["U"]).max() / max(np.abs(b["U"]).max(), 1e-300)),
+            "slip_rate_rel_err": float(np.abs(a["V"] - b["V"]).max() / max(np.abs(b["V"]).max(), 1e-300)),
+            "front_err_steps": float(np.abs(a["front"][both] - b["front"][both]).max() / b["dt"]) if both.any() else 0.,
+            "front_points_ruptured_in_one_only": int(((a["front"] >= 0) != (b["front"] >= 0)).sum()),
+            "ruptured_points": int(both.sum())}
+    print(json.dumps(line), flush=True)
+
+
 STZ_PARAMS_3D = (1.e-6, 0.3, 0.01, 0.1, 1.e4, 1.e-3, 0.1, 0.4, 1.e4)  # v0 f0 a muy c0 R beta chiw v1
 
 
@@ -235,7 +355,11 @@ def main():
     ap.add_argument("--nz", type=int, default=201)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=20)
-    ap.add_argument("--workload", choices=["planar3d", "ensemble", "stz3d"], default="planar3d")
+    ap.add_argument("--workload", choices=["planar3d", "ensemble", "stz3d", "planar2d", "rough2d"],
+                    default="planar3d")
+    ap.add_argument("--compare-precision", action="store_true",
+                    help="planar3d: FP32 vs FP64 errors after --compare-steps steps")
+    ap.add_argument("--compare-steps", type=int, default=1500)
     ap.add_argument("--stz-nx", type=int, default=301)
     ap.add_argument("--stz-ny", type=int, default=151)
     ap.add_argument("--stz-nz", type=int, default=151)
@@ -249,6 +373,12 @@ def main():
         print("bench.py: this build runs on one GPU only (the whole grid fits one MI355X); "
               "multi-GPU slab decomposition is designed but not built", file=sys.stderr)
         sys.exit(2)
+    if args.workload in ("planar2d", "rough2d"):
+        single_workload(args)
+        return
+    if args.workload == "planar3d" and args.compare_precision:
+        compare_precision(args)
+        return
     if args.workload == "stz3d":
         if "--precision" not in sys.argv:
             args.precision = "fp32"

@@ -6,9 +6,14 @@ reference's binary side-file layouts (native-endian float64, C order):
 * :func:`write_load_file` -- friction load file ``[3][n0][n1]`` (sn, s2, s3),
   read by friction::read_load (friction.cpp:507-567);
 * :func:`self_similar_profile` -- band-limited self-similar rough-fault
-  profile (Hurst exponent H, amplitude/wavelength ratio) for curved faults.
+  profile (Hurst exponent H, amplitude/wavelength ratio) for curved faults;
+* :func:`fault_surfaces` -- the block boundary curves of a two-block 2-D
+  problem whose shared face follows a fault profile (surface files
+  ``[2][n][1]``, surface.cpp:11-50).
 """
 from __future__ import annotations
+
+import os
 
 import numpy as np
 
@@ -55,3 +60,37 @@ def self_similar_profile(n, dx, hurst, amp_ratio, lmin, lmax, seed):
     if h.std() > 0:
         h *= target / h.std()
     return h
+
+
+def fault_surfaces(d, nx, ny, dx, h):
+    """Boundary curves of two stacked 2-D blocks (nx x ny points each, spacing
+    dx km) whose shared face is the fault y = Ly + h(x): block 1 gets it as
+    its top face (3), block 2 as its bottom face (2); side faces are straight
+    lines between the corners.  Returns [[4 paths of block 1], [4 of block 2]]."""
+    Lx, Ly = (nx - 1) * dx, (ny - 1) * dx
+    xs = np.linspace(0., Lx, nx)
+    yf = Ly + np.asarray(h, float)
+    paths = []
+    for blk in (0, 1):
+        faces = []
+        for face in range(4):
+            if face in (0, 1):
+                x = 0. if face == 0 else Lx
+                yc = yf[0 if face == 0 else -1]
+                yb, yt = (0., yc) if blk == 0 else (yc, 2 * Ly)
+                arr = np.zeros((2, ny, 1))
+                arr[0] = x
+                arr[1, :, 0] = np.linspace(yb, yt, ny)
+            else:
+                if (blk == 0 and face == 3) or (blk == 1 and face == 2):
+                    yy = yf
+                else:
+                    yy = np.full(nx, 0. if face == 2 else 2 * Ly)
+                arr = np.zeros((2, nx, 1))
+                arr[0, :, 0] = xs
+                arr[1, :, 0] = yy
+            pth = os.path.join(d, "surf_b%d_f%d.bin" % (blk, face))
+            np.ascontiguousarray(arr).tofile(pth)
+            faces.append(pth)
+        paths.append(faces)
+    return paths
