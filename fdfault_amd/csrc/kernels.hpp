@@ -1159,6 +1159,7 @@ struct Tiled3D {
   float fcv[3], fcg2[3], fcl[3], fcg[3], fA, fB;  // the same, rounded once for the FP32 path
   int roundtrip, hetstress;
   double s0[6];
+  int rows_nht;                  // > 0: row-tile kernel with this many halo entries per thread
   int face_active[6];            // external SAT faces
   // external SAT of a Cartesian face normal to axis d (other axes a < b) as
   // its 16 non-zero linear-map entries, [6 faces][16], in the bulk precision,
@@ -1546,6 +1547,265 @@ __global__ void __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(WP
     }
     lds_barrier();  // every wave is done reading this plane's tile
     // slide the window
+#pragma unroll
+    for (int n = 0; n < W - 1; n++)
+#pragma unroll
+      for (int q = 0; q < 6; q++) win[q][n] = win[q][n + 1];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Rectilinear 3-D block kernel with row-shaped tiles.
+//
+// Same update as bulk_tiled3d, but a workgroup owns TY whole rows of the
+// block (all nz points of each), threads mapped linearly over the TY*nz
+// points, so the memory it touches per plane and field is one contiguous run
+// of TY*nz values: whole DRAM pages instead of 256-byte pieces of them.  The
+// LDS tile [8][TY + 2r][nz] has y halos only (a row is complete in z), so
+// every one-sided z row reads its taps from LDS; the z derivative is
+// branch-free (per-lane coefficient row and tap offset) because rows do not
+// align with 64-lane waves.  One-sided y rows read LDS when their taps lie in
+// the tile, else global memory.  NHT halo entries per thread per plane;
+// rows of at most 256 points.
+template <typename R, typename C, int S, int TY, int NTMAX, int NHT, int WPE>
+__global__ void __launch_bounds__(NTMAX) __attribute__((amdgpu_waves_per_eu(WPE)))
+    bulk_rows3d(Tiled3D ta, const R* __restrict__ f, R* __restrict__ fn, R* __restrict__ df,
+                const double* __restrict__ hs) {
+#pragma clang fp contract(fast)
+  constexpr int r = S - 1;
+  constexpr int W = 2 * r + 1;
+  constexpr int LY = TY + 2 * r;
+  constexpr int NRB = 2 * (S - 1), NTB = 3 * (S - 1);
+  // LDS rows padded to ZS entries (a compile-time stride, so every tap is a
+  // lane base plus an immediate offset)
+  constexpr int ZS = 256;
+  extern __shared__ char smem_raw[];
+  R* tile = reinterpret_cast<R*>(smem_raw);  // [8][LY][ZS]
+  // z derivative coefficients: row 0 interior (taps -r..r, zero padded),
+  // rows 1..NRB left one-sided rows, NRB+1..2NRB right rows (forward order)
+  __shared__ C szc[1 + 2 * NRB][NTB];
+  __shared__ C sfd[NRB][NTB];
+  const int nz = ta.hi[2] - ta.lo[2];
+  const int NT = blockDim.x;
+  const int t = threadIdx.x;
+  for (int e = t; e < (1 + 2 * NRB) * NTB; e += NT) {
+    const int rw = e / NTB, n = e % NTB;
+    C c = 0;
+    if (rw == 0) c = n < W ? (C)c_fd[0][n] : (C)0;
+    else if (rw <= NRB) c = (C)c_fd[rw][n];
+    else c = -(C)c_fd[rw - NRB][NTB - 1 - n];
+    szc[rw][n] = c;
+  }
+  if (t < NRB * NTB) sfd[t / NTB][t % NTB] = (C)c_fd[1 + t / NTB][t % NTB];
+  const int nxcd = 8;
+  const int per = (ta.ntiles + nxcd - 1) / nxcd;
+  const int tile_id = (blockIdx.x % nxcd) * per + blockIdx.x / nxcd;
+  if (tile_id >= ta.ntiles) return;
+  const int yt = tile_id % ta.nyt, xc = tile_id / ta.nyt;
+  const int ty = t / nz, kk = t - ty * nz;
+  const int jb = ta.lo[1] + yt * TY, j = jb + ty, k = ta.lo[2] + kk;
+  const bool active = ty < TY && j < ta.hi[1];
+  const int ib = ta.lo[0] + xc * ta.cx;
+  const int ie = min(ib + ta.cx, ta.hi[0]);
+  const long long N = ta.N;
+  const long long sx = (long long)ta.Ny * ta.Nz, sy = ta.Nz;
+  const int jc = min(j, ta.hi[1] - 1), tyc = min(ty, TY - 1);
+  const unsigned colb = (unsigned)(((long long)jc * sy + k) * (long long)sizeof(R));
+  const unsigned psb = (unsigned)(sx * (long long)sizeof(R));
+  const rsrc_t rf = buf_rsrc(f, 9 * N), rd = buf_rsrc(df, 9 * N), rn = buf_rsrc(fn, 9 * N);
+  const unsigned fb = (unsigned)(N * (long long)sizeof(R));
+  const bool yin = j >= ta.mc[1] && j < ta.mrb[1];
+  // z row of this lane: coefficient row and first tap column
+  int zrow = 0, zoff = -r;
+  if (k < ta.mc[2]) { zrow = kk + 1; zoff = -kk; }
+  else if (k >= ta.mrb[2]) { zrow = NRB + (nz - kk); zoff = nz - NTB - kk; }
+  C kcv[3], kcg2[3], kcl[3], kcg[3], kA, kB;
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    if constexpr (sizeof(C) == 4) {
+      kcv[d] = ta.fcv[d]; kcg2[d] = ta.fcg2[d]; kcl[d] = ta.fcl[d]; kcg[d] = ta.fcg[d];
+    } else {
+      kcv[d] = ta.cv[d]; kcg2[d] = ta.cg2[d]; kcl[d] = ta.cl[d]; kcg[d] = ta.cg[d];
+    }
+  }
+  if constexpr (sizeof(C) == 4) { kA = ta.fA; kB = ta.fB; } else { kA = ta.A; kB = ta.B; }
+  C cd[r + 1];
+#pragma unroll
+  for (int n = 0; n <= r; n++) cd[n] = (C)c_fd[0][r + n];
+  auto xplane = [&](int i) { return (unsigned)min(max(i, ta.lo[0]), ta.hi[0] - 1) * psb; };
+  R win[6][W];
+#pragma unroll
+  for (int n = 0; n < W - 1; n++) {
+    const unsigned pw = xplane(ib - r + n);
+#pragma unroll
+    for (int q = 0; q < 6; q++) win[q][n] = bld<R>(rf, colb, q * fb + pw);
+  }
+  constexpr int tf[8] = {0, 1, 2, 4, 5, 6, 7, 8};  // f index of each tile field
+  constexpr int yq[6] = {0, 1, 2, 4, 6, 7};        // vx vy vz sxy syy syz
+  // halo entries: rows 0..r-1 and TY+r..TY+2r-1 of the tile, all nz columns
+  int hrow[NHT];
+  unsigned hoff[NHT];
+#pragma unroll
+  for (int u = 0; u < NHT; u++) {
+    const int e = t + u * NT;
+    const int hr = e / nz, hk = e - hr * nz;
+    const int lr = hr < r ? hr : TY + hr;
+    hrow[u] = hr < 2 * r ? lr * ZS + hk : -1;
+    const int gj = min(max(jb - r + lr, ta.lo[1]), ta.hi[1] - 1);
+    hoff[u] = (unsigned)(((long long)gj * sy + ta.lo[2] + (hr < 2 * r ? hk : 0)) * (long long)sizeof(R));
+  }
+  R nwin[6], nown[3], nhal[NHT][8];
+  auto issue = [&](int ip) {
+    const unsigned pw = xplane(ip + r), pp = (unsigned)ip * psb;
+#pragma unroll
+    for (int q = 0; q < 6; q++) nwin[q] = bld<R>(rf, colb, q * fb + pw);
+#pragma unroll
+    for (int q = 0; q < 3; q++) nown[q] = bld<R>(rf, colb, (6 + q) * fb + pp);
+#pragma unroll
+    for (int u = 0; u < NHT; u++)
+      if (hrow[u] >= 0) {
+#pragma unroll
+        for (int q = 0; q < 8; q++) nhal[u][q] = bld<R>(rf, hoff[u], tf[q] * fb + pp);
+      }
+  };
+  constexpr int LPL = LY * ZS;  // tile entries per field
+  const int own_at = (tyc + r) * ZS + kk;
+  issue(ib);
+  for (int i = ib; i < ie; i++) {
+    const long long pl = (long long)i * sx;
+    const unsigned plb = (unsigned)i * psb;
+#pragma unroll
+    for (int q = 0; q < 6; q++) win[q][W - 1] = nwin[q];
+    R own[3];
+#pragma unroll
+    for (int q = 0; q < 3; q++) own[q] = nown[q];
+    if (ty < TY) {
+      tile[0 * LPL + own_at] = win[0][r];
+      tile[1 * LPL + own_at] = win[1][r];
+      tile[2 * LPL + own_at] = win[2][r];
+      tile[3 * LPL + own_at] = win[4][r];
+      tile[4 * LPL + own_at] = win[5][r];
+      tile[5 * LPL + own_at] = own[0];
+      tile[6 * LPL + own_at] = own[1];
+      tile[7 * LPL + own_at] = own[2];
+    }
+#pragma unroll
+    for (int u = 0; u < NHT; u++)
+      if (hrow[u] >= 0) {
+#pragma unroll
+        for (int q = 0; q < 8; q++) tile[q * LPL + hrow[u]] = nhal[u][q];
+      }
+    lds_barrier();
+    if (i + 1 < ie) issue(i + 1);
+    R odf[9];
+#pragma unroll
+    for (int q = 0; q < 9; q++) odf[q] = bld<R>(rd, colb, q * fb + plb);
+    if (active) {
+      C dxv[6], dyv[6], dzv[6];
+      if (i >= ta.mc[0] && i < ta.mrb[0]) {
+#pragma unroll
+        for (int q = 0; q < 6; q++)
+          dxv[q] = central<C, r>(cd, [&](int n) { return (C)win[q][n]; });
+      } else {
+        edge_x<R, C, S>(dxv, rf, fb, psb, colb, i, ta.lo[0], ta.hi[0]);
+      }
+      constexpr int ymap[6] = {0, 1, 2, 3, 5, 6}, zmap[6] = {0, 1, 2, 4, 6, 7};
+      if (yin) {
+#pragma unroll
+        for (int q = 0; q < 6; q++)
+          dyv[q] = central<C, r>(cd, [&](int n) { return (C)tile[ymap[q] * LPL + (ty + n) * ZS + kk]; });
+      } else {
+        const bool left = (j - ta.lo[1]) < NRB;
+        const int row = left ? j - ta.lo[1] + 1 : ta.hi[1] - j;
+        const int t0 = left ? ta.lo[1] : ta.hi[1] - NTB;
+        if (t0 >= jb - r && t0 + NTB <= jb + TY + r) {
+#pragma unroll
+          for (int q = 0; q < 6; q++) {
+            C sy_ = 0.;
+#pragma unroll
+            for (int n = 0; n < NTB; n++) {
+              const int tj = left ? ta.lo[1] + n : ta.hi[1] - 1 - n;
+              sy_ += sfd[row - 1][n] * (C)tile[ymap[q] * LPL + (tj - (jb - r)) * ZS + kk];
+            }
+            dyv[q] = left ? sy_ : -sy_;
+          }
+        } else {
+          edge_plane<R, C, S>(dyv, rf, fb, plb, yq, k, (int)sy, j, ta.lo[1], ta.hi[1], sfd);
+        }
+      }
+      {  // z: NTB taps from column kk + zoff with the lane's coefficient row
+        const int zb = (ty + r) * ZS + kk + zoff;
+#pragma unroll
+        for (int q = 0; q < 6; q++) {
+          C sz = szc[zrow][0] * (C)tile[zmap[q] * LPL + zb];
+#pragma unroll
+          for (int n = 1; n < NTB; n++) sz += szc[zrow][n] * (C)tile[zmap[q] * LPL + zb + n];
+          dzv[q] = sz;
+        }
+      }
+      const long long id = pl + (long long)j * sy + k;
+      C acc[9];
+#pragma unroll
+      for (int q = 0; q < 9; q++) acc[q] = (C)odf[q] * kA;
+      acc[0] += kcv[0] * dxv[3] + kcv[1] * dyv[3] + kcv[2] * dzv[3];
+      acc[1] += kcv[0] * dxv[4] + kcv[1] * dyv[4] + kcv[2] * dzv[4];
+      acc[2] += kcv[0] * dxv[5] + kcv[1] * dyv[5] + kcv[2] * dzv[5];
+      acc[3] += kcg2[0] * dxv[0] + kcl[1] * dyv[1] + kcl[2] * dzv[2];
+      acc[4] += kcg[0] * dxv[1] + kcg[1] * dyv[0];
+      acc[5] += kcg[0] * dxv[2] + kcg[2] * dzv[0];
+      acc[6] += kcl[0] * dxv[0] + kcg2[1] * dyv[1] + kcl[2] * dzv[2];
+      acc[7] += kcg[1] * dyv[2] + kcg[2] * dzv[1];
+      acc[8] += kcl[0] * dxv[0] + kcl[1] * dyv[1] + kcg2[2] * dzv[2];
+      C fo[9];
+#pragma unroll
+      for (int q = 0; q < 6; q++) fo[q] = (C)win[q][r];
+      fo[6] = (C)own[0];
+      fo[7] = (C)own[1];
+      fo[8] = (C)own[2];
+      const int c3[3] = {i, j, k};
+#pragma unroll
+      for (int e = 0; e < 6; e++) {
+        if (!ta.face_active[e]) continue;
+        constexpr int sid[3][3] = {{3, 4, 5}, {4, 6, 7}, {5, 7, 8}};
+        const int d = e >> 1, a = d == 0 ? 1 : 0, b = d == 2 ? 1 : 2;
+        if (c3[d] != ((e & 1) ? ta.hi[d] - 1 : ta.lo[d])) continue;
+        const C* m = reinterpret_cast<const C*>(ta.fcoef) + e * 16;
+        const C vd = fo[d], sdd = fo[sid[d][d]];
+        acc[d] += (C)m[0] * vd + (C)m[1] * sdd;
+        acc[sid[d][d]] += (C)m[2] * vd + (C)m[3] * sdd;
+        acc[sid[a][a]] += (C)m[4] * vd + (C)m[5] * sdd;
+        acc[sid[b][b]] += (C)m[6] * vd + (C)m[7] * sdd;
+        const C va = fo[a], sda = fo[sid[d][a]], vb = fo[b], sdb = fo[sid[d][b]];
+        acc[a] += (C)m[8] * va + (C)m[9] * sda;
+        acc[sid[d][a]] += (C)m[10] * va + (C)m[11] * sda;
+        acc[b] += (C)m[12] * vb + (C)m[13] * sdb;
+        acc[sid[d][b]] += (C)m[14] * vb + (C)m[15] * sdb;
+      }
+      for (int e = 0; e < ta.nif; e++) {
+        if (c3[ta.if_dir[e]] != ta.if_fixed[e]) continue;
+        const long long o = (long long)(c3[ta.if_a[e]] - ta.if_orga[e]) * ta.if_n1[e] + (c3[ta.if_b[e]] - ta.if_orgb[e]);
+        const long long n = ta.if_n[e];
+        const double* sn = ta.if_sat[e] + (long long)(ta.if_side[e] * 2 + 0) * 9 * n;
+        const double* sv = sn + 9 * n;
+#pragma unroll
+        for (int q = 0; q < 9; q++) acc[q] -= (C)(sn[q * n + o] + sv[q * n + o]);
+      }
+#pragma unroll
+      for (int q = 0; q < 9; q++) {
+        bst<R>(rd, (R)acc[q], colb, q * fb + plb);
+        C v = fo[q] + kB * acc[q];
+        if (q >= 3 && ta.roundtrip) {
+          double w = (double)v;
+          w += ta.s0[q - 3];
+          if (ta.hetstress) w += hs[(q - 3) * N + id];
+          w -= ta.s0[q - 3];
+          if (ta.hetstress) w -= hs[(q - 3) * N + id];
+          v = (C)w;
+        }
+        bst<R>(rn, (R)v, colb, q * fb + plb);
+      }
+    }
+    lds_barrier();
 #pragma unroll
     for (int n = 0; n < W - 1; n++)
 #pragma unroll

@@ -163,6 +163,41 @@ class Solver : public SolverBase {
   int graph_flip = 0;
 };
 
+// row-tile kernel: FP32 4 rows per workgroup (<= 1024 threads, 4 waves per
+// SIMD), FP64 2 rows (<= 512 threads, 2 waves per SIMD)
+template <typename R>
+struct RowsCfg {
+  static constexpr int TY = sizeof(R) == 4 ? 4 : 2;
+  static constexpr int NTMAX = sizeof(R) == 4 ? 1024 : 512;
+  static constexpr int WPE = sizeof(R) == 4 ? 4 : 2;
+  static constexpr int NHT0 = sizeof(R) == 4 ? 1 : 2;  // halo entries per thread: NHT0 or NHT0 + 1
+};
+
+template <typename R, int S, int NHT>
+static const void* rows_kernel() {
+  using Cf = RowsCfg<R>;
+  return (const void*)bulk_rows3d<R, R, S, Cf::TY, Cf::NTMAX, NHT, Cf::WPE>;
+}
+
+template <typename R>
+static const void* rows_kernel_for(int s, int nht) {
+  constexpr int N0 = RowsCfg<R>::NHT0;
+  if (s == 2) return nht == N0 ? rows_kernel<R, 2, N0>() : rows_kernel<R, 2, N0 + 1>();
+  if (s == 3) return nht == N0 ? rows_kernel<R, 3, N0>() : rows_kernel<R, 3, N0 + 1>();
+  return nht == N0 ? rows_kernel<R, 4, N0>() : rows_kernel<R, 4, N0 + 1>();
+}
+
+template <typename R>
+static void launch_rows(const Tiled3D& ta, int s, const R* f, R* fn, R* df, const double* hs, hipStream_t st) {
+  using Cf = RowsCfg<R>;
+  const int nz = ta.hi[2] - ta.lo[2];
+  const int nt = ((Cf::TY * nz + 63) / 64) * 64;
+  const size_t lds = (size_t)8 * (Cf::TY + 2 * (s - 1)) * 256 * sizeof(R);  // rows padded to 256
+  const dim3 grid((unsigned)(((ta.ntiles + 7) / 8) * 8));
+  void* args[] = {(void*)&ta, (void*)&f, (void*)&fn, (void*)&df, (void*)&hs};
+  HIPCHK(hipLaunchKernel(rows_kernel_for<R>(s, ta.rows_nht), grid, dim3(nt), args, lds, st));
+}
+
 // The external SAT of a face normal to axis d, as the 9x9 map M[row][col] of
 // a point's fields, only couples (v_d, s_dd) into v_d, s_dd, s_aa, s_bb and
 // (v_c, s_dc) into v_c, s_dc for the two other axes c.  Extract those 16
@@ -380,6 +415,29 @@ Solver<R>::Solver(const std::string& path, const fdf_opts& o) {
       ta.hetstress = S.hetstress;
       for (int q = 0; q < 6; q++) ta.s0[q] = S.s0[q];
       for (int e = 0; e < 6; e++) ta.face_active[e] = D.face[e].active;
+      // row-shaped tiles when TY whole rows fit one workgroup and its halo
+      // needs at most NHT0 + 1 entries per thread (FDF_ROWS=0 keeps 64-wide tiles)
+      {
+        using Cf = RowsCfg<R>;
+        const int nz = ta.hi[2] - ta.lo[2], r = S.sbporder - 1;
+        const int nt = ((Cf::TY * nz + 63) / 64) * 64;
+        const int nht = (2 * r * nz + nt - 1) / nt;
+        const size_t lds = (size_t)8 * (Cf::TY + 2 * r) * 256 * sizeof(R);  // rows padded to 256
+        int dev = 0, maxlds = 0;
+        HIPCHK(hipGetDevice(&dev));
+        HIPCHK(hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev));
+        const char* ev = getenv("FDF_ROWS");
+        const bool want = !(ev && atoi(ev) == 0);
+        // (FP32 with sbporder 4 exceeds the 128-register budget of 1024 threads)
+        const bool fits = !(sizeof(R) == 4 && S.sbporder == 4);
+        if (want && fits && nz <= 256 && nt <= Cf::NTMAX && nht <= Cf::NHT0 + 1 && lds + 1024 <= (size_t)maxlds) {
+          ta.rows_nht = std::max(nht, Cf::NHT0);
+          ta.nyt = (ta.hi[1] - ta.lo[1] + Cf::TY - 1) / Cf::TY;
+          ta.ntiles = ta.nyt * ta.nxc;
+          HIPCHK(hipFuncSetAttribute(rows_kernel_for<R>(S.sbporder, ta.rows_nht),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        }
+      }
     }
     if (ok && ok2) {
       for (int d = 0; d < 2; d++) {
@@ -808,6 +866,10 @@ void Solver<R>::launch_tiled2(Tiled2D t2, int tw, int par, hipStream_t st) {
 
 template <typename R>
 void Solver<R>::launch_tiled(Tiled3D ta, int ty) {
+  if (ta.rows_nht > 0) {
+    launch_rows<R>(ta, S.sbporder, f[cur].p, f[cur ^ 1].p, df.p, hs.p, stream);
+    return;
+  }
   ta.nyt = (ta.hi[1] - ta.lo[1] + ty - 1) / ty;
   ta.ntiles = ta.nzt * ta.nyt * ta.nxc;
   if (ty == 2) launch_tiled_ty<R, 2>(ta, S.sbporder, f[cur].p, f[cur ^ 1].p, df.p, hs.p, stream);
