@@ -5,6 +5,10 @@ Metric: Mpt-updates/s per RK stage = N_points * nstages * steps / t (whole
 job, inputs resident in HBM, outputs off), plus the HBM-roofline fraction of
 the dominant kernel (the fused bulk stage kernel) and an in-run CPU baseline
 (the project's FP64 CPU oracle on a bounded sample of the same workload).
+The default line reports the FP64 path as `value` and, in the same run, the
+FP32-bulk / FP64-fault path with its speed ratio and its error against FP64
+after --check-steps steps.  Other workloads: --workload planar2d, rough2d,
+ensemble, stz3d.
 
 Default workload: 3-D mixed-mode planar-fault rupture, two blocks of
 401x201x201 points (grid 401x402x201), dx = 100 m, slip-weakening friction,
@@ -360,6 +364,10 @@ def main():
     ap.add_argument("--compare-precision", action="store_true",
                     help="planar3d: FP32 vs FP64 errors after --compare-steps steps")
     ap.add_argument("--compare-steps", type=int, default=1500)
+    ap.add_argument("--only", action="store_true",
+                    help="planar3d: time only --precision (no second precision, no error check)")
+    ap.add_argument("--check-steps", type=int, default=200,
+                    help="planar3d: FP32 vs FP64 error check after this many steps (0: skip)")
     ap.add_argument("--stz-nx", type=int, default=301)
     ap.add_argument("--stz-ny", type=int, default=151)
     ap.add_argument("--stz-nz", type=int, default=151)
@@ -390,29 +398,19 @@ def main():
         ensemble(args)
         return
     from fdfault_amd import lib
-    prec = lib.FP32 if args.precision == "fp32" else lib.FP64
-    w = 4 if prec == lib.FP32 else 8
+    headline = args.precision
+    other = "fp32" if headline == "fp64" else "fp64"
+    workload = "3d_planar_fault_slipweak_2x%dx%dx%d" % (args.nx, args.ny, args.nz)
+    res, err, cpu = {}, None, None
     with tempfile.TemporaryDirectory() as d:
         path = workload_3d(d, args.nx, args.ny, args.nz, sbporder=args.sbporder)
         print("workload written", flush=True)
-        g = lib.Solver(path, precision=prec)
-        print("opened: N=%d dt=%.6g" % (g.N, g.dt), flush=True)
-        g.step(args.warmup)
-        g.bulk_kernel_ms()
-        t0 = time.perf_counter()
-        g.step(args.steps)
-        t = time.perf_counter() - t0
-        kms, nlaunch = g.bulk_kernel_ms()
-        npts = g.N
-        nst = g.nstages
-        nblocks = g.info.nblocks
-        ntiled = g.info.tiled_blocks
-        g.close()
-        mpts = npts * nst * args.steps / t / 1e6
-        B = bytes_per_point(9, w)
-        pts_per_launch = npts / nblocks
-        achieved = B * pts_per_launch / (kms * 1e-3) / 1e9
-        cpu = None
+        for pname in ((headline,) if args.only else (headline, other)):
+            res[pname] = run_planar(lib, path, pname, args, workload)
+            print(pname, {k: v for k, v in res[pname].items() if k != "roofline"}, flush=True)
+        if args.check_steps > 0 and not args.only:
+            err = fp32_error(lib, path, args.check_steps)
+            print("fp32 vs fp64 after %d steps: %s" % (args.check_steps, err), flush=True)
         if not args.no_cpu:
             sd = os.path.join(d, "cpu")
             os.makedirs(sd)
@@ -423,27 +421,70 @@ def main():
             cpu = {"value": round(cv, 3), "unit": "Mpt-updates/s", "cores": threads, "kind": "port",
                    "sample": "same 3-D workload at 2x%dx%dx%d points (%d pts), %d steps, FP64 CPU oracle, "
                              "OpenMP" % (snx, sny, snz, cn, args.cpu_steps)}
-    workload = "3d_planar_fault_slipweak_2x%dx%dx%d" % (args.nx, args.ny, args.nz)
-    kernel = "bulk_tiled3d" if ntiled == nblocks else "bulk_kernel"
-    from fdfault_amd.profiling import lookup_traffic
-    tr = lookup_traffic("%s/%s/sbp%d" % (workload, args.precision, args.sbporder))
+    h, o = res[headline], res.get(other)
     line = {
         "metric": "Mpt-updates/s per RK stage & % HBM peak; FP32 vs FP64 at <=1e-4 slip err",
-        "value": round(mpts, 2), "unit": "Mpt-updates/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": round(t / args.steps * 1e3, 3), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f64" if w == 8 else "f32",
+        "value": h["value"], "unit": "Mpt-updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": h["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64" if headline == "fp64" else "f32",
         "data": "synthetic",
-        "config": {"workload": workload,
-                   "points": npts, "rk_stages": nst, "sbporder": args.sbporder,
-                   "precision": args.precision},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": tr["traffic"] if tr else None,
-                     "traffic_source": tr["summary"] if tr else None,
-                     "kernel": kernel, "avg_launch_ms": round(kms, 4), "launches": nlaunch,
-                     "bytes_per_point": B, "points_per_launch": int(pts_per_launch)},
+        "config": {"workload": workload, "points": h["points"], "rk_stages": h["rk_stages"],
+                   "sbporder": args.sbporder, "precision": headline},
+        "roofline": h["roofline"],
         "cpu_baseline": cpu,
+        "%s_path" % other: {"value": o["value"], "ms_per_step": o["ms_per_step"], "roofline": o["roofline"]} if o else None,
+        "fp32_over_fp64": round(res["fp32"]["value"] / res["fp64"]["value"], 3) if o else None,
+        "error_fp32_vs_fp64": err,
     }
     print(json.dumps(line), flush=True)
+
+
+def run_planar(lib, path, pname, args, workload):
+    """Timed steps of one precision on the 3-D planar workload; the bulk
+    kernel's average launch time from HIP events on its stream."""
+    prec = lib.FP32 if pname == "fp32" else lib.FP64
+    w = 4 if prec == lib.FP32 else 8
+    g = lib.Solver(path, precision=prec)
+    g.step(args.warmup)
+    g.bulk_kernel_ms()
+    t0 = time.perf_counter()
+    g.step(args.steps)
+    t = time.perf_counter() - t0
+    kms, nlaunch = g.bulk_kernel_ms()
+    npts, nst, nblocks = g.N, g.nstages, g.info.nblocks
+    ntiled, nrows = g.info.tiled_blocks, g.info.row_blocks
+    g.close()
+    B = bytes_per_point(9, w)
+    ppl = npts / nblocks
+    achieved = B * ppl / (kms * 1e-3) / 1e9
+    kernel = ("bulk_rows3d" if nrows == nblocks else "bulk_tiled3d") if ntiled == nblocks else "bulk_kernel"
+    from fdfault_amd.profiling import lookup_traffic
+    tr = lookup_traffic("%s/%s/sbp%d/%s" % (workload, pname, args.sbporder, kernel))
+    return {"value": round(npts * nst * args.steps / t / 1e6, 2), "ms_per_step": round(t / args.steps * 1e3, 3),
+            "points": npts, "rk_stages": nst,
+            "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": tr["traffic"] if tr else None,
+                         "traffic_source": tr["summary"] if tr else None, "kernel": kernel,
+                         "avg_launch_ms": round(kms, 4), "launches": nlaunch, "bytes_per_point": B,
+                         "points_per_launch": int(ppl)}}
+
+
+def fp32_error(lib, path, steps):
+    """FP32 bulk / FP64 fault against FP64 after `steps` steps: max relative
+    error of each field, of final slip and slip rate; front error in steps."""
+    out = {}
+    for prec in (lib.FP64, lib.FP32):
+        g = lib.Solver(path, precision=prec, graph=True)
+        g.step(steps)
+        out[prec] = (g.array("f"), g.iface(0, "U"), g.iface(0, "V"), g.front(0), g.dt)
+        g.close()
+    (f64, u64, v64, t64, dt), (f32, u32, v32, t32, _) = out[lib.FP64], out[lib.FP32]
+    rel = lambda a, b: float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+    both = (t64 >= 0) & (t32 >= 0)
+    return {"steps": steps, "max_field_rel_err": max(rel(f32[q], f64[q]) for q in range(f64.shape[0])),
+            "final_slip_rel_err": rel(u32, u64), "slip_rate_rel_err": rel(v32, v64),
+            "front_err_steps": float(np.abs(t32[both] - t64[both]).max() / dt) if both.any() else 0.,
+            "front_points_ruptured_in_one_only": int(((t64 >= 0) != (t32 >= 0)).sum())}
 
 
 if __name__ == "__main__":

@@ -75,7 +75,8 @@ class SolverBase {
   SetupH S;
   fdf_opts opts{};
   int cur_step = 0;
-  int ntiled = 0;  // blocks on the rectilinear tiled kernel
+  int ntiled = 0;  // blocks on a rectilinear tiled kernel
+  int nrows = 0;   // of which on the row-tile kernel
 };
 
 template <typename R>
@@ -647,7 +648,10 @@ Solver<R>::Solver(const std::string& path, const fdf_opts& o) {
     }
   }
   HIPCHK(hipStreamSynchronize(stream));
-  for (size_t b = 0; b < tiled_ok.size(); b++) ntiled += tiled_ok[b] ? 1 : 0;
+  for (size_t b = 0; b < tiled_ok.size(); b++) {
+    ntiled += tiled_ok[b] ? 1 : 0;
+    nrows += (tiled_ok[b] && nd == 3 && tiles[b].rows_nht > 0) ? 1 : 0;
+  }
   // t = 0: interface solve without SAT (domain.cpp:141-143) on f = 0
   {
     FaultStageDev fs{0., 0., 0., 0., 1, -1., nullptr, 0., 0};
@@ -1926,6 +1930,7 @@ static void fill_info(const fdf::SolverBase* sv, fdf_info* info) {
   info->cfl = S.cfl;
   info->t = sv->cur_step * S.dt;
   info->tiled_blocks = sv->ntiled;
+  info->row_blocks = sv->nrows;
 }
 
 int fdf_get_info(fdf_t* h, fdf_info* info) {

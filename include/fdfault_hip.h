@@ -45,7 +45,8 @@ typedef struct fdf_info {
   int nfields_total; /* including plastic fields                                */
   int nifaces, sbporder, plastic, nblocks, step, precision;
   double dt, ttot, cfl, t;
-  int tiled_blocks;  /* blocks updated by the rectilinear tiled kernel         */
+  int tiled_blocks;  /* blocks updated by a rectilinear tiled kernel           */
+  int row_blocks;    /* of which with row-shaped tiles (bulk_rows3d)          */
 } fdf_info;
 
 /* Replaces problem::problem(const char*) (src/problem.cpp:16-84): parses the
