@@ -1160,6 +1160,7 @@ struct Tiled3D {
   int roundtrip, hetstress;
   double s0[6];
   int rows_nht;                  // > 0: row-tile kernel with this many halo entries per thread
+  int rows_var;                  // row-tile variant: bit 0 double-buffered tile, bit 1 df loaded ahead
   int face_active[6];            // external SAT faces
   // external SAT of a Cartesian face normal to axis d (other axes a < b) as
   // its 16 non-zero linear-map entries, [6 faces][16], in the bulk precision,
@@ -1567,7 +1568,12 @@ __global__ void __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(WP
 // align with 64-lane waves.  One-sided y rows read LDS when their taps lie in
 // the tile, else global memory.  NHT halo entries per thread per plane;
 // rows of at most 256 points.
-template <typename R, typename C, int S, int TY, int NTMAX, int NHT, int WPE>
+//
+// NBUF = 2 double-buffers the LDS tile: plane i's tile is written while
+// plane i-1's buffer may still be read, so one barrier per plane suffices
+// (a wave writing buffer b at plane i+1 has passed plane i's barrier, which
+// every wave reaches only after its plane i-1 reads of b).
+template <typename R, typename C, int S, int TY, int NTMAX, int NHT, int WPE, int NBUF = 1>
 __global__ void __launch_bounds__(NTMAX) __attribute__((amdgpu_waves_per_eu(WPE)))
     bulk_rows3d(Tiled3D ta, const R* __restrict__ f, R* __restrict__ fn, R* __restrict__ df,
                 const double* __restrict__ hs) {
@@ -1580,7 +1586,7 @@ __global__ void __launch_bounds__(NTMAX) __attribute__((amdgpu_waves_per_eu(WPE)
   // lane base plus an immediate offset)
   constexpr int ZS = 256;
   extern __shared__ char smem_raw[];
-  R* tile = reinterpret_cast<R*>(smem_raw);  // [8][LY][ZS]
+  R* const tbase = reinterpret_cast<R*>(smem_raw);  // [NBUF][8][LY][ZS]
   // z derivative coefficients: row 0 interior (taps -r..r, zero padded),
   // rows 1..NRB left one-sided rows, NRB+1..2NRB right rows (forward order)
   __shared__ C szc[1 + 2 * NRB][NTB];
@@ -1674,6 +1680,7 @@ __global__ void __launch_bounds__(NTMAX) __attribute__((amdgpu_waves_per_eu(WPE)
   for (int i = ib; i < ie; i++) {
     const long long pl = (long long)i * sx;
     const unsigned plb = (unsigned)i * psb;
+    R* const tile = tbase + (NBUF == 2 ? ((i - ib) & 1) * (8 * LPL) : 0);
 #pragma unroll
     for (int q = 0; q < 6; q++) win[q][W - 1] = nwin[q];
     R own[3];
@@ -1805,7 +1812,7 @@ __global__ void __launch_bounds__(NTMAX) __attribute__((amdgpu_waves_per_eu(WPE)
         bst<R>(rn, (R)v, colb, q * fb + plb);
       }
     }
-    lds_barrier();
+    if constexpr (NBUF == 1) lds_barrier();  // every wave is done reading this plane's tile
 #pragma unroll
     for (int n = 0; n < W - 1; n++)
 #pragma unroll

@@ -166,6 +166,10 @@ class Solver : public SolverBase {
 
 // row-tile kernel: FP32 4 rows per workgroup (<= 1024 threads, 4 waves per
 // SIMD), FP64 2 rows (<= 512 threads, 2 waves per SIMD)
+// row-kernel variant: 1 = double-buffered LDS tile (FP32; FP64's does not
+// fit the LDS), 0 = single buffer
+static const int rows_var_default = 1;
+
 template <typename R>
 struct RowsCfg {
   static constexpr int TY = sizeof(R) == 4 ? 4 : 2;
@@ -175,17 +179,26 @@ struct RowsCfg {
 };
 
 template <typename R, int S, int NHT>
-static const void* rows_kernel() {
+static const void* rows_kernel(int var) {
   using Cf = RowsCfg<R>;
-  return (const void*)bulk_rows3d<R, R, S, Cf::TY, Cf::NTMAX, NHT, Cf::WPE>;
+  constexpr int NB2 = sizeof(R) == 4 ? 2 : 1;  // an FP64 double-buffered tile exceeds the LDS
+  if (var & 1) return (const void*)bulk_rows3d<R, R, S, Cf::TY, Cf::NTMAX, NHT, Cf::WPE, NB2>;
+  return (const void*)bulk_rows3d<R, R, S, Cf::TY, Cf::NTMAX, NHT, Cf::WPE, 1>;
 }
 
 template <typename R>
-static const void* rows_kernel_for(int s, int nht) {
+static const void* rows_kernel_for(int s, int nht, int var) {
   constexpr int N0 = RowsCfg<R>::NHT0;
-  if (s == 2) return nht == N0 ? rows_kernel<R, 2, N0>() : rows_kernel<R, 2, N0 + 1>();
-  if (s == 3) return nht == N0 ? rows_kernel<R, 3, N0>() : rows_kernel<R, 3, N0 + 1>();
-  return nht == N0 ? rows_kernel<R, 4, N0>() : rows_kernel<R, 4, N0 + 1>();
+  if (s == 2) return nht == N0 ? rows_kernel<R, 2, N0>(var) : rows_kernel<R, 2, N0 + 1>(var);
+  if (s == 3) return nht == N0 ? rows_kernel<R, 3, N0>(var) : rows_kernel<R, 3, N0 + 1>(var);
+  return nht == N0 ? rows_kernel<R, 4, N0>(var) : rows_kernel<R, 4, N0 + 1>(var);
+}
+
+// dynamic LDS of the row-tile kernel: [NBUF][8][TY + 2r][256]
+template <typename R>
+static size_t rows_lds(int s, int var) {
+  const int nb = (sizeof(R) == 4 && (var & 1)) ? 2 : 1;
+  return (size_t)nb * 8 * (RowsCfg<R>::TY + 2 * (s - 1)) * 256 * sizeof(R);
 }
 
 template <typename R>
@@ -193,10 +206,10 @@ static void launch_rows(const Tiled3D& ta, int s, const R* f, R* fn, R* df, cons
   using Cf = RowsCfg<R>;
   const int nz = ta.hi[2] - ta.lo[2];
   const int nt = ((Cf::TY * nz + 63) / 64) * 64;
-  const size_t lds = (size_t)8 * (Cf::TY + 2 * (s - 1)) * 256 * sizeof(R);  // rows padded to 256
+  const size_t lds = rows_lds<R>(s, ta.rows_var);
   const dim3 grid((unsigned)(((ta.ntiles + 7) / 8) * 8));
   void* args[] = {(void*)&ta, (void*)&f, (void*)&fn, (void*)&df, (void*)&hs};
-  HIPCHK(hipLaunchKernel(rows_kernel_for<R>(s, ta.rows_nht), grid, dim3(nt), args, lds, st));
+  HIPCHK(hipLaunchKernel(rows_kernel_for<R>(s, ta.rows_nht, ta.rows_var), grid, dim3(nt), args, lds, st));
 }
 
 // The external SAT of a face normal to axis d, as the 9x9 map M[row][col] of
@@ -417,26 +430,31 @@ Solver<R>::Solver(const std::string& path, const fdf_opts& o) {
       for (int q = 0; q < 6; q++) ta.s0[q] = S.s0[q];
       for (int e = 0; e < 6; e++) ta.face_active[e] = D.face[e].active;
       // row-shaped tiles when TY whole rows fit one workgroup and its halo
-      // needs at most NHT0 + 1 entries per thread (FDF_ROWS=0 keeps 64-wide tiles)
+      // needs at most NHT0 + 1 entries per thread (FDF_ROWS=0 keeps 64-wide
+      // tiles; FDF_ROWS=1 single-buffered, 2 double-buffered LDS tile)
       {
         using Cf = RowsCfg<R>;
         const int nz = ta.hi[2] - ta.lo[2], r = S.sbporder - 1;
         const int nt = ((Cf::TY * nz + 63) / 64) * 64;
         const int nht = (2 * r * nz + nt - 1) / nt;
-        const size_t lds = (size_t)8 * (Cf::TY + 2 * r) * 256 * sizeof(R);  // rows padded to 256
         int dev = 0, maxlds = 0;
         HIPCHK(hipGetDevice(&dev));
         HIPCHK(hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev));
         const char* ev = getenv("FDF_ROWS");
         const bool want = !(ev && atoi(ev) == 0);
+        int var = rows_var_default;
+        if (ev && atoi(ev) > 0) var = (atoi(ev) - 1) & 1;
+        if (rows_lds<R>(S.sbporder, var) + 1024 > (size_t)maxlds) var &= ~1;
+        const size_t lds = rows_lds<R>(S.sbporder, var);
         // (FP32 with sbporder 4 exceeds the 128-register budget of 1024 threads)
         const bool fits = !(sizeof(R) == 4 && S.sbporder == 4);
         if (want && fits && nz <= 256 && nt <= Cf::NTMAX && nht <= Cf::NHT0 + 1 && lds + 1024 <= (size_t)maxlds) {
           ta.rows_nht = std::max(nht, Cf::NHT0);
+          ta.rows_var = var;
           ta.nyt = (ta.hi[1] - ta.lo[1] + Cf::TY - 1) / Cf::TY;
-          ta.ntiles = ta.nyt * ta.nxc;
-          HIPCHK(hipFuncSetAttribute(rows_kernel_for<R>(S.sbporder, ta.rows_nht),
+          HIPCHK(hipFuncSetAttribute(rows_kernel_for<R>(S.sbporder, ta.rows_nht, var),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+          ta.ntiles = ta.nyt * ta.nxc;
         }
       }
     }

@@ -209,6 +209,35 @@ def test_kernel_level_random_state_3d(tmp_path, sbp):
         assert _rel(g.array("f"), o.array("f")) <= 1e-12
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("var", [1, 2])
+def test_row_kernel_variants_random_state(tmp_path, monkeypatch, precision, var):
+    """Both variants of the row-tile kernel (FDF_ROWS=1 single-buffered,
+    2 double-buffered LDS tile) reproduce the oracle's RK stages from a
+    random state."""
+    from fdfault_amd import lib
+    monkeypatch.setenv("FDF_ROWS", str(var))
+    p = write_case(str(tmp_path), "rv", ndim=3, nx=37, ny=22, nz=71, dx=0.1, nt=2, sbporder=3)
+    o = O.Oracle(p)
+    g = lib.Solver(p, precision=lib.FP32 if precision == "fp32" else lib.FP64)
+    assert g.info.row_blocks == 2
+    tol = 1e-12 if precision == "fp64" else 2e-6
+    rng = np.random.default_rng(13)
+    f = rng.normal(size=o.array("f").shape)
+    df = rng.normal(size=o.array("df").shape)
+    for h in (o, g):
+        h.set_array("f", f)
+        h.set_array("df", df)
+    for st in range(o.nstages):
+        o.stage(0, st)
+        g.stage(0, st)
+        edf, ef = _rel(g.array("df"), o.array("df")), _rel(g.array("f"), o.array("f"))
+        print(precision, "FDF_ROWS", var, "stage", st, "df %.2e f %.2e" % (edf, ef), flush=True)
+        assert edf <= tol and ef <= tol
+    g.close()
+    o.close()
+
+
 @pytest.mark.parametrize("plastic", [False, True])
 def test_graph_replay_matches_eager(tmp_path, plastic):
     """hipGraph replay of whole steps (device step counter) gives the same
