@@ -645,11 +645,14 @@ __device__ double stz_newton_from(double xmin, double xmax, double x0, const dou
 // interface::apply_bcs for one fault point (interface.cpp:573-970) with the
 // friction laws of friction.cpp:259-435, slipweak.cpp:206-308 and
 // stz.cpp:263-356, then friction::calc_df/update (friction.cpp:465-505).
-template <int M, typename R>
+// LAW >= 0 fixes the friction law at compile time (only that law's code is
+// generated); LAW = -1 reads it from F.law.
+template <int M, typename R, int LAW = -1>
 __device__ __forceinline__ void fault_point(const FaultDev& F, const GridDev& g, const FaultStageDev& fs,
                                             const R* __restrict__ f, const R* __restrict__ mat,
                                             const double* __restrict__ hs, int* __restrict__ err,
                                             int* __restrict__ newton_iters, int p, const float* mlp) {
+  const int law = LAW >= 0 ? LAW : F.law;
   const int ii = p / F.n1, jj = p % F.n1;
   const long long N = g.N;
   const long long stride[3] = {(long long)g.Ny * g.Nz, (long long)g.Nz, 1};
@@ -734,7 +737,7 @@ __device__ __forceinline__ void fault_point(const FaultDev& F, const GridDev& g,
   double hv11, hs11, hv12, hs12, hv13, hs13, hv22, hs22, hv23, hs23;
   locked(r1.v1, r2.v1, r1.s11, r2.s11, zp1, zp2, hv11, hs11);
   const double hv21 = hv11, hs21 = hs11;
-  if (F.law == 0) {
+  if (law == 0) {
     locked(r1.v2, r2.v2, r1.s12, r2.s12, zs1, zs2, hv12, hs12);
     locked(r1.v3, r2.v3, r1.s13, r2.s13, zs1, zs2, hv13, hs13);
     hv22 = hv12; hs22 = hs12; hv23 = hv13; hs23 = hs13;
@@ -762,7 +765,7 @@ __device__ __forceinline__ void fault_point(const FaultDev& F, const GridDev& g,
     const double phi3 = eta * (s13 / z1 - v13 + s23 / z2 + v23);
     const double phi = sqrt(phi2 * phi2 + phi3 * phi3);
     double V, S, dstatedt = 0.;
-    if (F.law == 2) {  // slipweak::solve_fs / calc_mu
+    if (law == 2) {  // slipweak::solve_fs / calc_mu
       double c0t = 0.;
       for (int k = 0; k < F.nperts; k++) c0t += pert_val(F.pert_xy, F.pert_par, k, 3, n, p, t);
       if (F.param_file) c0t += F.pf[3 * n + p];
@@ -795,7 +798,7 @@ __device__ __forceinline__ void fault_point(const FaultDev& F, const GridDev& g,
     } else {  // friction::solve_fs with frictionless (mu = 0) or stz calc_mu
       if (snc < 0.) {
         double mu = 0.;
-        if (F.law == 3) {
+        if (law == 3) {
           double pr[8] = {phi, eta, snc, F.state[p], 0., 0., 0., 0.};
           for (int k = 0; k < F.nperts; k++) {
             pr[4] += pert_val(F.pert_xy, F.pert_par, k, 0, n, p, t);
@@ -862,7 +865,7 @@ __device__ __forceinline__ void fault_point(const FaultDev& F, const GridDev& g,
         S = 0.;
         V = phi / eta;
       }
-      if (F.law == 3) {  // stz::calc_dstatedt / chihat
+      if (law == 3) {  // stz::calc_dstatedt / chihat
         double c0t = 0., Rt = 0., betat = 0., chiwt = 0., v1t = 0.;
         for (int k = 0; k < F.nperts; k++) {
           c0t += pert_val(F.pert_xy, F.pert_par, k, 4, n, p, t);
@@ -922,7 +925,7 @@ __device__ __forceinline__ void fault_point(const FaultDev& F, const GridDev& g,
         F.dux[o] += fs.dt * F.vx[o];
         F.ux[o] += fs.B * F.dux[o];
       }
-      if (F.law == 3) {
+      if (law == 3) {
         F.dstate[p] = F.dstate[p] * fs.A;
         F.dstate[p] += fs.dt * dstatedt;
         F.state[p] += fs.B * F.dstate[p];
@@ -983,19 +986,26 @@ __device__ __forceinline__ void fault_point(const FaultDev& F, const GridDev& g,
   }
 }
 
-template <int M, typename R>
+// one thread per fault point; LAW is the interface's friction law (0 locked,
+// 1 frictionless, 2 slip weakening, 3 STZ)
+template <int M, typename R, int LAW>
 __global__ void __launch_bounds__(256) fault_kernel(FaultDev F, GridDev g, FaultStageDev fs, const R* __restrict__ f,
                                                     const R* __restrict__ mat, const double* __restrict__ hs,
                                                     int* __restrict__ err, int* __restrict__ newton_iters) {
-  // the proposal network (if any) is read from LDS: one copy per workgroup
-  __shared__ float smlp[MLP_NP];
-  if (F.mlp) {
-    for (int i = threadIdx.x; i < MLP_NP; i += blockDim.x) smlp[i] = F.mlp[i];
-    __syncthreads();
-  }
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= F.n) return;
-  fault_point<M, R>(F, g, fs, f, mat, hs, err, newton_iters, p, F.mlp ? smlp : nullptr);
+  if constexpr (LAW == 3) {
+    // the proposal network (if any) is read from LDS: one copy per workgroup
+    __shared__ float smlp[MLP_NP];
+    if (F.mlp) {
+      for (int i = threadIdx.x; i < MLP_NP; i += blockDim.x) smlp[i] = F.mlp[i];
+      __syncthreads();
+    }
+    if (p >= F.n) return;
+    fault_point<M, R, 3>(F, g, fs, f, mat, hs, err, newton_iters, p, F.mlp ? smlp : nullptr);
+  } else {
+    if (p >= F.n) return;
+    fault_point<M, R, LAW>(F, g, fs, f, mat, hs, err, newton_iters, p, nullptr);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1841,6 +1851,7 @@ struct Tiled2D {
   int Ny;             // y size of the grid (row stride)
   long long N;        // grid points
   int cx;             // x planes per chunk
+  int tw;             // y points per workgroup (the kernel's TW)
   int nyt, nxc, ntiles;
   double cv[2], cg2[2], cl[2], cg[2], A, B;
   float fcv[2], fcg2[2], fcl[2], fcg[2], fA, fB;

@@ -131,6 +131,7 @@ class Solver : public SolverBase {
   std::vector<Tiled3D> tiles;
   std::vector<Tiled2D> tiles2;
   int tile2_tw = 256, tile2_cx = 8;  // 2-D tile: y points per workgroup, x planes per chunk
+  bool tile2_env = false;            // FDF_TILE2_TW / FDF_TILE2_CX fix the 2-D tile
   struct FaceBufs {
     DBuf<double> nrm, dl;
   };
@@ -166,6 +167,24 @@ class Solver : public SolverBase {
 
 // row-tile kernel: FP32 4 rows per workgroup (<= 1024 threads, 4 waves per
 // SIMD), FP64 2 rows (<= 512 threads, 2 waves per SIMD)
+template <typename R, int M>
+static const void* fault_kernel_law(int law) {
+  switch (law) {
+    case 0: return (const void*)fault_kernel<M, R, 0>;
+    case 1: return (const void*)fault_kernel<M, R, 1>;
+    case 2: return (const void*)fault_kernel<M, R, 2>;
+    default: return (const void*)fault_kernel<M, R, 3>;
+  }
+}
+
+// the fault kernel of a mode (0 2-D mode II, 1 mode III, 2 3-D) and law
+template <typename R>
+static const void* fault_kernel_for(int mode, int law) {
+  if (mode == 0) return fault_kernel_law<R, 0>(law);
+  if (mode == 1) return fault_kernel_law<R, 1>(law);
+  return fault_kernel_law<R, 2>(law);
+}
+
 // row-kernel variant: 1 = double-buffered LDS tile (FP32; FP64's does not
 // fit the LDS), 0 = single buffer
 static const int rows_var_default = 1;
@@ -276,9 +295,9 @@ Solver<R>::Solver(const std::string& path, const fdf_opts& o) {
   tile_ty = ty_default;
   if (const char* e = getenv("FDF_TILE_TY")) tile_ty = atoi(e);
   if (tile_ty != 2 && tile_ty != 4 && tile_ty != 8) tile_ty = ty_default;
-  if (const char* e = getenv("FDF_TILE2_TW")) tile2_tw = atoi(e);
+  if (const char* e = getenv("FDF_TILE2_TW")) tile2_tw = atoi(e), tile2_env = true;
   if (tile2_tw != 64 && tile2_tw != 128 && tile2_tw != 256) tile2_tw = 256;
-  if (const char* e = getenv("FDF_TILE2_CX")) tile2_cx = atoi(e) > 0 ? atoi(e) : 8;
+  if (const char* e = getenv("FDF_TILE2_CX")) tile2_cx = atoi(e) > 0 ? atoi(e) : 8, tile2_env = true;
   HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
   for (int e = 0; e < 2 * NEV; e++) HIPCHK(hipEventCreate(&evs[e]));
   const size_t N = S.N;
@@ -468,7 +487,22 @@ Solver<R>::Solver(const std::string& path, const fdf_opts& o) {
       t2.Ny = S.nx[1];
       t2.N = (long long)N;
       t2.cx = tile2_cx;
-      t2.nyt = (t2.hi[1] - t2.lo[1] + tile2_tw - 1) / tile2_tw;
+      t2.tw = tile2_tw;
+      if (!tile2_env) {
+        // small blocks: 64-wide tiles and short x chunks, so that the launch
+        // still has about 16 workgroups per CU (a 2x801x401 block at 256 x 8
+        // would have 202 workgroups for 256 CUs)
+        int dev = 0, ncu = 0;
+        HIPCHK(hipGetDevice(&dev));
+        HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+        const long long nx = t2.hi[0] - t2.lo[0], ny = t2.hi[1] - t2.lo[1];
+        const long long want = 16LL * ncu;
+        if (((ny + 255) / 256) * ((nx + 7) / 8) < want) {
+          t2.tw = 64;
+          t2.cx = (int)std::max(1LL, std::min(8LL, ((ny + 63) / 64) * nx / want));
+        }
+      }
+      t2.nyt = (t2.hi[1] - t2.lo[1] + t2.tw - 1) / t2.tw;
       t2.nxc = (t2.hi[0] - t2.lo[0] + t2.cx - 1) / t2.cx;
       t2.ntiles = t2.nyt * t2.nxc;
       for (int d = 0; d < 2; d++) {
@@ -791,11 +825,11 @@ void Solver<R>::launch_fault(int i, const FaultStageDev& fs) {
   const FaultDev& F = fdev[i];
   const int nb = (F.n + 255) / 256;
   const R* fc = f[cur].p;
-  switch (mode_id()) {
-    case 0: hipLaunchKernelGGL((fault_kernel<0, R>), dim3(nb), dim3(256), 0, stream, F, G, fs, fc, mat.p, hs.p, err.p, nullptr); break;
-    case 1: hipLaunchKernelGGL((fault_kernel<1, R>), dim3(nb), dim3(256), 0, stream, F, G, fs, fc, mat.p, hs.p, err.p, nullptr); break;
-    default: hipLaunchKernelGGL((fault_kernel<2, R>), dim3(nb), dim3(256), 0, stream, F, G, fs, fc, mat.p, hs.p, err.p, nullptr); break;
-  }
+  const void* k = fault_kernel_for<R>(mode_id(), F.law);
+  void* args[] = {(void*)&F, (void*)&G, (void*)&fs, (void*)&fc, (void*)&mat.p, (void*)&hs.p, (void*)&err.p, nullptr};
+  int* no_iters = nullptr;
+  args[7] = (void*)&no_iters;
+  HIPCHK(hipLaunchKernel(k, dim3(nb), dim3(256), args, 0, stream));
   HIPCHK(hipGetLastError());
 }
 
@@ -906,7 +940,7 @@ void Solver<R>::launch_bulk(int b, const StageDev& sd) {
   const bool tiled = tiled_ok[b];
   if (tiled && S.ndim == 2) {
     Tiled2D t2 = stage_tile2(b, sd);
-    launch_tiled2(t2, tile2_tw, cur, stream);
+    launch_tiled2(t2, t2.tw, cur, stream);
     HIPCHK(hipGetLastError());
     return;
   }
