@@ -102,6 +102,7 @@ def single_workload(args):
         fms, _ = g.fault_kernel_ms()
         npts, nst, nb, nt_ = g.N, g.nstages, g.info.nblocks, g.nt
         tiled = g.info.tiled_blocks == nb
+        curv = g.info.curv_blocks == nb
         g.close()
         # throughput: whole time steps replayed as hipGraphs
         g = lib.Solver(path, precision=prec, graph=True)
@@ -130,7 +131,8 @@ def single_workload(args):
                    "minutes_per_full_run": round(t / args.steps * nt_ / 60., 3)},
         "roofline": {"bound": "hbm", "achieved": round(B * ppl / (kms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(B * ppl / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "bulk_tiled2d" if tiled else "bulk_kernel", "avg_launch_ms": round(kms, 4),
+                     "kernel": "bulk_tiled2d" if tiled else ("bulk_curv2d" if curv else "bulk_kernel"),
+                     "avg_launch_ms": round(kms, 4),
                      "launches": nl, "bytes_per_point": B, "points_per_launch": int(ppl),
                      "fault_ms_per_stage": round(fms, 4),
                      "note": "working set %.0f MB: may be resident in the 256 MB Infinity Cache" % (

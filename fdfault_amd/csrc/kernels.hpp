@@ -1859,6 +1859,7 @@ struct Tiled2D {
   double s0[3];       // sxx sxy syy
   int face_active[4];
   const void* fcoef;  // [4 faces][10] external SAT entries in the bulk precision
+  const void* smap[4];  // curvilinear blocks: per face, per point SAT map [n][rows][5] (bulk precision)
   int nif;
   int if_dir[4], if_fixed[4], if_a[4], if_orga[4], if_n[4], if_side[4];
   const double* if_sat[4];
@@ -2074,6 +2075,243 @@ __global__ void __launch_bounds__(TW) bulk_tiled2d_batch(const Tiled2D* __restri
   }
 }
 
+// ---------------------------------------------------------------------------
+// Curvilinear 2-D mode II block (elastic, or plastic with szz), homogeneous
+// material, no dissipation.  Same tiling as tile2d (TW y points per
+// workgroup, x planes marched with a register window, y taps from an LDS
+// row), but metric and Jacobian vary per point.  block.cpp:1048-1278 writes
+// the velocity rates in conservative form, (dt/(rho dx_d J_i)) sum_n c_n J_n
+// (m_d . s)_n, and the stress rates with the point's own metric row times
+// the velocity derivative.  So the x window carries vx, vy and the x fluxes
+// Fx = J (m00 sxx + m01 sxy), Fy = J (m00 sxy + m01 syy) of each plane, and
+// the LDS row carries vx, vy and the y fluxes Gx = J (m10 sxx + m11 sxy),
+// Gy = J (m10 sxy + m11 syy).  With PL the szz rate is nu (rate sxx + rate
+// syy), assigned before the SAT terms (block.cpp:1280-1295).  Points on
+// external faces add the SAT of boundary.cpp:265-492 through its linear map
+// at that point (ta.smap, built by face_map_kernel from ext_sat).
+template <typename R, typename C, int S, int TW, bool PL>
+__global__ void __launch_bounds__(TW) bulk_curv2d(Tiled2D ta, BlockDev bd, const R* __restrict__ mt,
+                                                  const R* __restrict__ jac, int par) {
+#pragma clang fp contract(fast)
+  constexpr int NF = PL ? 6 : 5;
+  constexpr int r = S - 1, W = 2 * r + 1;
+  constexpr int NRB = 2 * (S - 1), NTB = 3 * (S - 1);
+  constexpr int H = NTB;            // LDS halo per side
+  constexpr int LW = TW + 2 * H;
+  __shared__ C row[4][LW];          // vx vy Gx Gy
+  __shared__ C sfd[NRB][NTB];
+  const int nxcd = 8;
+  const int per = (ta.ntiles + nxcd - 1) / nxcd;
+  const int tile_id = (blockIdx.x % nxcd) * per + blockIdx.x / nxcd;
+  if (tile_id >= ta.ntiles) return;
+  const R* __restrict__ f = reinterpret_cast<const R*>(ta.f[par]);
+  R* __restrict__ fn = const_cast<R*>(reinterpret_cast<const R*>(ta.f[par ^ 1]));
+  R* __restrict__ df = reinterpret_cast<R*>(ta.df);
+  const int t = threadIdx.x;
+  if (t < NRB * NTB) sfd[t / NTB][t % NTB] = (C)c_fd[1 + t / NTB][t % NTB];
+  const int yt = tile_id % ta.nyt, xc = tile_id / ta.nyt;
+  const int jb = ta.lo[1] + yt * TW, j = jb + t;
+  const bool active = j < ta.hi[1];
+  const int ib = ta.lo[0] + xc * ta.cx, ie = min(ib + ta.cx, ta.hi[0]);
+  const long long N = ta.N;
+  const int jc = min(j, ta.hi[1] - 1);
+  const unsigned colb = (unsigned)jc * (unsigned)sizeof(R);
+  const unsigned psb = (unsigned)ta.Ny * (unsigned)sizeof(R);
+  const unsigned fb = (unsigned)(N * (long long)sizeof(R));
+  const rsrc_t rf = buf_rsrc(f, NF * N), rd = buf_rsrc(df, NF * N), rn = buf_rsrc(fn, NF * N);
+  const rsrc_t rm = buf_rsrc(mt, 4 * N), rj = buf_rsrc(jac, N);
+  const bool yin = j >= ta.mc[1] && j < ta.mrb[1];
+  C kv[2], kg2l[2], kg[2], kl[2];
+#pragma unroll
+  for (int d = 0; d < 2; d++) {
+    kv[d] = (C)bd.K[d][0];
+    kg2l[d] = (C)bd.K[d][1];
+    kg[d] = (C)bd.K[d][2];
+    kl[d] = (C)bd.K[d][3];
+  }
+  const C kA = (C)ta.A, kB = (C)ta.B, knu = (C)bd.nu;
+  C cd[r + 1];
+#pragma unroll
+  for (int n = 0; n <= r; n++) cd[n] = (C)c_fd[0][r + n];
+  auto xplane = [&](int i) { return (unsigned)min(max(i, ta.lo[0]), ta.hi[0] - 1) * psb; };
+  // x quantities of one plane at this column: vx, vy, Fx, Fy
+  auto xq = [&](unsigned pw, C* o) {
+    const C vx = (C)bld<R>(rf, colb, pw), vy = (C)bld<R>(rf, colb, fb + pw);
+    const C sxx = (C)bld<R>(rf, colb, 2 * fb + pw), sxy = (C)bld<R>(rf, colb, 3 * fb + pw);
+    const C syy = (C)bld<R>(rf, colb, 4 * fb + pw);
+    const C J = (C)bld<R>(rj, colb, pw), m0 = (C)bld<R>(rm, colb, pw), m1 = (C)bld<R>(rm, colb, fb + pw);
+    o[0] = vx;
+    o[1] = vy;
+    o[2] = J * (m0 * sxx + m1 * sxy);
+    o[3] = J * (m0 * sxy + m1 * syy);
+  };
+  // halo entry of this thread (threads 0..2H-1): row position and y index
+  const bool hal = t < 2 * H;
+  const int hpos = t < H ? t : TW + t;
+  const int hj = min(max(jb - H + hpos, ta.lo[1]), ta.hi[1] - 1);
+  const unsigned hoff = (unsigned)hj * (unsigned)sizeof(R);
+  C win[4][W];
+#pragma unroll
+  for (int n = 0; n < W - 1; n++) {
+    C o[4];
+    xq(xplane(ib - r + n), o);
+#pragma unroll
+    for (int q = 0; q < 4; q++) win[q][n] = o[q];
+  }
+  // loads in flight for the next plane: the window's leading plane (vx vy
+  // sxx sxy syy J m00 m01), this column's own values (sxx sxy syy [szz] J
+  // m00 m01 m10 m11) and the halo entry (vx vy sxx sxy syy J m10 m11)
+  R nlead[8], nown[9], nhal[8];
+  auto issue = [&](int ip) {
+    const unsigned pw = xplane(ip + r), pp = (unsigned)ip * psb;
+#pragma unroll
+    for (int q = 0; q < 5; q++) nlead[q] = bld<R>(rf, colb, q * fb + pw);
+    nlead[5] = bld<R>(rj, colb, pw);
+    nlead[6] = bld<R>(rm, colb, pw);
+    nlead[7] = bld<R>(rm, colb, fb + pw);
+#pragma unroll
+    for (int q = 0; q < 3; q++) nown[q] = bld<R>(rf, colb, (2 + q) * fb + pp);
+    nown[3] = PL ? bld<R>(rf, colb, 5 * fb + pp) : (R)0;
+    nown[4] = bld<R>(rj, colb, pp);
+#pragma unroll
+    for (int q = 0; q < 4; q++) nown[5 + q] = bld<R>(rm, colb, q * fb + pp);
+    if (hal) {
+#pragma unroll
+      for (int q = 0; q < 5; q++) nhal[q] = bld<R>(rf, hoff, q * fb + pp);
+      nhal[5] = bld<R>(rj, hoff, pp);
+      nhal[6] = bld<R>(rm, hoff, 2 * fb + pp);
+      nhal[7] = bld<R>(rm, hoff, 3 * fb + pp);
+    }
+  };
+  issue(ib);
+  for (int i = ib; i < ie; i++) {
+    const unsigned plb = (unsigned)i * psb;
+    {
+      const C J = (C)nlead[5], m0 = (C)nlead[6], m1 = (C)nlead[7];
+      const C sxx = (C)nlead[2], sxy = (C)nlead[3], syy = (C)nlead[4];
+      win[0][W - 1] = (C)nlead[0];
+      win[1][W - 1] = (C)nlead[1];
+      win[2][W - 1] = J * (m0 * sxx + m1 * sxy);
+      win[3][W - 1] = J * (m0 * sxy + m1 * syy);
+    }
+    const C sxx = (C)nown[0], sxy = (C)nown[1], syy = (C)nown[2], szz = (C)nown[3];
+    const C Ji = (C)nown[4], m00 = (C)nown[5], m01 = (C)nown[6], m10 = (C)nown[7], m11 = (C)nown[8];
+    row[0][H + t] = win[0][r];
+    row[1][H + t] = win[1][r];
+    row[2][H + t] = Ji * (m10 * sxx + m11 * sxy);
+    row[3][H + t] = Ji * (m10 * sxy + m11 * syy);
+    if (hal) {
+      const C J = (C)nhal[5], m0 = (C)nhal[6], m1 = (C)nhal[7];
+      row[0][hpos] = (C)nhal[0];
+      row[1][hpos] = (C)nhal[1];
+      row[2][hpos] = J * (m0 * (C)nhal[2] + m1 * (C)nhal[3]);
+      row[3][hpos] = J * (m0 * (C)nhal[3] + m1 * (C)nhal[4]);
+    }
+    lds_barrier();
+    if (i + 1 < ie) issue(i + 1);
+    R odf[5];  // (the szz rate is assigned, so its old value is not read)
+#pragma unroll
+    for (int q = 0; q < 5; q++) odf[q] = bld<R>(rd, colb, q * fb + plb);
+    if (active) {
+      C dxv[4], dyv[4];  // d/dx of vx vy Fx Fy; d/dy of vx vy Gx Gy
+      if (i >= ta.mc[0] && i < ta.mrb[0]) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) dxv[q] = central<C, r>(cd, [&](int n) { return win[q][n]; });
+      } else {  // one-sided row along x: taps are whole planes (wave-uniform)
+        const bool left = i - ta.lo[0] < NRB;
+        const int rw = left ? i - ta.lo[0] + 1 : ta.hi[0] - i;
+#pragma unroll
+        for (int n = 0; n < NTB; n++) {
+          const int tap = left ? ta.lo[0] + n : ta.hi[0] - 1 - n;
+          const C c = (C)c_fd[rw][n];
+          C o[4];
+          xq((unsigned)tap * psb, o);
+#pragma unroll
+          for (int q = 0; q < 4; q++) dxv[q] = n == 0 ? c * o[q] : dxv[q] + c * o[q];
+        }
+        if (!left)
+#pragma unroll
+          for (int q = 0; q < 4; q++) dxv[q] = -dxv[q];
+      }
+      if (yin) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) dyv[q] = central<C, r>(cd, [&](int n) { return row[q][t + H - r + n]; });
+      } else {  // one-sided row along y: taps in the LDS row
+        const bool left = j - ta.lo[1] < NRB;
+        const int rw = left ? j - ta.lo[1] + 1 : ta.hi[1] - j;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          C sy = 0.;
+#pragma unroll
+          for (int n = 0; n < NTB; n++) {
+            const int tj = left ? ta.lo[1] + n : ta.hi[1] - 1 - n;
+            sy += sfd[rw - 1][n] * row[q][tj - jb + H];
+          }
+          dyv[q] = left ? sy : -sy;
+        }
+      }
+      C acc[6];
+#pragma unroll
+      for (int q = 0; q < 5; q++) acc[q] = (C)odf[q] * kA;
+      acc[0] += (kv[0] * dxv[2] + kv[1] * dyv[2]) / Ji;
+      acc[1] += (kv[0] * dxv[3] + kv[1] * dyv[3]) / Ji;
+      acc[2] += kg2l[0] * m00 * dxv[0] + kl[0] * m01 * dxv[1] + kg2l[1] * m10 * dyv[0] + kl[1] * m11 * dyv[1];
+      acc[3] += kg[0] * (m00 * dxv[1] + m01 * dxv[0]) + kg[1] * (m10 * dyv[1] + m11 * dyv[0]);
+      acc[4] += kg2l[0] * m01 * dxv[1] + kl[0] * m00 * dxv[0] + kg2l[1] * m11 * dyv[1] + kl[1] * m10 * dyv[0];
+      acc[5] = PL ? knu * (acc[2] + acc[4]) : (C)0;
+      const long long id = (long long)i * ta.Ny + j;
+      const int c2[2] = {i, j};
+      const C fo[6] = {win[0][r], win[1][r], sxx, sxy, syy, szz};
+      // external boundaries of this block, in face order (block.cpp:429-440)
+#pragma unroll
+      for (int fc = 0; fc < 4; fc++) {
+        if (!bd.face[fc].active) continue;
+        const int d = fc >> 1, a = 1 - d;
+        if (c2[d] != ((fc & 1) ? bd.hi[d] - 1 : bd.lo[d])) continue;
+        const R* m = reinterpret_cast<const R*>(ta.smap[fc]) + (long long)(c2[a] - bd.lo[a]) * (NF * 5);
+#pragma unroll
+        for (int p = 0; p < NF; p++) {
+          C sp = (C)m[p * 5] * fo[0];
+#pragma unroll
+          for (int q = 1; q < 5; q++) sp += (C)m[p * 5 + q] * fo[q];
+          acc[p] += sp;
+        }
+      }
+      // interface SAT terms written by the fault kernel (interface.cpp:826-962)
+      for (int e = 0; e < ta.nif; e++) {
+        const int cd_ = ta.if_dir[e] == 0 ? i : j, ca = ta.if_a[e] == 0 ? i : j;
+        if (cd_ != ta.if_fixed[e]) continue;
+        const long long o = ca - ta.if_orga[e];
+        const long long n = ta.if_n[e];
+        const double* sn = ta.if_sat[e] + (long long)(ta.if_side[e] * 2 + 0) * NF * n;
+        const double* sv = sn + NF * n;
+#pragma unroll
+        for (int q = 0; q < NF; q++) acc[q] -= (C)(sn[q * n + o] + sv[q * n + o]);
+      }
+      constexpr int s0i[5] = {-1, -1, 0, 1, 2};
+#pragma unroll
+      for (int q = 0; q < NF; q++) {
+        bst<R>(rd, (R)acc[q], colb, q * fb + plb);
+        C v = fo[q] + kB * acc[q];
+        if (!PL && q >= 2 && ta.roundtrip) {  // set_stress / remove_stress rounding (FP64 elastic only)
+          double w = (double)v;
+          w += ta.s0[s0i[q]];
+          if (ta.hetstress) w += ta.hs[(q - 2) * N + id];
+          w -= ta.s0[s0i[q]];
+          if (ta.hetstress) w -= ta.hs[(q - 2) * N + id];
+          v = (C)w;
+        }
+        bst<R>(rn, (R)v, colb, q * fb + plb);
+      }
+    }
+    lds_barrier();  // every wave is done reading this plane's row
+#pragma unroll
+    for (int n = 0; n < W - 1; n++)
+#pragma unroll
+      for (int q = 0; q < 4; q++) win[q][n] = win[q][n + 1];
+  }
+}
+
 // ensemble fault update: work item = (instance, first fault point of a
 // chunk of 256); the f arrays of the stage parity come with the instance
 struct FaultBatchDev {
@@ -2120,6 +2358,24 @@ __global__ void face_matrix_kernel(FaceDev F, GridDev g, double dt, int u, int v
   double acc[10] = {0., 0., 0., 0., 0., 0., 0., 0., 0., 0.};
   ext_sat<M, double, double>(acc, F, unit, nullptr, gg, q, u, v, dt);
   for (int p = 0; p < NF; p++) out[p * NF + q] = acc[p];
+}
+
+// Linear maps of the external SAT at every point of a face: out[u][p][q] is
+// the rate of field p at face point u per unit of field q (q over the mode's
+// fields; p over nr rows, which include szz for plastic mode II).
+template <int M>
+__global__ void face_map_kernel(FaceDev F, GridDev g, double dt, int n0, int nr, const double* __restrict__ unit,
+                                double* __restrict__ out) {
+  constexpr int NC = Nf<M>::v;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n0 * NC) return;
+  const int u = t / NC, q = t % NC;
+  GridDev gg = g;
+  gg.N = NC;
+  gg.hetmat = 0;
+  double acc[10] = {0., 0., 0., 0., 0., 0., 0., 0., 0., 0.};
+  ext_sat<M, double, double>(acc, F, unit, nullptr, gg, q, u, 0, dt);
+  for (int p = 0; p < nr; p++) out[((long long)u * nr + p) * NC + q] = acc[p];
 }
 
 // SAT terms of the face points of a block whose bulk update came from

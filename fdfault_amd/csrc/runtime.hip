@@ -77,6 +77,7 @@ class SolverBase {
   int cur_step = 0;
   int ntiled = 0;  // blocks on a rectilinear tiled kernel
   int nrows = 0;   // of which on the row-tile kernel
+  int ncurv = 0;   // 2-D blocks on the curvilinear tiled kernel
 };
 
 template <typename R>
@@ -128,6 +129,8 @@ class Solver : public SolverBase {
   int cur = 0;
   std::vector<BlockDev> bdev;
   std::vector<bool> tiled_ok;
+  std::vector<bool> curv_ok;  // 2-D mode II block on the curvilinear tiled kernel
+  std::vector<DBuf<R>> smaps;  // per block and face: the external SAT map of each face point
   std::vector<Tiled3D> tiles;
   std::vector<Tiled2D> tiles2;
   int tile2_tw = 256, tile2_cx = 8;  // 2-D tile: y points per workgroup, x planes per chunk
@@ -397,6 +400,12 @@ Solver<R>::Solver(const std::string& path, const fdf_opts& o) {
     bool ok = (ok3 || ok2) && !S.hetmat && !(S.cdiss > 0.) && !(sizeof(R) == 8 && o.exact_fp64) &&
               (double)S.nf * (double)N * sizeof(R) < 4294967296.;
     for (int d = 0; d < nd && ok; d++) ok = B.mrb[d] > B.mc[d] && B.nx[d] >= 3 * (S.sbporder - 1);
+    // any other 2-D mode II block (curvilinear, or plastic) may take the
+    // curvilinear tiled kernel, which reads the metric per point
+    bool okc = nd == 2 && S.mode == 2 && !S.hetmat && !(S.cdiss > 0.) && !(sizeof(R) == 8 && o.exact_fp64) &&
+               (double)S.nf * (double)N * sizeof(R) < 4294967296. && 4. * (double)N * sizeof(R) < 4294967296. &&
+               !(getenv("FDF_CURV") && atoi(getenv("FDF_CURV")) == 0);
+    for (int d = 0; d < nd && okc; d++) okc = B.mrb[d] > B.mc[d] && B.nx[d] >= 3 * (S.sbporder - 1);
     double mdiag[3] = {0., 0., 0.};
     if (ok) {
       // metric diagonal and uniform, Jacobian uniform, to 1e-9 relative (the
@@ -477,7 +486,7 @@ Solver<R>::Solver(const std::string& path, const fdf_opts& o) {
         }
       }
     }
-    if (ok && ok2) {
+    if ((ok && ok2) || okc) {
       for (int d = 0; d < 2; d++) {
         t2.lo[d] = B.mlb[d];
         t2.hi[d] = B.prb[d];
@@ -522,6 +531,7 @@ Solver<R>::Solver(const std::string& path, const fdf_opts& o) {
       for (int e = 0; e < 4; e++) t2.face_active[e] = D.face[e].active;
     }
     tiled_ok.push_back(ok);
+    curv_ok.push_back(okc);
     tiles.push_back(ta);
     tiles2.push_back(t2);
   }
@@ -700,7 +710,52 @@ Solver<R>::Solver(const std::string& path, const fdf_opts& o) {
     }
   }
   HIPCHK(hipStreamSynchronize(stream));
+  for (size_t b = 0; b < curv_ok.size(); b++) {
+    // the rectilinear kernel is preferred where it applies
+    curv_ok[b] = curv_ok[b] && !tiled_ok[b] && bdev[b].nif <= 4;
+    if (!curv_ok[b]) continue;
+    const BlockDev& D = bdev[b];
+    Tiled2D& t2 = tiles2[b];
+    t2.nif = D.nif;
+    for (int e = 0; e < D.nif; e++) {
+      const IfFaceDev& X = D.ifc[e];
+      t2.if_dir[e] = X.dir;
+      t2.if_fixed[e] = X.fixed;
+      t2.if_a[e] = X.a;
+      t2.if_orga[e] = X.org_a;
+      t2.if_n[e] = X.n;
+      t2.if_side[e] = X.side;
+      t2.if_sat[e] = X.sat;
+    }
+    // the SAT of every external face point as a [rows][5] map (rows: the
+    // evolved fields, szz included for plastic mode II)
+    smaps.resize(bdev.size() * 4);
+    const int nr = S.nf;
+    std::vector<double> unit(25, 0.);
+    for (int q = 0; q < 5; q++) unit[q * 5 + q] = 1.;
+    DBuf<double> dunit5;
+    dunit5.upload(unit);
+    for (int fc = 0; fc < 4; fc++) {
+      t2.smap[fc] = nullptr;
+      if (!D.face[fc].active) continue;
+      if (D.face[fc].n1 != 1) throw Error{"2-D face with more than one point across"};
+      const int n = D.face[fc].n0;
+      DBuf<double> out;
+      out.alloc((size_t)n * nr * 5);
+      hipLaunchKernelGGL(face_map_kernel<0>, dim3((n * 5 + 255) / 256), dim3(256), 0, stream, D.face[fc], G, S.dt, n,
+                         nr, dunit5.p, out.p);
+      HIPCHK(hipGetLastError());
+      std::vector<double> h((size_t)n * nr * 5);
+      HIPCHK(hipMemcpyAsync(h.data(), out.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+      HIPCHK(hipStreamSynchronize(stream));
+      std::vector<R> hr(h.size());
+      for (size_t q = 0; q < h.size(); q++) hr[q] = (R)h[q];
+      smaps[b * 4 + fc].upload(hr);
+      t2.smap[fc] = smaps[b * 4 + fc].p;
+    }
+  }
   for (size_t b = 0; b < tiled_ok.size(); b++) {
+    ncurv += curv_ok[b] ? 1 : 0;
     ntiled += tiled_ok[b] ? 1 : 0;
     nrows += (tiled_ok[b] && nd == 3 && tiles[b].rows_nht > 0) ? 1 : 0;
   }
@@ -895,6 +950,22 @@ static void launch_tiled2_tw(const Tiled2D& t2, int s, int par, hipStream_t st) 
   else hipLaunchKernelGGL((bulk_tiled2d<R, R, 4, TW>), grid, blk, 0, st, t2, par);
 }
 
+template <typename R, int TW, bool PL>
+static void launch_curv2_pl(const Tiled2D& t2, const BlockDev& D, const R* mt, const R* jac, int s, int par,
+                            hipStream_t st) {
+  const dim3 grid((unsigned)(((t2.ntiles + 7) / 8) * 8)), blk(TW);
+  if (s == 2) hipLaunchKernelGGL((bulk_curv2d<R, R, 2, TW, PL>), grid, blk, 0, st, t2, D, mt, jac, par);
+  else if (s == 3) hipLaunchKernelGGL((bulk_curv2d<R, R, 3, TW, PL>), grid, blk, 0, st, t2, D, mt, jac, par);
+  else hipLaunchKernelGGL((bulk_curv2d<R, R, 4, TW, PL>), grid, blk, 0, st, t2, D, mt, jac, par);
+}
+
+template <typename R, int TW>
+static void launch_curv2_tw(const Tiled2D& t2, const BlockDev& D, const R* mt, const R* jac, int s, bool plastic,
+                            int par, hipStream_t st) {
+  if (plastic) launch_curv2_pl<R, TW, true>(t2, D, mt, jac, s, par, st);
+  else launch_curv2_pl<R, TW, false>(t2, D, mt, jac, s, par, st);
+}
+
 // stage descriptor of a 2-D rectilinear block: RK coefficients and arrays
 template <typename R>
 Tiled2D Solver<R>::stage_tile2(int b, const StageDev& sd) const {
@@ -941,6 +1012,16 @@ void Solver<R>::launch_bulk(int b, const StageDev& sd) {
   if (tiled && S.ndim == 2) {
     Tiled2D t2 = stage_tile2(b, sd);
     launch_tiled2(t2, t2.tw, cur, stream);
+    HIPCHK(hipGetLastError());
+    return;
+  }
+  if (curv_ok[b]) {
+    Tiled2D t2 = stage_tile2(b, sd);
+    t2.nyt = (t2.hi[1] - t2.lo[1] + t2.tw - 1) / t2.tw;
+    t2.ntiles = t2.nyt * t2.nxc;
+    if (t2.tw == 64) launch_curv2_tw<R, 64>(t2, D, metric.p, jac.p, S.sbporder, S.plastic, cur, stream);
+    else if (t2.tw == 128) launch_curv2_tw<R, 128>(t2, D, metric.p, jac.p, S.sbporder, S.plastic, cur, stream);
+    else launch_curv2_tw<R, 256>(t2, D, metric.p, jac.p, S.sbporder, S.plastic, cur, stream);
     HIPCHK(hipGetLastError());
     return;
   }
@@ -1983,6 +2064,7 @@ static void fill_info(const fdf::SolverBase* sv, fdf_info* info) {
   info->t = sv->cur_step * S.dt;
   info->tiled_blocks = sv->ntiled;
   info->row_blocks = sv->nrows;
+  info->curv_blocks = sv->ncurv;
 }
 
 int fdf_get_info(fdf_t* h, fdf_info* info) {

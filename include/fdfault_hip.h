@@ -47,6 +47,7 @@ typedef struct fdf_info {
   double dt, ttot, cfl, t;
   int tiled_blocks;  /* blocks updated by a rectilinear tiled kernel           */
   int row_blocks;    /* of which with row-shaped tiles (bulk_rows3d)          */
+  int curv_blocks;   /* 2-D mode II blocks on the curvilinear tiled kernel      */
 } fdf_info;
 
 /* Replaces problem::problem(const char*) (src/problem.cpp:16-84): parses the

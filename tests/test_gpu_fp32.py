@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 import oracle_lib as O
-from cases import write_case
+from cases import curved_surfaces, write_case
 from fdfault_amd.problems import Pert
 
 pytestmark = pytest.mark.gpu
@@ -51,3 +51,17 @@ def test_fp32_3d_slip(tmp_path):
     err_u, dt_err, mismatch = _compare(p, 30)
     assert err_u <= 1e-4
     assert dt_err <= 1.0
+
+
+def test_fp32_curved_plastic_slip(tmp_path):
+    """FP32 on the curvilinear tiled kernel: rough plastic fault, prestress
+    from s0 (sxy 52 over syy -120), nucleation +30 MPa of shear."""
+    loads = [Pert("boxcar", 0., 8.0, 1.5, 0., 0., (0., 30., 0.))]
+    surf = curved_surfaces(str(tmp_path), 161, 81, 0.1, amp=0.2)
+    p = write_case(str(tmp_path), "f32c", nx=161, ny=81, dx=0.1, nt=100, loads=loads, front=("V", 0.001),
+                   surf=surf, material="plastic", plastic=dict(mu=0.7, c=0., beta=0.35, eta=0.05 * 32.04),
+                   s0=(-100., 52., 0., -120., 0., -55.))
+    err_u, dt_err, mismatch = _compare(p, 100)
+    assert err_u <= 1e-4
+    assert dt_err <= 1.0
+    assert mismatch <= 2

@@ -209,6 +209,49 @@ def test_kernel_level_random_state_3d(tmp_path, sbp):
         assert _rel(g.array("f"), o.array("f")) <= 1e-12
 
 
+@pytest.mark.parametrize("plastic", [False, True])
+def test_kernel_level_random_state_curvilinear(tmp_path, plastic):
+    """One RK stage from a random state on curvilinear 2-D blocks, elastic and
+    plastic: the curvilinear tiled kernel (per-point metric, conservative
+    velocity rates, per-point external SAT maps, szz rate) against the
+    oracle's df and f."""
+    from fdfault_amd import lib
+    nx, ny = 61, 31
+    surf = curved_surfaces(str(tmp_path), nx, ny, 0.1, amp=0.15)
+    kw = dict(surf=surf)
+    if plastic:
+        kw.update(material="plastic", plastic=dict(mu=0.68, c=0., beta=0.35, eta=0.05 * 32.04),
+                  s0=(-100., 52., 0., -120., 0., -55.))
+    p = write_case(str(tmp_path), "rc", nx=nx, ny=ny, dx=0.1, nt=2, **kw)
+    o = O.Oracle(p)
+    g = lib.Solver(p)
+    assert g.info.curv_blocks == 2 and g.info.tiled_blocks == 0
+    rng = np.random.default_rng(17)
+    f = rng.normal(size=o.array("f").shape)
+    df = rng.normal(size=o.array("df").shape)
+    for h in (o, g):
+        h.set_array("f", f)
+        h.set_array("df", df)
+    for st in range(o.nstages):
+        o.stage(0, st)
+        g.stage(0, st)
+        edf, ef = _rel(g.array("df"), o.array("df")), _rel(g.array("f"), o.array("f"))
+        print("plastic", plastic, "stage", st, "df %.2e f %.2e" % (edf, ef), flush=True)
+        assert edf <= 1e-12 and ef <= 1e-12
+    g.close()
+    o.close()
+
+
+def test_mode2_curved_plastic(tmp_path):
+    """Rough-fault plastic rupture on curvilinear blocks (the curvilinear
+    tiled kernel with the plastic return map and friction) against the oracle."""
+    surf = curved_surfaces(str(tmp_path), 61, 31, 0.1, amp=0.1)
+    plastic = dict(mu=0.68, c=0., beta=0.35, eta=0.05 * 32.04)
+    p = write_case(str(tmp_path), "cpl", nx=61, ny=31, dx=0.1, nt=15, surf=surf, material="plastic",
+                   plastic=plastic, s0=(-100., 52., 0., -120., 0., -55.), loads=NUC2)
+    _check(p, 15)
+
+
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 @pytest.mark.parametrize("var", [1, 2])
 def test_row_kernel_variants_random_state(tmp_path, monkeypatch, precision, var):
