@@ -118,22 +118,30 @@ def single_workload(args):
             cpu = {"value": round(cv, 3), "unit": "Mpt-updates/s", "cores": threads, "kind": "port",
                    "sample": "the same problem (%d pts), %d steps, FP64 CPU oracle, OpenMP" % (cn, args.cpu_steps)}
     nf = 6 if rough else 5
-    B = bytes_per_point(nf, w, curvilinear=rough, ndim=2, plastic_words=2 * (4 + 2) if rough else 0)
+    # the bulk kernel's own algorithmic bytes (the plastic fields are read and
+    # written once per step by the plasticity kernel, not by this kernel)
+    B = bytes_per_point(nf, w, curvilinear=rough, ndim=2)
+    B_stage = bytes_per_point(nf, w, curvilinear=rough, ndim=2, plastic_words=2 * (4 + 2) if rough else 0)
     ppl = npts / nb
+    kernel = "bulk_tiled2d" if tiled else ("bulk_curv2d" if curv else "bulk_kernel")
+    wname = "rough2d_plastic_2x1601x801" if rough else "planar2d_2x801x401"
+    from fdfault_amd.profiling import lookup_traffic
+    tr = lookup_traffic("%s/%s/sbp%d/%s" % (wname, args.precision, 4 if rough else 3, kernel))
     line = {
         "metric": "Mpt-updates/s per RK stage (%s)" % ("2-D rough-fault plastic" if rough else "2-D planar fault"),
         "value": round(npts * nst * args.steps / t / 1e6, 2), "unit": "Mpt-updates/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(t / args.steps * 1e3, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64" if w == 8 else "f32",
         "data": "synthetic",
-        "config": {"workload": ("rough2d_plastic_2x1601x801" if rough else "planar2d_2x801x401"), "points": npts,
+        "config": {"workload": wname, "points": npts,
                    "rk_stages": nst, "sbporder": 4 if rough else 3, "precision": args.precision, "nt": nt_,
                    "minutes_per_full_run": round(t / args.steps * nt_ / 60., 3)},
         "roofline": {"bound": "hbm", "achieved": round(B * ppl / (kms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(B * ppl / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "bulk_tiled2d" if tiled else ("bulk_curv2d" if curv else "bulk_kernel"),
-                     "avg_launch_ms": round(kms, 4),
-                     "launches": nl, "bytes_per_point": B, "points_per_launch": int(ppl),
+                     "unit": "GB/s", "frac": round(B * ppl / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                     "traffic": tr["traffic"] if tr else None, "traffic_source": tr["summary"] if tr else None,
+                     "kernel": kernel, "avg_launch_ms": round(kms, 4),
+                     "launches": nl, "bytes_per_point": B, "bytes_per_point_stage_with_plasticity": B_stage,
+                     "points_per_launch": int(ppl),
                      "fault_ms_per_stage": round(fms, 4),
                      "note": "working set %.0f MB: may be resident in the 256 MB Infinity Cache" % (
                          npts * (4 * nf + (5 if rough else 0)) * w / 1e6)},
@@ -344,7 +352,9 @@ def ensemble(args):
         "vs_baseline": None, "dtype": "f64" if w == 8 else "f32", "data": "synthetic",
         "config": {"workload": "ensemble_2d_slipweak_%d_instances" % len(paths), "instances": len(paths),
                    "points": npts, "nt_per_rupture": args.ens_nt, "sbporder": 3, "precision": args.precision,
-                   "tile_width": tun["tile_width"], "x_chunk": tun["chunk"], "tuned_from_cache": tun["from_cache"]},
+                   "tile_width": tun["tile_width"], "x_chunk": tun["chunk"],
+                   "launch": "persistent grid" if tun["persistent"] else "one workgroup per tile",
+                   "tuned_from_cache": tun["from_cache"]},
         "mpt_updates_per_s": round(npts * nst * args.steps / t / 1e6, 1),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,

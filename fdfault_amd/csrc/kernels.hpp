@@ -2066,10 +2066,14 @@ __global__ void __launch_bounds__(TW) bulk_tiled2d(Tiled2D ta, int par) {
 
 // ensemble: a persistent grid walks the work list (block descriptor, tile)
 // of every instance; uneven instance sizes only change the list length
+// (the grid is a multiple of 8: workgroup b runs on XCD b % 8, and the items
+// a round of workgroups takes are mapped so that each XCD gets a contiguous
+// run of them, neighbouring y tiles sharing that XCD's L2)
 template <typename R, typename C, int S, int TW>
 __global__ void __launch_bounds__(TW) bulk_tiled2d_batch(const Tiled2D* __restrict__ descs,
                                                         const int2* __restrict__ work, int nwork, int par) {
-  for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
+  const int first = (blockIdx.x % 8) * (gridDim.x / 8) + blockIdx.x / 8;
+  for (int w = first; w < nwork; w += gridDim.x) {
     const int2 it = work[w];
     tile2d<R, C, S, TW>(descs[it.x], it.y, par);
   }

@@ -133,7 +133,7 @@ class Solver : public SolverBase {
   std::vector<DBuf<R>> smaps;  // per block and face: the external SAT map of each face point
   std::vector<Tiled3D> tiles;
   std::vector<Tiled2D> tiles2;
-  int tile2_tw = 256, tile2_cx = 8;  // 2-D tile: y points per workgroup, x planes per chunk
+  int tile2_tw = 64, tile2_cx = 2;  // 2-D tile: y points per workgroup, x planes per chunk
   bool tile2_env = false;            // FDF_TILE2_TW / FDF_TILE2_CX fix the 2-D tile
   struct FaceBufs {
     DBuf<double> nrm, dl;
@@ -299,8 +299,8 @@ Solver<R>::Solver(const std::string& path, const fdf_opts& o) {
   if (const char* e = getenv("FDF_TILE_TY")) tile_ty = atoi(e);
   if (tile_ty != 2 && tile_ty != 4 && tile_ty != 8) tile_ty = ty_default;
   if (const char* e = getenv("FDF_TILE2_TW")) tile2_tw = atoi(e), tile2_env = true;
-  if (tile2_tw != 64 && tile2_tw != 128 && tile2_tw != 256) tile2_tw = 256;
-  if (const char* e = getenv("FDF_TILE2_CX")) tile2_cx = atoi(e) > 0 ? atoi(e) : 8, tile2_env = true;
+  if (tile2_tw != 64 && tile2_tw != 128 && tile2_tw != 256) tile2_tw = 64;
+  if (const char* e = getenv("FDF_TILE2_CX")) tile2_cx = atoi(e) > 0 ? atoi(e) : 2, tile2_env = true;
   HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
   for (int e = 0; e < 2 * NEV; e++) HIPCHK(hipEventCreate(&evs[e]));
   const size_t N = S.N;
@@ -498,18 +498,14 @@ Solver<R>::Solver(const std::string& path, const fdf_opts& o) {
       t2.cx = tile2_cx;
       t2.tw = tile2_tw;
       if (!tile2_env) {
-        // small blocks: 64-wide tiles and short x chunks, so that the launch
-        // still has about 16 workgroups per CU (a 2x801x401 block at 256 x 8
-        // would have 202 workgroups for 256 CUs)
+        // 64-wide tiles, 2-plane chunks (short marches keep the DRAM access
+        // stream dense: measured best on 2x1601x801 and 2x801x401); blocks
+        // that would give fewer than 16 workgroups per CU march one plane
         int dev = 0, ncu = 0;
         HIPCHK(hipGetDevice(&dev));
         HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
         const long long nx = t2.hi[0] - t2.lo[0], ny = t2.hi[1] - t2.lo[1];
-        const long long want = 16LL * ncu;
-        if (((ny + 255) / 256) * ((nx + 7) / 8) < want) {
-          t2.tw = 64;
-          t2.cx = (int)std::max(1LL, std::min(8LL, ((ny + 63) / 64) * nx / want));
-        }
+        if (((ny + t2.tw - 1) / t2.tw) * ((nx + t2.cx - 1) / t2.cx) < 16LL * ncu) t2.cx = 1;
       }
       t2.nyt = (t2.hi[1] - t2.lo[1] + t2.tw - 1) / t2.tw;
       t2.nxc = (t2.hi[0] - t2.lo[0] + t2.cx - 1) / t2.cx;
@@ -1412,10 +1408,12 @@ double Solver<R>::bulk_ms(int* n) {
 // instance is set up exactly like a single run (parse, grid, t = 0 solve,
 // outputs); the batch then replaces the per-instance stage launches by two
 // launches per RK stage over every instance: fault_batch_kernel (all fault
-// points) and the persistent bulk_tiled2d_batch (all tiles of all blocks).
+// points) and bulk_tiled2d_batch (all tiles of all blocks), the latter either
+// as a persistent grid walking the work list or with one workgroup per item.
 // Stage descriptors are precomputed per RK stage, so one time step is a fixed
-// launch sequence that replays from a hipGraph.  Tile width and x-chunk come
-// from a timing autotuner run at open (cached per device and ensemble).
+// launch sequence that replays from a hipGraph.  Tile width, x-chunk and
+// launch mode come from a timing autotuner run at open (cached per device and
+// ensemble).
 class BatchBase {
  public:
   virtual ~BatchBase() {}
@@ -1425,6 +1423,7 @@ class BatchBase {
   virtual void sync() = 0;
   virtual void finish() = 0;
   int tw = 0, cx = 0;
+  int persist = 1;  // 1: persistent grid over the work list, 0: one workgroup per item
   double tune_ms = 0.;
   int tuned_from_cache = 0;
 };
@@ -1441,9 +1440,9 @@ class Batch : public BatchBase {
   void finish() override;
 
  private:
-  void build(int tw_, int cx_);
+  void build(int tw_, int cx_, int persist_);
   void launch_stage(int st);
-  double time_config(int tw_, int cx_);
+  double time_config(int tw_, int cx_, int persist_);
   void autotune();
   template <int TW>
   void launch_bulk_tw(int st);
@@ -1530,9 +1529,10 @@ Batch<R>::~Batch() {
 
 // per-stage descriptors and the tile work list for tile width tw_, chunk cx_
 template <typename R>
-void Batch<R>::build(int tw_, int cx_) {
+void Batch<R>::build(int tw_, int cx_, int persist_) {
   tw = tw_;
   cx = cx_;
+  persist = persist_;
   std::vector<int2> wl;
   for (int st = 0; st < nst; st++) {
     std::vector<Tiled2D> d;
@@ -1553,14 +1553,15 @@ void Batch<R>::build(int tw_, int cx_) {
   }
   work.upload(wl);
   nwork = (int)wl.size();
-  // persistent grid: as many workgroups as the device keeps resident
+  // persistent grid: as many workgroups as the device keeps resident (a
+  // multiple of 8 for the XCD mapping); otherwise one per work item
   int dev = 0, ncu = 0, occ = 0;
   HIPCHK(hipGetDevice(&dev));
   HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
   if (tw == 64) HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, bulk_tiled2d_batch<R, R, 3, 64>, 64, 0));
   else if (tw == 128) HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, bulk_tiled2d_batch<R, R, 3, 128>, 128, 0));
   else HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, bulk_tiled2d_batch<R, R, 3, 256>, 256, 0));
-  grid = std::max(1, std::min(nwork, ncu * std::max(occ, 1)));
+  grid = persist ? std::max(8, (std::min(nwork, ncu * std::max(occ, 1)) / 8) * 8) : ((nwork + 7) / 8) * 8;
   for (int q = 0; q < 2; q++)
     if (graphs[q]) {
       HIPCHK(hipGraphExecDestroy(graphs[q]));
@@ -1601,8 +1602,8 @@ void Batch<R>::launch_stage(int st) {
 // df is saved and restored around the trial launches, f_new is overwritten
 // by the next stage anyway)
 template <typename R>
-double Batch<R>::time_config(int tw_, int cx_) {
-  build(tw_, cx_);
+double Batch<R>::time_config(int tw_, int cx_, int persist_) {
+  build(tw_, cx_, persist_);
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
@@ -1643,27 +1644,30 @@ void Batch<R>::autotune() {
     mix((unsigned long long)I->S.blocks.size());
   }
   char key[256];
-  snprintf(key, sizeof(key), "%s:%s:s%d:n%zu:%016llx", prop.gcnArchName, sizeof(R) == 4 ? "fp32" : "fp64", sbp,
+  snprintf(key, sizeof(key), "v2:%s:%s:s%d:n%zu:%016llx", prop.gcnArchName, sizeof(R) == 4 ? "fp32" : "fp64", sbp,
            inst.size(), h);
   std::string cache;
   if (const char* e = getenv("FDF_TUNE_CACHE")) cache = e;
   else if (const char* home = getenv("HOME")) cache = std::string(home) + "/.cache/fdfault_amd_tune.txt";
-  int ftw = 0, fcx = 0;
+  int ftw = 0, fcx = 0, fp = -1;
   if (const char* e = getenv("FDF_BATCH_TW")) ftw = atoi(e);
   if (const char* e = getenv("FDF_BATCH_CX")) fcx = atoi(e);
+  if (const char* e = getenv("FDF_BATCH_PERSIST")) fp = atoi(e) ? 1 : 0;
   if (ftw && fcx) {  // fixed by the caller
-    build(ftw, fcx);
+    build(ftw, fcx, fp < 0 ? 1 : fp);
     return;
   }
-  if (!cache.empty()) {
+  if (!cache.empty() && fp < 0) {
     if (FILE* fh = fopen(cache.c_str(), "r")) {
-      char k[256];
-      int a = 0, c = 0;
+      char line[512], k[256];
+      int a = 0, c = 0, m = 0;
       double ms = 0.;
-      while (fscanf(fh, "%255s %d %d %lf", k, &a, &c, &ms) == 4)
-        if (!strcmp(k, key) && (a == 64 || a == 128 || a == 256) && c > 0) {
+      while (fgets(line, sizeof(line), fh))
+        if (sscanf(line, "%255s %d %d %d %lf", k, &a, &c, &m, &ms) == 5 && !strcmp(k, key) &&
+            (a == 64 || a == 128 || a == 256) && c > 0 && (m == 0 || m == 1)) {
           ftw = a;
           fcx = c;
+          fp = m;
           tune_ms = ms;
         }
       fclose(fh);
@@ -1671,7 +1675,7 @@ void Batch<R>::autotune() {
   }
   if (ftw) {
     tuned_from_cache = 1;
-    build(ftw, fcx);
+    build(ftw, fcx, fp);
     return;
   }
   // save df of every instance, time the candidates, restore
@@ -1681,24 +1685,28 @@ void Batch<R>::autotune() {
     HIPCHK(hipMemcpyAsync(saved[i].p, inst[i]->df.p, inst[i]->df.n * sizeof(R), hipMemcpyDeviceToDevice, stream));
   }
   double best = 1e300;
-  int btw = 256, bcx = 8;
-  for (int a : {64, 128, 256})
-    for (int c : {2, 4, 8, 16, 32}) {
-      const double ms = time_config(a, c);
-      if (ms < best) {
-        best = ms;
-        btw = a;
-        bcx = c;
+  int btw = 256, bcx = 8, bp = 1;
+  for (int m : {1, 0}) {
+    if (fp >= 0 && m != fp) continue;
+    for (int a : {64, 128, 256})
+      for (int c : {1, 2, 4, 8, 16}) {
+        const double ms = time_config(a, c, m);
+        if (ms < best) {
+          best = ms;
+          btw = a;
+          bcx = c;
+          bp = m;
+        }
       }
-    }
+  }
   for (size_t i = 0; i < inst.size(); i++)
     HIPCHK(hipMemcpyAsync(inst[i]->df.p, saved[i].p, inst[i]->df.n * sizeof(R), hipMemcpyDeviceToDevice, stream));
   HIPCHK(hipStreamSynchronize(stream));
   tune_ms = best;
-  build(btw, bcx);
-  if (!cache.empty())
+  build(btw, bcx, bp);
+  if (!cache.empty() && fp < 0)
     if (FILE* fh = fopen(cache.c_str(), "a")) {
-      fprintf(fh, "%s %d %d %.6f\n", key, btw, bcx, best);
+      fprintf(fh, "%s %d %d %d %.6f\n", key, btw, bcx, bp, best);
       fclose(fh);
     }
 }
@@ -2143,6 +2151,8 @@ int fdf_batch_tuning(fdf_batch_t* h, int* tile_width, int* chunk, double* stage_
   if (from_cache) *from_cache = h->b->tuned_from_cache;
   return 0;
 }
+
+int fdf_batch_persistent(fdf_batch_t* h) { return h ? h->b->persist : -2; }
 
 int fdf_batch_finish(fdf_batch_t* h) {
   return guarded([&] { h->b->finish(); });

@@ -45,7 +45,7 @@ EXPORTS = ["fdf_open", "fdf_step", "fdf_stage", "fdf_run", "fdf_read_field", "fd
            "fdf_write_array", "fdf_read_iface", "fdf_read_front", "fdf_finish", "fdf_get_info",
            "fdf_host_setup", "fdf_last_error", "fdf_close", "fdf_bulk_kernel_ms", "fdf_fault_kernel_ms",
            "fdf_open_batch", "fdf_batch_step", "fdf_batch_size", "fdf_batch_get_info", "fdf_batch_read_iface",
-           "fdf_batch_read_front", "fdf_batch_read_array", "fdf_batch_tuning", "fdf_batch_finish",
+           "fdf_batch_read_front", "fdf_batch_read_array", "fdf_batch_tuning", "fdf_batch_persistent", "fdf_batch_finish",
            "fdf_batch_close", "fdf_mlp_param_count", "fdf_mlp_train", "fdf_mlp_grad", "fdf_mlp_predict",
            "fdf_stz_set_model", "fdf_stz_log", "fdf_stz_log_read", "fdf_stz_stats"]
 
@@ -92,6 +92,7 @@ def load():
     L.fdf_batch_read_front.argtypes = [C.c_void_p, C.c_int, C.c_int, dp, C.c_long]
     L.fdf_batch_read_array.argtypes = [C.c_void_p, C.c_int, C.c_int, dp, C.c_long]
     L.fdf_batch_tuning.argtypes = [C.c_void_p, ip, ip, dp, ip]
+    L.fdf_batch_persistent.argtypes = [C.c_void_p]
     L.fdf_batch_finish.argtypes = [C.c_void_p]
     L.fdf_batch_close.argtypes = [C.c_void_p]
     fp = C.POINTER(C.c_float)
@@ -358,7 +359,8 @@ class Batch:
         tw, cx, fc = C.c_int(), C.c_int(), C.c_int()
         ms = C.c_double()
         load().fdf_batch_tuning(self.h, C.byref(tw), C.byref(cx), C.byref(ms), C.byref(fc))
-        return {"tile_width": tw.value, "chunk": cx.value, "stage_ms": ms.value, "from_cache": bool(fc.value)}
+        return {"tile_width": tw.value, "chunk": cx.value, "stage_ms": ms.value, "from_cache": bool(fc.value),
+                "persistent": bool(load().fdf_batch_persistent(self.h))}
 
     def iface(self, inst: int, i: int, which: str):
         L = load()

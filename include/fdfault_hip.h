@@ -120,9 +120,10 @@ double fdf_fault_kernel_ms(fdf_t* h, int* nstages);
  * instance is set up as fdf_open would (same outputs and front files), then
  * every RK stage of every instance runs as one fault launch and one
  * persistent bulk launch.  Instances must share sbporder and rkorder; grid
- * sizes and time steps may differ.  The bulk kernel's tile width and x-chunk are
- * chosen at open by timing candidates (cached in $FDF_TUNE_CACHE or
- * ~/.cache/fdfault_amd_tune.txt; FDF_BATCH_TW/FDF_BATCH_CX fix them). */
+ * sizes and time steps may differ.  The bulk kernel's tile width, x-chunk and
+ * launch mode are chosen at open by timing candidates (cached in
+ * $FDF_TUNE_CACHE or ~/.cache/fdfault_amd_tune.txt; FDF_BATCH_TW/FDF_BATCH_CX
+ * fix the tile). */
 typedef struct fdf_batch_s fdf_batch_t;
 
 int fdf_open_batch(const char* const* in_paths, int n, const fdf_opts* opts, fdf_batch_t** out);
@@ -139,6 +140,10 @@ int fdf_batch_read_array(fdf_batch_t* h, int inst, int which, double* host, long
 /* The autotuner's choice: tile width (y points per workgroup), x planes per
  * chunk, the timed bulk-stage time (ms) and whether it came from the cache. */
 int fdf_batch_tuning(fdf_batch_t* h, int* tile_width, int* chunk, double* stage_ms, int* from_cache);
+/* The autotuner's launch mode of the bulk kernel: 1 a persistent grid of the
+ * resident workgroups walking the work list, 0 one workgroup per work item
+ * (FDF_BATCH_PERSIST fixes it). */
+int fdf_batch_persistent(fdf_batch_t* h);
 /* fdf_finish for every instance. */
 int fdf_batch_finish(fdf_batch_t* h);
 void fdf_batch_close(fdf_batch_t* h);

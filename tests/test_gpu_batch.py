@@ -128,6 +128,25 @@ def test_batch_matches_single_runs(tmp_path, monkeypatch, precision, graph):
     b.close()
 
 
+def test_batch_launch_modes_agree(tmp_path, monkeypatch):
+    """The persistent grid and the one-workgroup-per-tile launch of the batched
+    bulk kernel compute the same tiles: bit-identical results."""
+    from fdfault_amd import lib
+    paths = [_case(tmp_path, "m%d" % i, nx, ny, nt=20) for i, (nx, ny) in enumerate(SIZES)]
+    monkeypatch.setenv("FDF_BATCH_TW", "64")
+    monkeypatch.setenv("FDF_BATCH_CX", "2")
+    res = []
+    for mode in ("1", "0"):
+        monkeypatch.setenv("FDF_BATCH_PERSIST", mode)
+        b = lib.Batch(paths, precision=lib.FP32, graph=True)
+        assert b.tuning()["persistent"] == (mode == "1")
+        b.step(20)
+        res.append([(b.array(i, "f"), b.iface(i, 0, "U")) for i in range(len(paths))])
+        b.close()
+    for (fa, ua), (fb, ub) in zip(res[0], res[1]):
+        assert np.array_equal(fa, fb) and np.array_equal(ua, ub)
+
+
 def test_batch_fp64_instance_matches_oracle(tmp_path, monkeypatch):
     from fdfault_amd import lib
     monkeypatch.setenv("FDF_TUNE_CACHE", str(tmp_path / "tune.txt"))
@@ -155,7 +174,8 @@ def test_batch_autotune_cache_reused(tmp_path, monkeypatch):
     b = lib.Batch(paths, precision=lib.FP32)
     tb = b.tuning()
     b.close()
-    assert tb["from_cache"] and (tb["tile_width"], tb["chunk"]) == (ta["tile_width"], ta["chunk"])
+    assert tb["from_cache"] and (tb["tile_width"], tb["chunk"], tb["persistent"]) == (
+        ta["tile_width"], ta["chunk"], ta["persistent"])
     assert os.path.getsize(str(tmp_path / "tune.txt")) > 0
 
 
