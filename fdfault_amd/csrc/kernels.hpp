@@ -19,6 +19,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "mlp.hpp"
 
 namespace fdf {
@@ -646,12 +648,22 @@ __device__ double stz_newton_from(double xmin, double xmax, double x0, const dou
 // friction laws of friction.cpp:259-435, slipweak.cpp:206-308 and
 // stz.cpp:263-356, then friction::calc_df/update (friction.cpp:465-505).
 // LAW >= 0 fixes the friction law at compile time (only that law's code is
-// generated); LAW = -1 reads it from F.law.
-template <int M, typename R, int LAW = -1>
+// generated); LAW = -1 reads it from F.law.  With a Hook (the STZ proposal of
+// a whole wave), every lane of the wave calls this function, lanes past the
+// end with live = false and the last point: once the locked shear traction,
+// normal stress and impedance are known (before any write), all lanes call
+// hook(phi, sn, eta, chi) for the proposed log V, and the dead lanes return.
+// Without a Hook, the network in mlp (if any) is evaluated per point.
+struct NoHook {
+  __device__ double operator()(double, double, double, double) const { return NAN; }
+};
+
+template <int M, typename R, int LAW = -1, typename Hook = NoHook>
 __device__ __forceinline__ void fault_point(const FaultDev& F, const GridDev& g, const FaultStageDev& fs,
                                             const R* __restrict__ f, const R* __restrict__ mat,
                                             const double* __restrict__ hs, int* __restrict__ err,
-                                            int* __restrict__ newton_iters, int p, const float* mlp) {
+                                            int* __restrict__ newton_iters, int p, const float* mlp,
+                                            const Hook& hook = Hook(), bool live = true) {
   const int law = LAW >= 0 ? LAW : F.law;
   const int ii = p / F.n1, jj = p % F.n1;
   const long long N = g.N;
@@ -764,6 +776,11 @@ __device__ __forceinline__ void fault_point(const FaultDev& F, const GridDev& g,
     const double phi2 = eta * (s12 / z1 - v12 + s22 / z2 + v22);
     const double phi3 = eta * (s13 / z1 - v13 + s23 / z2 + v23);
     const double phi = sqrt(phi2 * phi2 + phi3 * phi3);
+    double lv_in = NAN;
+    if constexpr (!std::is_same<Hook, NoHook>::value) {
+      lv_in = hook(phi, snc, eta, F.state[p]);
+      if (!live) return;
+    }
     double V, S, dstatedt = 0.;
     if (law == 2) {  // slipweak::solve_fs / calc_mu
       double c0t = 0.;
@@ -817,7 +834,7 @@ __device__ __forceinline__ void fault_point(const FaultDev& F, const GridDev& g,
               // from the force balance mu = (phi - eta V)/|sn| or from the
               // inverted flow law mu = a (log(V/v0) + f0 + 1/chi - log(1 - muy/mu))
               // (stz.cpp:436-484), whichever leaves the smaller residual
-              const double lv = mlp_logv(mlp, phi, snc, pr[3], eta);
+              const double lv = isnan(lv_in) ? mlp_logv(mlp, phi, snc, pr[3], eta) : lv_in;
               const double xmin = pr[7], xmax = -phi / snc;
               const double ma = (phi - eta * exp(lv)) / (-snc);
               const double lb = lv - log(pr[4]) + pr[5] + 1. / pr[3];
@@ -994,14 +1011,36 @@ __global__ void __launch_bounds__(256) fault_kernel(FaultDev F, GridDev g, Fault
                                                     int* __restrict__ err, int* __restrict__ newton_iters) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if constexpr (LAW == 3) {
-    // the proposal network (if any) is read from LDS: one copy per workgroup
-    __shared__ float smlp[MLP_NP];
     if (F.mlp) {
-      for (int i = threadIdx.x; i < MLP_NP; i += blockDim.x) smlp[i] = F.mlp[i];
-      __syncthreads();
+      // the proposal network runs for whole waves on the matrix cores, from
+      // LDS (one copy per workgroup): its loads are issued first and land in
+      // LDS once the points' inputs are known, so they overlap the fault
+      // point's own loads (256 threads, 4 full waves; MLP_NP <= 18 * 256)
+      __shared__ float smlp[MLP_NP];
+      constexpr int PER = (MLP_NP + 255) / 256;
+      float wreg[PER];
+#pragma unroll
+      for (int u = 0; u < PER; u++) {
+        const int i = threadIdx.x + 256 * u;
+        wreg[u] = i < MLP_NP ? F.mlp[i] : 0.f;
+      }
+      auto hook = [&](double phi, double sn, double eta, double chi) {
+#pragma unroll
+        for (int u = 0; u < PER; u++) {
+          const int i = threadIdx.x + 256 * u;
+          if (i < MLP_NP) smlp[i] = wreg[u];
+        }
+        __syncthreads();
+        float x[MLP_IN];
+        mlp_inputs(smlp, phi, sn, chi, eta, x);
+        return mlp_out(smlp, mlp_eval_wave(smlp, x));
+      };
+      const bool live = p < F.n;
+      fault_point<M, R, 3>(F, g, fs, f, mat, hs, err, newton_iters, live ? p : F.n - 1, smlp, hook, live);
+      return;
     }
     if (p >= F.n) return;
-    fault_point<M, R, 3>(F, g, fs, f, mat, hs, err, newton_iters, p, F.mlp ? smlp : nullptr);
+    fault_point<M, R, 3>(F, g, fs, f, mat, hs, err, newton_iters, p, nullptr);
   } else {
     if (p >= F.n) return;
     fault_point<M, R, LAW>(F, g, fs, f, mat, hs, err, newton_iters, p, nullptr);

@@ -52,6 +52,87 @@ __device__ __forceinline__ float mlp_eval(const float* __restrict__ P, const flo
   return y;
 }
 
+// The network for the 64 points of a wave (one per lane) with the 64 x 64
+// hidden layer on the matrix cores: H2 = W2 H1 as a (64 neurons) x (64
+// points) product of four 32 x 32 tiles, eight v_mfma_f32_32x32x8f16 steps
+// each (FP16 operands, FP32 accumulation).  Operand layouts of that
+// instruction (wave64): A[m][k] sits in lane m + 32 (k / 4), element k % 4;
+// B[k][n] in lane n + 32 (k / 4), element k % 4; D[m][n] in lane
+// n + 32 ((m / 4) % 2), register (m % 4) + 4 (m / 8).  So lane l (column
+// c = l % 32, half h = l / 32) computes the first-layer activations of
+// neurons 8 kb + 4 h + i for the points c and c + 32, holds rows 32 mt + c of
+// W2, and finishes the output layer over its 32 rows of the two point
+// columns; the two lanes sharing a column add their halves.  Every lane of
+// the wave must be active (the inputs and halves move between lanes).
+typedef _Float16 mlp_h4 __attribute__((ext_vector_type(4)));
+typedef float mlp_f16x __attribute__((ext_vector_type(16)));
+
+__device__ __forceinline__ float mlp_eval_wave(const float* __restrict__ P, const float* x) {
+  const int lane = threadIdx.x & 63, col = lane & 31, kh = lane >> 5;
+  float xin[2][MLP_IN];
+#pragma unroll
+  for (int t = 0; t < 2; t++)
+#pragma unroll
+    for (int i = 0; i < MLP_IN; i++) xin[t][i] = __shfl(x[i], col + 32 * t, 64);
+  mlp_h4 B[2][8];
+#pragma unroll
+  for (int kb = 0; kb < 8; kb++)
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int k = 8 * kb + 4 * kh + i;
+#pragma unroll
+      for (int t = 0; t < 2; t++) {
+        float s = P[MLP_B1 + k];
+#pragma unroll
+        for (int c = 0; c < MLP_IN; c++) s += P[MLP_W1 + k * MLP_IN + c] * xin[t][c];
+        B[t][kb][i] = (_Float16)mlp_tanh(s);
+      }
+    }
+  mlp_f16x acc[2][2];
+#pragma unroll
+  for (int mt = 0; mt < 2; mt++)
+#pragma unroll
+    for (int t = 0; t < 2; t++)
+#pragma unroll
+      for (int j = 0; j < 16; j++) acc[mt][t][j] = 0.f;
+#pragma unroll
+  for (int mt = 0; mt < 2; mt++)
+#pragma unroll
+    for (int kb = 0; kb < 8; kb++) {
+      mlp_h4 a;
+#pragma unroll
+      for (int i = 0; i < 4; i++) a[i] = (_Float16)P[MLP_W2 + (32 * mt + col) * MLP_H + 8 * kb + 4 * kh + i];
+#pragma unroll
+      for (int t = 0; t < 2; t++) acc[mt][t] = __builtin_amdgcn_mfma_f32_32x32x8f16(a, B[t][kb], acc[mt][t], 0, 0, 0);
+    }
+  float s[2] = {0.f, 0.f};
+#pragma unroll
+  for (int t = 0; t < 2; t++)
+#pragma unroll
+    for (int mt = 0; mt < 2; mt++)
+#pragma unroll
+      for (int j = 0; j < 16; j++) {
+        const int m = 32 * mt + 8 * (j / 4) + 4 * kh + (j % 4);
+        s[t] += P[MLP_W3 + m] * mlp_tanh(acc[mt][t][j] + P[MLP_B2 + m]);
+      }
+  const float other = __shfl_xor(kh ? s[0] : s[1], 32, 64);
+  return (kh ? s[1] : s[0]) + other + P[MLP_B3];
+}
+
+// normalised inputs of a raw (phi, sn, chi, eta)
+__device__ __forceinline__ void mlp_inputs(const float* __restrict__ P, double phi, double sn, double chi, double eta,
+                                           float* x) {
+  const double raw[MLP_IN] = {phi, sn, chi, eta};
+#pragma unroll
+  for (int i = 0; i < MLP_IN; i++)
+    x[i] = (float)((raw[i] - (double)P[MLP_NORM + i]) / (double)P[MLP_NORM + MLP_IN + i]);
+}
+
+// log(V) from a normalised network output
+__device__ __forceinline__ double mlp_out(const float* __restrict__ P, float y) {
+  return (double)y * (double)P[MLP_NORM + 2 * MLP_IN + 1] + (double)P[MLP_NORM + 2 * MLP_IN];
+}
+
 // log(V) proposed for raw inputs (phi, sn, chi, eta)
 __device__ __forceinline__ double mlp_logv(const float* __restrict__ P, double phi, double sn, double chi, double eta) {
   const double raw[MLP_IN] = {phi, sn, chi, eta};
@@ -205,6 +286,23 @@ __global__ void mlp_predict_kernel(const float* __restrict__ P, const float* __r
 #pragma unroll
   for (int i = 0; i < MLP_IN; i++) x[i] = X[(long long)b * MLP_IN + i];
   Y[b] = mlp_eval(P, x);
+}
+
+// the matrix-core evaluation for n normalised inputs (a whole wave per 64
+// points; lanes past n evaluate a zero input and store nothing)
+__global__ void __launch_bounds__(256) mlp_predict_wave_kernel(const float* __restrict__ Pg,
+                                                               const float* __restrict__ X, int n,
+                                                               float* __restrict__ Y) {
+  __shared__ float P[MLP_NP];
+  for (int i = threadIdx.x; i < MLP_NP; i += blockDim.x) P[i] = Pg[i];
+  __syncthreads();
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  float x[MLP_IN] = {0.f, 0.f, 0.f, 0.f};
+  if (b < n)
+#pragma unroll
+    for (int i = 0; i < MLP_IN; i++) x[i] = X[(long long)b * MLP_IN + i];
+  const float y = mlp_eval_wave(P, x);
+  if (b < n) Y[b] = y;
 }
 
 }  // namespace fdf

@@ -2192,6 +2192,20 @@ int fdf_mlp_predict(const float* params, const float* X, long n, float* Y) {
   });
 }
 
+int fdf_mlp_predict_mfma(const float* params, const float* X, long n, float* Y) {
+  return guarded([&] {
+    if (!params || !X || !Y || n < 1) throw Error{"fdf_mlp_predict_mfma: bad arguments"};
+    fdf::DBuf<float> P, dx, dy;
+    P.upload(params, fdf::MLP_NP);
+    dx.upload(X, (size_t)n * fdf::MLP_IN);
+    dy.alloc((size_t)n);
+    hipLaunchKernelGGL(fdf::mlp_predict_wave_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, nullptr, P.p,
+                       dx.p, (int)n, dy.p);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpy(Y, dy.p, (size_t)n * sizeof(float), hipMemcpyDeviceToHost));
+  });
+}
+
 int fdf_mlp_train(const double* rows, long n, int epochs, int batch, float lr, unsigned seed, float* params,
                   float* epoch_loss) {
   return guarded([&] {

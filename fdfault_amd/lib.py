@@ -46,7 +46,7 @@ EXPORTS = ["fdf_open", "fdf_step", "fdf_stage", "fdf_run", "fdf_read_field", "fd
            "fdf_host_setup", "fdf_last_error", "fdf_close", "fdf_bulk_kernel_ms", "fdf_fault_kernel_ms",
            "fdf_open_batch", "fdf_batch_step", "fdf_batch_size", "fdf_batch_get_info", "fdf_batch_read_iface",
            "fdf_batch_read_front", "fdf_batch_read_array", "fdf_batch_tuning", "fdf_batch_persistent", "fdf_batch_finish",
-           "fdf_batch_close", "fdf_mlp_param_count", "fdf_mlp_train", "fdf_mlp_grad", "fdf_mlp_predict",
+           "fdf_batch_close", "fdf_mlp_param_count", "fdf_mlp_train", "fdf_mlp_grad", "fdf_mlp_predict", "fdf_mlp_predict_mfma",
            "fdf_stz_set_model", "fdf_stz_log", "fdf_stz_log_read", "fdf_stz_stats"]
 
 _lib = None
@@ -99,6 +99,7 @@ def load():
     L.fdf_mlp_train.argtypes = [dp, C.c_long, C.c_int, C.c_int, C.c_float, C.c_uint, fp, fp]
     L.fdf_mlp_grad.argtypes = [fp, fp, fp, C.c_long, fp, fp]
     L.fdf_mlp_predict.argtypes = [fp, fp, C.c_long, fp]
+    L.fdf_mlp_predict_mfma.argtypes = [fp, fp, C.c_long, fp]
     L.fdf_stz_set_model.argtypes = [C.c_void_p, fp, C.c_int]
     L.fdf_stz_log.argtypes = [C.c_void_p, C.c_long]
     L.fdf_stz_log_read.restype = C.c_long
@@ -304,11 +305,14 @@ def mlp_grad(P, X, T):
     return G, float(loss[0])
 
 
-def mlp_predict(P, X):
+def mlp_predict(P, X, mfma=False):
+    """Normalised outputs for normalised inputs; mfma=True uses the fault
+    kernel's evaluation (matrix cores, FP16 operands)."""
     P = np.ascontiguousarray(P, np.float32)
     X = np.ascontiguousarray(X, np.float32)
     Y = np.zeros(X.shape[0], np.float32)
-    if load().fdf_mlp_predict(_fp(P), _fp(X), X.shape[0], _fp(Y)) != 0:
+    fn = load().fdf_mlp_predict_mfma if mfma else load().fdf_mlp_predict
+    if fn(_fp(P), _fp(X), X.shape[0], _fp(Y)) != 0:
         raise _err()
     return Y
 

@@ -171,6 +171,9 @@ int fdf_mlp_train(const double* rows, long n, int epochs, int batch, float lr, u
 int fdf_mlp_grad(const float* params, const float* X, const float* T, long n, float* grad, float* loss);
 /* Normalised network outputs for n normalised inputs (GPU). */
 int fdf_mlp_predict(const float* params, const float* X, long n, float* Y);
+/* The same with the evaluation the fault kernel uses: the hidden 64 x 64
+ * layer on the matrix cores with FP16 operands and FP32 accumulation. */
+int fdf_mlp_predict_mfma(const float* params, const float* X, long n, float* Y);
 /* Use (params != NULL, n = fdf_mlp_param_count()) or stop using the network
  * in this problem's STZ solves. */
 int fdf_stz_set_model(fdf_t* h, const float* params, int n);

@@ -55,6 +55,24 @@ def test_gpu_predict_matches_oracle():
     assert np.abs(Y - Yo).max() <= 2e-5 * max(1.0, np.abs(Yo).max())
 
 
+def test_gpu_predict_mfma_matches_oracle():
+    """The fault kernel's evaluation (hidden layer on the matrix cores, FP16
+    operands, FP32 accumulation) against the FP64 oracle: within the FP16
+    rounding of weights and activations.  The sizes leave a partial last wave
+    (lanes past n evaluate a dummy input)."""
+    from fdfault_amd import lib
+    rng = np.random.default_rng(6)
+    P = _rand_params(8, 0.3)
+    for n in (4000, 77, 1):
+        X = rng.normal(size=(n, 4)).astype(np.float32)
+        Y = lib.mlp_predict(P, X, mfma=True)
+        Yo = O.mlp_forward(P.astype(np.float64), X.astype(np.float64))
+        err = np.abs(Y - Yo).max()
+        print("n", n, "max |mfma - oracle|", err, "max |y|", np.abs(Yo).max(), flush=True)
+        assert err <= 5e-3 * max(1.0, np.abs(Yo).max())
+        assert err > 0.0 or n == 1  # FP16 operands: not the FP32 path's numbers
+
+
 def test_gpu_training_fits_smooth_target():
     from fdfault_amd import lib
     rng = np.random.default_rng(5)
