@@ -210,22 +210,26 @@ __global__ void __launch_bounds__(256) mlp_fwd_bwd_kernel(const float* __restric
   if (threadIdx.x == 0 && loss) atomicAdd(loss, red[0]);
 }
 
-// weight gradients: one workgroup per output group, reduction over the batch
-//   G[W2 j k] = sum_b D2[b][j] H1[b][k]    G[b2 j] = sum_b D2[b][j]
-//   G[W1 j i] = sum_b D1[b][j] X[b][i]     G[b1 j] = sum_b D1[b][j]
-//   G[W3 j]   = sum_b DY[b] H2[b][j]       G[b3]   = sum_b DY[b]
-// workgroup g < 64: row j = g of W2 (64 outputs + b2); g == 64: W1, b1, W3, b3
+// weight gradients as reductions over the batch, split into MLP_GS slices
+// (blockIdx.y) written to GP[slice][4545]; mlp_gsum_kernel adds the slices in
+// a fixed order.  Blocks 0..63: row g of W2 and b2 (4 waves split the slice,
+// lane k owns column k); blocks 64..127: row j of W1, b1 j and W3 j (256
+// threads split the slice, tree reduction in LDS); block 128: b3.
+constexpr int MLP_GS = 8;
+
 __global__ void __launch_bounds__(256) mlp_grad_kernel(const float* __restrict__ X, const int* __restrict__ idx, int B,
                                                        const float* __restrict__ H1, const float* __restrict__ H2,
                                                        const float* __restrict__ D1, const float* __restrict__ D2,
-                                                       const float* __restrict__ DY, float* __restrict__ G) {
-  const int g = blockIdx.x, t = threadIdx.x;
+                                                       const float* __restrict__ DY, float* __restrict__ GP) {
+  const int g = blockIdx.x, sl = blockIdx.y, t = threadIdx.x;
+  const int b0 = (int)((long long)B * sl / MLP_GS), b1 = (int)((long long)B * (sl + 1) / MLP_GS);
+  float* __restrict__ G = GP + (long long)sl * MLP_NW;
   __shared__ float part[4][MLP_H + 1];
+  __shared__ float red[6][256];
   if (g < MLP_H) {
-    // 4 waves split the batch; lane k of each wave owns output (j, k)
     const int k = t & 63, wv = t >> 6;
     float s = 0.f, sb = 0.f;
-    for (int b = wv; b < B; b += 4) {
+    for (int b = b0 + wv; b < b1; b += 4) {
       const float d = D2[(long long)b * MLP_H + g];
       s += d * H1[(long long)b * MLP_H + k];
       sb += d;
@@ -235,31 +239,51 @@ __global__ void __launch_bounds__(256) mlp_grad_kernel(const float* __restrict__
     __syncthreads();
     if (t < MLP_H) G[MLP_W2 + g * MLP_H + t] = part[0][t] + part[1][t] + part[2][t] + part[3][t];
     if (t == 0) G[MLP_B2 + g] = part[0][MLP_H] + part[1][MLP_H] + part[2][MLP_H] + part[3][MLP_H];
-  } else {
-    // thread t < 64: W1 row t and b1 t; 64 <= t < 128: W3 (t-64); t == 128: b3
-    if (t < MLP_H) {
-      float w[MLP_IN] = {0.f, 0.f, 0.f, 0.f}, sb = 0.f;
-      for (int b = 0; b < B; b++) {
-        const float d = D1[(long long)b * MLP_H + t];
-        const int s = idx ? idx[b] : b;
+    return;
+  }
+  const int j = g - MLP_H;  // (block 128: j = 64, the b3 block)
+  float v[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  for (int b = b0 + t; b < b1; b += 256) {
+    if (j < MLP_H) {
+      const float d = D1[(long long)b * MLP_H + j];
+      const int s = idx ? idx[b] : b;
 #pragma unroll
-        for (int i = 0; i < MLP_IN; i++) w[i] += d * X[(long long)s * MLP_IN + i];
-        sb += d;
-      }
-#pragma unroll
-      for (int i = 0; i < MLP_IN; i++) G[MLP_W1 + t * MLP_IN + i] = w[i];
-      G[MLP_B1 + t] = sb;
-    } else if (t < 2 * MLP_H) {
-      const int j = t - MLP_H;
-      float s = 0.f;
-      for (int b = 0; b < B; b++) s += DY[b] * H2[(long long)b * MLP_H + j];
-      G[MLP_W3 + j] = s;
-    } else if (t == 2 * MLP_H) {
-      float s = 0.f;
-      for (int b = 0; b < B; b++) s += DY[b];
-      G[MLP_B3] = s;
+      for (int i = 0; i < MLP_IN; i++) v[i] += d * X[(long long)s * MLP_IN + i];
+      v[4] += d;
+      v[5] += DY[b] * H2[(long long)b * MLP_H + j];
+    } else {
+      v[0] += DY[b];
     }
   }
+#pragma unroll
+  for (int q = 0; q < 6; q++) red[q][t] = v[q];
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (t < o)
+#pragma unroll
+      for (int q = 0; q < 6; q++) red[q][t] += red[q][t + o];
+    __syncthreads();
+  }
+  if (t == 0) {
+    if (j < MLP_H) {
+#pragma unroll
+      for (int i = 0; i < MLP_IN; i++) G[MLP_W1 + j * MLP_IN + i] = red[i][0];
+      G[MLP_B1 + j] = red[4][0];
+      G[MLP_W3 + j] = red[5][0];
+    } else {
+      G[MLP_B3] = red[0][0];
+    }
+  }
+}
+
+// G = sum of the MLP_GS slices of GP, in slice order
+__global__ void mlp_gsum_kernel(const float* __restrict__ GP, float* __restrict__ G) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= MLP_NW) return;
+  float s = 0.f;
+#pragma unroll
+  for (int sl = 0; sl < MLP_GS; sl++) s += GP[(long long)sl * MLP_NW + i];
+  G[i] = s;
 }
 
 // Adam (Kingma & Ba 2015) on the trained parameters; step counts from 1

@@ -1844,12 +1844,13 @@ void Solver<R>::stz_stats(unsigned long long out[4]) {
 
 // ---- MLP training on the GPU ---------------------------------------------
 struct MlpBufs {
-  DBuf<float> P, G, M, V, X, T, H1, H2, D1, D2, DY, loss;
+  DBuf<float> P, G, GP, M, V, X, T, H1, H2, D1, D2, DY, loss;
   DBuf<int> idx;
 };
 
 static void mlp_alloc(MlpBufs& b, long n, int B) {
   b.G.alloc(MLP_NW);
+  b.GP.alloc((size_t)MLP_GS * MLP_NW);
   b.H1.alloc((size_t)B * MLP_H);
   b.H2.alloc((size_t)B * MLP_H);
   b.D1.alloc((size_t)B * MLP_H);
@@ -1864,8 +1865,9 @@ static void mlp_alloc(MlpBufs& b, long n, int B) {
 static void mlp_batch_grad(MlpBufs& b, const int* idx, int B, hipStream_t st) {
   hipLaunchKernelGGL(mlp_fwd_bwd_kernel, dim3((B + 255) / 256), dim3(256), 0, st, b.P.p, b.X.p, b.T.p, idx, B, b.H1.p,
                      b.H2.p, b.D1.p, b.D2.p, b.DY.p, b.loss.p);
-  hipLaunchKernelGGL(mlp_grad_kernel, dim3(MLP_H + 1), dim3(256), 0, st, b.X.p, idx, B, b.H1.p, b.H2.p, b.D1.p,
-                     b.D2.p, b.DY.p, b.G.p);
+  hipLaunchKernelGGL(mlp_grad_kernel, dim3(2 * MLP_H + 1, MLP_GS), dim3(256), 0, st, b.X.p, idx, B, b.H1.p, b.H2.p,
+                     b.D1.p, b.D2.p, b.DY.p, b.GP.p);
+  hipLaunchKernelGGL(mlp_gsum_kernel, dim3((MLP_NW + 255) / 256), dim3(256), 0, st, b.GP.p, b.G.p);
   HIPCHK(hipGetLastError());
 }
 
