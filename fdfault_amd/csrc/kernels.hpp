@@ -1752,10 +1752,14 @@ __global__ void __launch_bounds__(NTMAX) __attribute__((amdgpu_waves_per_eu(WPE)
         for (int q = 0; q < 8; q++) tile[q * LPL + hrow[u]] = nhal[u][q];
       }
     lds_barrier();
+    // every wave issues the next plane's loads and this plane's df loads
+    // before any wave starts the arithmetic (measured: 2.5 % faster FP32)
+    __builtin_amdgcn_s_setprio(3);
     if (i + 1 < ie) issue(i + 1);
     R odf[9];
 #pragma unroll
     for (int q = 0; q < 9; q++) odf[q] = bld<R>(rd, colb, q * fb + plb);
+    __builtin_amdgcn_s_setprio(0);
     if (active) {
       C dxv[6], dyv[6], dzv[6];
       if (i >= ta.mc[0] && i < ta.mrb[0]) {
