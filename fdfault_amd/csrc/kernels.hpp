@@ -1209,7 +1209,8 @@ struct Tiled3D {
   int roundtrip, hetstress;
   double s0[6];
   int rows_nht;                  // > 0: row-tile kernel with this many halo entries per thread
-  int rows_var;                  // row-tile variant: bit 0 double-buffered tile, bit 1 df loaded ahead
+  int rows_var;                  // row-tile variant: 1 double-buffered LDS tile, 0 single
+  int rows_shape;                // row-tile workgroup: 0 TY = 4 rows, LDS stride 256; 1 TY = 6, stride 160 (FP32)
   int face_active[6];            // external SAT faces
   // external SAT of a Cartesian face normal to axis d (other axes a < b) as
   // its 16 non-zero linear-map entries, [6 faces][16], in the bulk precision,
@@ -1622,7 +1623,7 @@ __global__ void __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(WP
 // plane i-1's buffer may still be read, so one barrier per plane suffices
 // (a wave writing buffer b at plane i+1 has passed plane i's barrier, which
 // every wave reaches only after its plane i-1 reads of b).
-template <typename R, typename C, int S, int TY, int NTMAX, int NHT, int WPE, int NBUF = 1>
+template <typename R, typename C, int S, int TY, int NTMAX, int NHT, int WPE, int NBUF = 1, int ZS = 256>
 __global__ void __launch_bounds__(NTMAX) __attribute__((amdgpu_waves_per_eu(WPE)))
     bulk_rows3d(Tiled3D ta, const R* __restrict__ f, R* __restrict__ fn, R* __restrict__ df,
                 const double* __restrict__ hs) {
@@ -1631,9 +1632,8 @@ __global__ void __launch_bounds__(NTMAX) __attribute__((amdgpu_waves_per_eu(WPE)
   constexpr int W = 2 * r + 1;
   constexpr int LY = TY + 2 * r;
   constexpr int NRB = 2 * (S - 1), NTB = 3 * (S - 1);
-  // LDS rows padded to ZS entries (a compile-time stride, so every tap is a
-  // lane base plus an immediate offset)
-  constexpr int ZS = 256;
+  // LDS rows padded to ZS >= nz entries (a compile-time stride, so every
+  // tap is a lane base plus an immediate offset)
   extern __shared__ char smem_raw[];
   R* const tbase = reinterpret_cast<R*>(smem_raw);  // [NBUF][8][LY][ZS]
   // z derivative coefficients: row 0 interior (taps -r..r, zero padded),

@@ -192,46 +192,65 @@ static const void* fault_kernel_for(int mode, int law) {
 // fit the LDS), 0 = single buffer
 static const int rows_var_default = 1;
 
-template <typename R>
+// row-tile workgroup shapes: 0 = FP32 4 rows (<= 1024 threads, 4 waves per
+// SIMD) or FP64 2 rows (<= 512 threads, 2 waves per SIMD), LDS rows padded to
+// 256 points; 1 = FP32 6 rows of at most 160 points (<= 1024 threads), LDS
+// rows padded to 160, so that short rows still give 15-16 waves per CU
+template <typename R, int SH>
 struct RowsCfg {
-  static constexpr int TY = sizeof(R) == 4 ? 4 : 2;
-  static constexpr int NTMAX = sizeof(R) == 4 ? 1024 : 512;
-  static constexpr int WPE = sizeof(R) == 4 ? 4 : 2;
-  static constexpr int NHT0 = sizeof(R) == 4 ? 1 : 2;  // halo entries per thread: NHT0 or NHT0 + 1
+  static constexpr bool F32 = sizeof(R) == 4;
+  static constexpr int TY = F32 ? (SH == 1 ? 6 : 4) : 2;
+  static constexpr int ZS = SH == 1 ? 160 : 256;
+  static constexpr int NTMAX = F32 ? 1024 : 512;
+  static constexpr int WPE = F32 ? 4 : 2;
+  static constexpr int NHT0 = F32 ? 1 : 2;  // halo entries per thread: NHT0 or NHT0 + 1
 };
 
-template <typename R, int S, int NHT>
-static const void* rows_kernel(int var) {
-  using Cf = RowsCfg<R>;
+template <typename R>
+static int rows_ty(int shape) { return shape == 1 ? RowsCfg<R, 1>::TY : RowsCfg<R, 0>::TY; }
+template <typename R>
+static int rows_zs(int shape) { return shape == 1 ? RowsCfg<R, 1>::ZS : RowsCfg<R, 0>::ZS; }
+
+template <typename R, int S, int NHT, int SH>
+static const void* rows_kernel_sh(int var) {
+  using Cf = RowsCfg<R, SH>;
   constexpr int NB2 = sizeof(R) == 4 ? 2 : 1;  // an FP64 double-buffered tile exceeds the LDS
-  if (var & 1) return (const void*)bulk_rows3d<R, R, S, Cf::TY, Cf::NTMAX, NHT, Cf::WPE, NB2>;
-  return (const void*)bulk_rows3d<R, R, S, Cf::TY, Cf::NTMAX, NHT, Cf::WPE, 1>;
+  if (var & 1) return (const void*)bulk_rows3d<R, R, S, Cf::TY, Cf::NTMAX, NHT, Cf::WPE, NB2, Cf::ZS>;
+  return (const void*)bulk_rows3d<R, R, S, Cf::TY, Cf::NTMAX, NHT, Cf::WPE, 1, Cf::ZS>;
+}
+
+template <typename R, int S, int NHT>
+static const void* rows_kernel(int var, int shape) {
+  if constexpr (sizeof(R) == 4) {
+    if (shape == 1) return rows_kernel_sh<R, S, NHT, 1>(var);
+  }
+  return rows_kernel_sh<R, S, NHT, 0>(var);
 }
 
 template <typename R>
-static const void* rows_kernel_for(int s, int nht, int var) {
-  constexpr int N0 = RowsCfg<R>::NHT0;
-  if (s == 2) return nht == N0 ? rows_kernel<R, 2, N0>(var) : rows_kernel<R, 2, N0 + 1>(var);
-  if (s == 3) return nht == N0 ? rows_kernel<R, 3, N0>(var) : rows_kernel<R, 3, N0 + 1>(var);
-  return nht == N0 ? rows_kernel<R, 4, N0>(var) : rows_kernel<R, 4, N0 + 1>(var);
+static const void* rows_kernel_for(int s, int nht, int var, int shape) {
+  constexpr int N0 = RowsCfg<R, 0>::NHT0;
+  if (s == 2) return nht == N0 ? rows_kernel<R, 2, N0>(var, shape) : rows_kernel<R, 2, N0 + 1>(var, shape);
+  if (s == 3) return nht == N0 ? rows_kernel<R, 3, N0>(var, shape) : rows_kernel<R, 3, N0 + 1>(var, shape);
+  return nht == N0 ? rows_kernel<R, 4, N0>(var, shape) : rows_kernel<R, 4, N0 + 1>(var, shape);
 }
 
-// dynamic LDS of the row-tile kernel: [NBUF][8][TY + 2r][256]
+// dynamic LDS of the row-tile kernel: [NBUF][8][TY + 2r][ZS]
 template <typename R>
-static size_t rows_lds(int s, int var) {
+static size_t rows_lds(int s, int var, int shape) {
   const int nb = (sizeof(R) == 4 && (var & 1)) ? 2 : 1;
-  return (size_t)nb * 8 * (RowsCfg<R>::TY + 2 * (s - 1)) * 256 * sizeof(R);
+  return (size_t)nb * 8 * (rows_ty<R>(shape) + 2 * (s - 1)) * rows_zs<R>(shape) * sizeof(R);
 }
 
 template <typename R>
 static void launch_rows(const Tiled3D& ta, int s, const R* f, R* fn, R* df, const double* hs, hipStream_t st) {
-  using Cf = RowsCfg<R>;
   const int nz = ta.hi[2] - ta.lo[2];
-  const int nt = ((Cf::TY * nz + 63) / 64) * 64;
-  const size_t lds = rows_lds<R>(s, ta.rows_var);
+  const int nt = ((rows_ty<R>(ta.rows_shape) * nz + 63) / 64) * 64;
+  const size_t lds = rows_lds<R>(s, ta.rows_var, ta.rows_shape);
   const dim3 grid((unsigned)(((ta.ntiles + 7) / 8) * 8));
   void* args[] = {(void*)&ta, (void*)&f, (void*)&fn, (void*)&df, (void*)&hs};
-  HIPCHK(hipLaunchKernel(rows_kernel_for<R>(s, ta.rows_nht, ta.rows_var), grid, dim3(nt), args, lds, st));
+  HIPCHK(hipLaunchKernel(rows_kernel_for<R>(s, ta.rows_nht, ta.rows_var, ta.rows_shape), grid, dim3(nt), args, lds,
+                         st));
 }
 
 // The external SAT of a face normal to axis d, as the 9x9 map M[row][col] of
@@ -459,11 +478,15 @@ Solver<R>::Solver(const std::string& path, const fdf_opts& o) {
       for (int e = 0; e < 6; e++) ta.face_active[e] = D.face[e].active;
       // row-shaped tiles when TY whole rows fit one workgroup and its halo
       // needs at most NHT0 + 1 entries per thread (FDF_ROWS=0 keeps 64-wide
-      // tiles; FDF_ROWS=1 single-buffered, 2 double-buffered LDS tile)
+      // tiles; FDF_ROWS=1 single-buffered, 2 double-buffered LDS tile;
+      // FDF_ROWS_SHAPE=0 keeps 4-row workgroups for short rows)
       {
-        using Cf = RowsCfg<R>;
         const int nz = ta.hi[2] - ta.lo[2], r = S.sbporder - 1;
-        const int nt = ((Cf::TY * nz + 63) / 64) * 64;
+        const char* es = getenv("FDF_ROWS_SHAPE");
+        const int shape = (sizeof(R) == 4 && nz <= RowsCfg<R, 1>::ZS && !(es && atoi(es) == 0)) ? 1 : 0;
+        const int TYs = rows_ty<R>(shape), NTM = shape ? RowsCfg<R, 1>::NTMAX : RowsCfg<R, 0>::NTMAX;
+        const int N0 = RowsCfg<R, 0>::NHT0;
+        const int nt = ((TYs * nz + 63) / 64) * 64;
         const int nht = (2 * r * nz + nt - 1) / nt;
         int dev = 0, maxlds = 0;
         HIPCHK(hipGetDevice(&dev));
@@ -472,15 +495,16 @@ Solver<R>::Solver(const std::string& path, const fdf_opts& o) {
         const bool want = !(ev && atoi(ev) == 0);
         int var = rows_var_default;
         if (ev && atoi(ev) > 0) var = (atoi(ev) - 1) & 1;
-        if (rows_lds<R>(S.sbporder, var) + 1024 > (size_t)maxlds) var &= ~1;
-        const size_t lds = rows_lds<R>(S.sbporder, var);
+        if (rows_lds<R>(S.sbporder, var, shape) + 1024 > (size_t)maxlds) var &= ~1;
+        const size_t lds = rows_lds<R>(S.sbporder, var, shape);
         // (FP32 with sbporder 4 exceeds the 128-register budget of 1024 threads)
         const bool fits = !(sizeof(R) == 4 && S.sbporder == 4);
-        if (want && fits && nz <= 256 && nt <= Cf::NTMAX && nht <= Cf::NHT0 + 1 && lds + 1024 <= (size_t)maxlds) {
-          ta.rows_nht = std::max(nht, Cf::NHT0);
+        if (want && fits && nz <= 256 && nt <= NTM && nht <= N0 + 1 && lds + 1024 <= (size_t)maxlds) {
+          ta.rows_nht = std::max(nht, N0);
           ta.rows_var = var;
-          ta.nyt = (ta.hi[1] - ta.lo[1] + Cf::TY - 1) / Cf::TY;
-          HIPCHK(hipFuncSetAttribute(rows_kernel_for<R>(S.sbporder, ta.rows_nht, var),
+          ta.rows_shape = shape;
+          ta.nyt = (ta.hi[1] - ta.lo[1] + TYs - 1) / TYs;
+          HIPCHK(hipFuncSetAttribute(rows_kernel_for<R>(S.sbporder, ta.rows_nht, var, shape),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
           ta.ntiles = ta.nyt * ta.nxc;
         }

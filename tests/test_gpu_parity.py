@@ -252,14 +252,17 @@ def test_mode2_curved_plastic(tmp_path):
     _check(p, 15)
 
 
-@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("precision,shape", [("fp64", "auto"), ("fp32", "auto"), ("fp32", "0")])
 @pytest.mark.parametrize("var", [1, 2])
-def test_row_kernel_variants_random_state(tmp_path, monkeypatch, precision, var):
-    """Both variants of the row-tile kernel (FDF_ROWS=1 single-buffered,
-    2 double-buffered LDS tile) reproduce the oracle's RK stages from a
-    random state."""
+def test_row_kernel_variants_random_state(tmp_path, monkeypatch, precision, shape, var):
+    """Every variant of the row-tile kernel (FDF_ROWS=1 single-buffered,
+    2 double-buffered LDS tile; for FP32 rows of at most 160 points the
+    6-row workgroup, or with FDF_ROWS_SHAPE=0 the 4-row one) reproduces the
+    oracle's RK stages from a random state."""
     from fdfault_amd import lib
     monkeypatch.setenv("FDF_ROWS", str(var))
+    if shape != "auto":
+        monkeypatch.setenv("FDF_ROWS_SHAPE", shape)
     p = write_case(str(tmp_path), "rv", ndim=3, nx=37, ny=22, nz=71, dx=0.1, nt=2, sbporder=3)
     o = O.Oracle(p)
     g = lib.Solver(p, precision=lib.FP32 if precision == "fp32" else lib.FP64)
@@ -275,7 +278,7 @@ def test_row_kernel_variants_random_state(tmp_path, monkeypatch, precision, var)
         o.stage(0, st)
         g.stage(0, st)
         edf, ef = _rel(g.array("df"), o.array("df")), _rel(g.array("f"), o.array("f"))
-        print(precision, "FDF_ROWS", var, "stage", st, "df %.2e f %.2e" % (edf, ef), flush=True)
+        print(precision, "shape", shape, "FDF_ROWS", var, "stage", st, "df %.2e f %.2e" % (edf, ef), flush=True)
         assert edf <= tol and ef <= tol
     g.close()
     o.close()
