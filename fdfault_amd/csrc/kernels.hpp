@@ -1210,7 +1210,7 @@ struct Tiled3D {
   double s0[6];
   int rows_nht;                  // > 0: row-tile kernel with this many halo entries per thread
   int rows_var;                  // row-tile variant: 1 double-buffered LDS tile, 0 single
-  int rows_shape;                // row-tile workgroup: 0 TY = 4 rows, LDS stride 256; 1 TY = 6, stride 160 (FP32)
+  int rows_shape;                // row-tile workgroup (FP32): 0 TY = 4 rows, 1 TY = 6 (LDS stride 160), 2 TY = 5
   int face_active[6];            // external SAT faces
   // external SAT of a Cartesian face normal to axis d (other axes a < b) as
   // its 16 non-zero linear-map entries, [6 faces][16], in the bulk precision,

@@ -194,12 +194,13 @@ static const int rows_var_default = 1;
 
 // row-tile workgroup shapes: 0 = FP32 4 rows (<= 1024 threads, 4 waves per
 // SIMD) or FP64 2 rows (<= 512 threads, 2 waves per SIMD), LDS rows padded to
-// 256 points; 1 = FP32 6 rows of at most 160 points (<= 1024 threads), LDS
-// rows padded to 160, so that short rows still give 15-16 waves per CU
+// 256 points; 1 = FP32 6 rows of at most 160 points, LDS rows padded to 160;
+// 2 = FP32 5 rows of at most 204 points.  As many rows as fit 1024 threads,
+// so that a workgroup fills the 16 wave slots of a CU.
 template <typename R, int SH>
 struct RowsCfg {
   static constexpr bool F32 = sizeof(R) == 4;
-  static constexpr int TY = F32 ? (SH == 1 ? 6 : 4) : 2;
+  static constexpr int TY = F32 ? (SH == 1 ? 6 : (SH == 2 ? 5 : 4)) : 2;
   static constexpr int ZS = SH == 1 ? 160 : 256;
   static constexpr int NTMAX = F32 ? 1024 : 512;
   static constexpr int WPE = F32 ? 4 : 2;
@@ -207,9 +208,13 @@ struct RowsCfg {
 };
 
 template <typename R>
-static int rows_ty(int shape) { return shape == 1 ? RowsCfg<R, 1>::TY : RowsCfg<R, 0>::TY; }
+static int rows_ty(int shape) {
+  return shape == 1 ? RowsCfg<R, 1>::TY : (shape == 2 ? RowsCfg<R, 2>::TY : RowsCfg<R, 0>::TY);
+}
 template <typename R>
-static int rows_zs(int shape) { return shape == 1 ? RowsCfg<R, 1>::ZS : RowsCfg<R, 0>::ZS; }
+static int rows_zs(int shape) {
+  return shape == 1 ? RowsCfg<R, 1>::ZS : (shape == 2 ? RowsCfg<R, 2>::ZS : RowsCfg<R, 0>::ZS);
+}
 
 template <typename R, int S, int NHT, int SH>
 static const void* rows_kernel_sh(int var) {
@@ -223,6 +228,7 @@ template <typename R, int S, int NHT>
 static const void* rows_kernel(int var, int shape) {
   if constexpr (sizeof(R) == 4) {
     if (shape == 1) return rows_kernel_sh<R, S, NHT, 1>(var);
+    if (shape == 2) return rows_kernel_sh<R, S, NHT, 2>(var);
   }
   return rows_kernel_sh<R, S, NHT, 0>(var);
 }
@@ -483,8 +489,10 @@ Solver<R>::Solver(const std::string& path, const fdf_opts& o) {
       {
         const int nz = ta.hi[2] - ta.lo[2], r = S.sbporder - 1;
         const char* es = getenv("FDF_ROWS_SHAPE");
-        const int shape = (sizeof(R) == 4 && nz <= RowsCfg<R, 1>::ZS && !(es && atoi(es) == 0)) ? 1 : 0;
-        const int TYs = rows_ty<R>(shape), NTM = shape ? RowsCfg<R, 1>::NTMAX : RowsCfg<R, 0>::NTMAX;
+        int shape = 0;
+        if (sizeof(R) == 4 && !(es && atoi(es) == 0))
+          shape = nz <= RowsCfg<R, 1>::ZS ? 1 : (RowsCfg<R, 2>::TY * nz <= 1024 ? 2 : 0);
+        const int TYs = rows_ty<R>(shape), NTM = RowsCfg<R, 0>::NTMAX;
         const int N0 = RowsCfg<R, 0>::NHT0;
         const int nt = ((TYs * nz + 63) / 64) * 64;
         const int nht = (2 * r * nz + nt - 1) / nt;

@@ -252,22 +252,25 @@ def test_mode2_curved_plastic(tmp_path):
     _check(p, 15)
 
 
-@pytest.mark.parametrize("precision,shape", [("fp64", "auto"), ("fp32", "auto"), ("fp32", "0")])
+@pytest.mark.parametrize("precision,shape,nz", [("fp64", "auto", 71), ("fp32", "auto", 71), ("fp32", "0", 71),
+                                                ("fp32", "auto", 181)])
 @pytest.mark.parametrize("var", [1, 2])
-def test_row_kernel_variants_random_state(tmp_path, monkeypatch, precision, shape, var):
+def test_row_kernel_variants_random_state(tmp_path, monkeypatch, precision, shape, nz, var):
     """Every variant of the row-tile kernel (FDF_ROWS=1 single-buffered,
-    2 double-buffered LDS tile; for FP32 rows of at most 160 points the
-    6-row workgroup, or with FDF_ROWS_SHAPE=0 the 4-row one) reproduces the
-    oracle's RK stages from a random state."""
+    2 double-buffered LDS tile; FP32 workgroups of 6 rows for rows of at
+    most 160 points, 5 rows up to 204, or with FDF_ROWS_SHAPE=0 always 4)
+    reproduces the oracle's RK stages from a random state."""
     from fdfault_amd import lib
     monkeypatch.setenv("FDF_ROWS", str(var))
     if shape != "auto":
         monkeypatch.setenv("FDF_ROWS_SHAPE", shape)
-    p = write_case(str(tmp_path), "rv", ndim=3, nx=37, ny=22, nz=71, dx=0.1, nt=2, sbporder=3)
+    p = write_case(str(tmp_path), "rv", ndim=3, nx=37, ny=22, nz=nz, dx=0.1, nt=2, sbporder=3)
     o = O.Oracle(p)
     g = lib.Solver(p, precision=lib.FP32 if precision == "fp32" else lib.FP64)
     assert g.info.row_blocks == 2
-    tol = 1e-12 if precision == "fp64" else 2e-6
+    # FP32: rounding of ~30-term stencil sums accumulated over 5 stages of random
+    # data stays near 1e-6; an index or coefficient error would be O(0.1)
+    tol = 1e-12 if precision == "fp64" else 1e-5
     rng = np.random.default_rng(13)
     f = rng.normal(size=o.array("f").shape)
     df = rng.normal(size=o.array("df").shape)
