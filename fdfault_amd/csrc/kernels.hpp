@@ -536,10 +536,11 @@ __global__ void advance_step(int* step) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *step += 1;
 }
 
-__device__ __forceinline__ double pert_val(const double* xy, const double* par, int k, int q, int n, int p, double t) {
+// value of perturbation k, quantity q, at shape value xyv = xy[k][p]
+__device__ __forceinline__ double pert_valx(double xyv, const double* par, int k, int q, int n, int p, double t) {
   const double t0 = par[k * 10];
   const double tf = (t >= t0) ? 1. : t / t0;
-  return par[k * 10 + 1 + q] * xy[(long long)k * n + p] * tf;
+  return par[k * 10 + 1 + q] * xyv * tf;
 }
 
 // STZ slip rate and its derivative, stz.cpp:436-484
@@ -665,6 +666,38 @@ __device__ __forceinline__ void fault_point(const FaultDev& F, const GridDev& g,
                                             int* __restrict__ newton_iters, int p, const float* mlp,
                                             const Hook& hook = Hook(), bool live = true) {
   const int law = LAW >= 0 ? LAW : F.law;
+  // this point's fault state and perturbation shape values, loaded before any
+  // other memory access so that they are in flight together (and before any
+  // write of this function, which may not alias them)
+  double u_p = 0., du_p = 0., ux_p[2] = {0., 0.}, dux_p[2] = {0., 0.}, state_p = 0., dstate_p = 0., front_p = 0.;
+  double lxy[4] = {0., 0., 0., 0.}, pxy[4] = {0., 0., 0., 0.};
+  if (law != 0) {
+    const int nn_ = F.n;
+    u_p = F.u[p];
+    du_p = F.du[p];
+#pragma unroll
+    for (int q = 0; q < ((M == 2) ? 2 : 1); q++) {
+      ux_p[q] = F.ux[(long long)q * nn_ + p];
+      dux_p[q] = F.dux[(long long)q * nn_ + p];
+    }
+    if (law == 3) {
+      state_p = F.state[p];
+      dstate_p = F.dstate[p];
+    }
+    if (F.front && F.front_field == 0) front_p = F.front[p];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      if (k < F.nloads) lxy[k] = F.load_xy[(long long)k * nn_ + p];
+      if (k < F.nperts) pxy[k] = F.pert_xy[(long long)k * nn_ + p];
+    }
+  }
+  // shape value of load / perturbation k at this point (preloaded for k < 4)
+  auto load_x = [&](int k) {
+    return k == 0 ? lxy[0] : k == 1 ? lxy[1] : k == 2 ? lxy[2] : k == 3 ? lxy[3] : F.load_xy[(long long)k * F.n + p];
+  };
+  auto pert_x = [&](int k) {
+    return k == 0 ? pxy[0] : k == 1 ? pxy[1] : k == 2 ? pxy[2] : k == 3 ? pxy[3] : F.pert_xy[(long long)k * F.n + p];
+  };
   const int ii = p / F.n1, jj = p % F.n1;
   const long long N = g.N;
   const long long stride[3] = {(long long)g.Ny * g.Nz, (long long)g.Nz, 1};
@@ -768,9 +801,9 @@ __device__ __forceinline__ void fault_point(const FaultDev& F, const GridDev& g,
       s13 += F.lf[2 * n + p]; s23 += F.lf[2 * n + p];
     }
     for (int k = 0; k < F.nloads; k++) {
-      snc += pert_val(F.load_xy, F.load_par, k, 0, n, p, t);
-      const double l2 = pert_val(F.load_xy, F.load_par, k, 1, n, p, t);
-      const double l3 = pert_val(F.load_xy, F.load_par, k, 2, n, p, t);
+      snc += pert_valx(load_x(k), F.load_par, k, 0, n, p, t);
+      const double l2 = pert_valx(load_x(k), F.load_par, k, 1, n, p, t);
+      const double l3 = pert_valx(load_x(k), F.load_par, k, 2, n, p, t);
       s12 += l2; s22 += l2; s13 += l3; s23 += l3;
     }
     const double phi2 = eta * (s12 / z1 - v12 + s22 / z2 + v22);
@@ -778,28 +811,28 @@ __device__ __forceinline__ void fault_point(const FaultDev& F, const GridDev& g,
     const double phi = sqrt(phi2 * phi2 + phi3 * phi3);
     double lv_in = NAN;
     if constexpr (!std::is_same<Hook, NoHook>::value) {
-      lv_in = hook(phi, snc, eta, F.state[p]);
+      lv_in = hook(phi, snc, eta, state_p);
       if (!live) return;
     }
     double V, S, dstatedt = 0.;
     if (law == 2) {  // slipweak::solve_fs / calc_mu
       double c0t = 0.;
-      for (int k = 0; k < F.nperts; k++) c0t += pert_val(F.pert_xy, F.pert_par, k, 3, n, p, t);
+      for (int k = 0; k < F.nperts; k++) c0t += pert_valx(pert_x(k), F.pert_par, k, 3, n, p, t);
       if (F.param_file) c0t += F.pf[3 * n + p];
       if (snc < 0.) {
         double dct = 0., must = 0., mudt = 0., trupt = 0., tct = 0.;
         for (int k = 0; k < F.nperts; k++) {
-          dct += pert_val(F.pert_xy, F.pert_par, k, 0, n, p, t);
-          must += pert_val(F.pert_xy, F.pert_par, k, 1, n, p, t);
-          mudt += pert_val(F.pert_xy, F.pert_par, k, 2, n, p, t);
-          trupt += pert_val(F.pert_xy, F.pert_par, k, 4, n, p, t);
-          tct += pert_val(F.pert_xy, F.pert_par, k, 5, n, p, t);
+          dct += pert_valx(pert_x(k), F.pert_par, k, 0, n, p, t);
+          must += pert_valx(pert_x(k), F.pert_par, k, 1, n, p, t);
+          mudt += pert_valx(pert_x(k), F.pert_par, k, 2, n, p, t);
+          trupt += pert_valx(pert_x(k), F.pert_par, k, 4, n, p, t);
+          tct += pert_valx(pert_x(k), F.pert_par, k, 5, n, p, t);
         }
         if (F.param_file) {
           dct += F.pf[p]; must += F.pf[n + p]; mudt += F.pf[2 * n + p];
           trupt += F.pf[4 * n + p]; tct += F.pf[5 * n + p];
         }
-        const double u = F.u[p];
+        const double u = u_p;
         const double f1 = (dct == 0. || u >= dct) ? 1. : u / dct;
         double f2;
         if (trupt <= 0. || t < trupt) f2 = 0.;
@@ -816,12 +849,12 @@ __device__ __forceinline__ void fault_point(const FaultDev& F, const GridDev& g,
       if (snc < 0.) {
         double mu = 0.;
         if (law == 3) {
-          double pr[8] = {phi, eta, snc, F.state[p], 0., 0., 0., 0.};
+          double pr[8] = {phi, eta, snc, state_p, 0., 0., 0., 0.};
           for (int k = 0; k < F.nperts; k++) {
-            pr[4] += pert_val(F.pert_xy, F.pert_par, k, 0, n, p, t);
-            pr[5] += pert_val(F.pert_xy, F.pert_par, k, 1, n, p, t);
-            pr[6] += pert_val(F.pert_xy, F.pert_par, k, 2, n, p, t);
-            pr[7] += pert_val(F.pert_xy, F.pert_par, k, 3, n, p, t);
+            pr[4] += pert_valx(pert_x(k), F.pert_par, k, 0, n, p, t);
+            pr[5] += pert_valx(pert_x(k), F.pert_par, k, 1, n, p, t);
+            pr[6] += pert_valx(pert_x(k), F.pert_par, k, 2, n, p, t);
+            pr[7] += pert_valx(pert_x(k), F.pert_par, k, 3, n, p, t);
           }
           if (F.param_file) { pr[4] += F.pf[p]; pr[5] += F.pf[n + p]; pr[6] += F.pf[2 * n + p]; pr[7] += F.pf[3 * n + p]; }
           if (!(pr[3] > 0.)) atomicOr(err, 8);
@@ -885,11 +918,11 @@ __device__ __forceinline__ void fault_point(const FaultDev& F, const GridDev& g,
       if (law == 3) {  // stz::calc_dstatedt / chihat
         double c0t = 0., Rt = 0., betat = 0., chiwt = 0., v1t = 0.;
         for (int k = 0; k < F.nperts; k++) {
-          c0t += pert_val(F.pert_xy, F.pert_par, k, 4, n, p, t);
-          Rt += pert_val(F.pert_xy, F.pert_par, k, 5, n, p, t);
-          betat += pert_val(F.pert_xy, F.pert_par, k, 6, n, p, t);
-          chiwt += pert_val(F.pert_xy, F.pert_par, k, 7, n, p, t);
-          v1t += pert_val(F.pert_xy, F.pert_par, k, 8, n, p, t);
+          c0t += pert_valx(pert_x(k), F.pert_par, k, 4, n, p, t);
+          Rt += pert_valx(pert_x(k), F.pert_par, k, 5, n, p, t);
+          betat += pert_valx(pert_x(k), F.pert_par, k, 6, n, p, t);
+          chiwt += pert_valx(pert_x(k), F.pert_par, k, 7, n, p, t);
+          v1t += pert_valx(pert_x(k), F.pert_par, k, 8, n, p, t);
         }
         if (F.param_file) {
           c0t += F.pf[4 * n + p]; Rt += F.pf[5 * n + p]; betat += F.pf[6 * n + p];
@@ -897,7 +930,7 @@ __device__ __forceinline__ void fault_point(const FaultDev& F, const GridDev& g,
         }
         if (V > v1t) atomicOr(err, 16);
         const double chihat = (V == 0.) ? chiwt : chiwt / log(v1t / V);
-        dstatedt = V * S / c0t * (1. - F.state[p] / chihat) - Rt * exp(-betat / F.state[p]);
+        dstatedt = V * S / c0t * (1. - state_p / chihat) - Rt * exp(-betat / state_p);
       }
     }
     double vv2, vv3;
@@ -927,28 +960,33 @@ __device__ __forceinline__ void fault_point(const FaultDev& F, const GridDev& g,
       hs13 -= F.lf[2 * n + p]; hs23 -= F.lf[2 * n + p];
     }
     for (int k = 0; k < F.nloads; k++) {
-      const double l2 = pert_val(F.load_xy, F.load_par, k, 1, n, p, t);
-      const double l3 = pert_val(F.load_xy, F.load_par, k, 2, n, p, t);
+      const double l2 = pert_valx(load_x(k), F.load_par, k, 1, n, p, t);
+      const double l3 = pert_valx(load_x(k), F.load_par, k, 2, n, p, t);
       hs12 -= l2; hs22 -= l2; hs13 -= l3; hs23 -= l3;
     }
     if (!fs.no_sat) {  // friction::scale_df + calc_df + update for this point
-      F.du[p] = F.du[p] * fs.A;
-      F.du[p] += fs.dt * V;
-      F.u[p] += fs.B * F.du[p];
+      double du = du_p * fs.A;
+      du += fs.dt * V;
+      F.du[p] = du;
+      F.u[p] = u_p + fs.B * du;
       const int nc = (M == 2) ? 2 : 1;
+      const double vxl[2] = {(M == 1) ? vv3 : vv2, vv3};  // the F.vx values written above
+#pragma unroll
       for (int q = 0; q < nc; q++) {
         const long long o = (long long)q * n + p;
-        F.dux[o] = F.dux[o] * fs.A;
-        F.dux[o] += fs.dt * F.vx[o];
-        F.ux[o] += fs.B * F.dux[o];
+        double dux = dux_p[q] * fs.A;
+        dux += fs.dt * vxl[q];
+        F.dux[o] = dux;
+        F.ux[o] = ux_p[q] + fs.B * dux;
       }
       if (law == 3) {
-        F.dstate[p] = F.dstate[p] * fs.A;
-        F.dstate[p] += fs.dt * dstatedt;
-        F.state[p] += fs.B * F.dstate[p];
+        double dst = dstate_p * fs.A;
+        dst += fs.dt * dstatedt;
+        F.dstate[p] = dst;
+        F.state[p] = state_p + fs.B * dst;
       }
     }
-    if (F.front && tfront >= 0. && F.front_field == 0 && F.front[p] < 0. && V >= F.front_value)
+    if (F.front && tfront >= 0. && F.front_field == 0 && front_p < 0. && V >= F.front_value)
       F.front[p] = tfront;
   }
   if (fs.no_sat) return;
