@@ -33,6 +33,24 @@ template <typename T>
 struct DBuf {
   T* p = nullptr;
   size_t n = 0;
+  // owns its allocation: movable, not copyable
+  DBuf() = default;
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  DBuf(DBuf&& o) noexcept : p(o.p), n(o.n) {
+    o.p = nullptr;
+    o.n = 0;
+  }
+  DBuf& operator=(DBuf&& o) noexcept {
+    if (this != &o) {
+      free();
+      p = o.p;
+      n = o.n;
+      o.p = nullptr;
+      o.n = 0;
+    }
+    return *this;
+  }
   void alloc(size_t count) {
     free();
     n = count;
@@ -738,6 +756,7 @@ Solver<R>::Solver(const std::string& path, const fdf_opts& o) {
     }
   }
   HIPCHK(hipStreamSynchronize(stream));
+  smaps.resize(curv_ok.size() * 4);  // (sized once: the maps' device pointers are kept)
   for (size_t b = 0; b < curv_ok.size(); b++) {
     // the rectilinear kernel is preferred where it applies
     curv_ok[b] = curv_ok[b] && !tiled_ok[b] && bdev[b].nif <= 4;
@@ -757,7 +776,6 @@ Solver<R>::Solver(const std::string& path, const fdf_opts& o) {
     }
     // the SAT of every external face point as a [rows][5] map (rows: the
     // evolved fields, szz included for plastic mode II)
-    smaps.resize(bdev.size() * 4);
     const int nr = S.nf;
     std::vector<double> unit(25, 0.);
     for (int q = 0; q < 5; q++) unit[q * 5 + q] = 1.;
