@@ -122,7 +122,8 @@ def single_workload(args):
     # written once per step by the plasticity kernel, not by this kernel)
     B = bytes_per_point(nf, w, curvilinear=rough, ndim=2)
     B_stage = bytes_per_point(nf, w, curvilinear=rough, ndim=2, plastic_words=2 * (4 + 2) if rough else 0)
-    ppl = npts / nb
+    # point updates per bulk launch (blocks may share one launch)
+    ppl = npts * nst * args.steps / nl
     kernel = "bulk_tiled2d" if tiled else ("bulk_curv2d" if curv else "bulk_kernel")
     wname = "rough2d_plastic_2x1601x801" if rough else "planar2d_2x801x401"
     from fdfault_amd.profiling import lookup_traffic

@@ -2175,8 +2175,8 @@ __global__ void __launch_bounds__(TW) bulk_tiled2d_batch(const Tiled2D* __restri
 // external faces add the SAT of boundary.cpp:265-492 through its linear map
 // at that point (ta.smap, built by face_map_kernel from ext_sat).
 template <typename R, typename C, int S, int TW, bool PL>
-__global__ void __launch_bounds__(TW) bulk_curv2d(Tiled2D ta, BlockDev bd, const R* __restrict__ mt,
-                                                  const R* __restrict__ jac, int par) {
+__device__ __forceinline__ void curv2d_tile(const Tiled2D& ta, const BlockDev& bd, const R* __restrict__ mt,
+                                            const R* __restrict__ jac, int tile_id, int par) {
 #pragma clang fp contract(fast)
   constexpr int NF = PL ? 6 : 5;
   constexpr int r = S - 1, W = 2 * r + 1;
@@ -2185,10 +2185,6 @@ __global__ void __launch_bounds__(TW) bulk_curv2d(Tiled2D ta, BlockDev bd, const
   constexpr int LW = TW + 2 * H;
   __shared__ C row[4][LW];          // vx vy Gx Gy
   __shared__ C sfd[NRB][NTB];
-  const int nxcd = 8;
-  const int per = (ta.ntiles + nxcd - 1) / nxcd;
-  const int tile_id = (blockIdx.x % nxcd) * per + blockIdx.x / nxcd;
-  if (tile_id >= ta.ntiles) return;
   const R* __restrict__ f = reinterpret_cast<const R*>(ta.f[par]);
   R* __restrict__ fn = const_cast<R*>(reinterpret_cast<const R*>(ta.f[par ^ 1]));
   R* __restrict__ df = reinterpret_cast<R*>(ta.df);
@@ -2395,6 +2391,31 @@ __global__ void __launch_bounds__(TW) bulk_curv2d(Tiled2D ta, BlockDev bd, const
 #pragma unroll
       for (int q = 0; q < 4; q++) win[q][n] = win[q][n + 1];
   }
+}
+
+// one block: workgroup b is tile b in XCD-aware order
+template <typename R, typename C, int S, int TW, bool PL>
+__global__ void __launch_bounds__(TW) bulk_curv2d(Tiled2D ta, BlockDev bd, const R* __restrict__ mt,
+                                                  const R* __restrict__ jac, int par) {
+  const int nxcd = 8;
+  const int per = (ta.ntiles + nxcd - 1) / nxcd;
+  const int tile_id = (blockIdx.x % nxcd) * per + blockIdx.x / nxcd;
+  if (tile_id >= ta.ntiles) return;
+  curv2d_tile<R, C, S, TW, PL>(ta, bd, mt, jac, tile_id, par);
+}
+
+// every block of a problem in one launch: work item (block, tile), one
+// workgroup per item (grid a multiple of 8, XCD-grouped as in
+// bulk_tiled2d_batch)
+template <typename R, typename C, int S, int TW, bool PL>
+__global__ void __launch_bounds__(TW) bulk_curv2d_multi(const Tiled2D* __restrict__ descs,
+                                                        const BlockDev* __restrict__ bds, const int2* __restrict__ work,
+                                                        int nwork, const R* __restrict__ mt, const R* __restrict__ jac,
+                                                        int par) {
+  const int w = (blockIdx.x % 8) * (gridDim.x / 8) + blockIdx.x / 8;
+  if (w >= nwork) return;
+  const int2 it = work[w];
+  curv2d_tile<R, C, S, TW, PL>(descs[it.x], bds[it.x], mt, jac, it.y, par);
 }
 
 // ensemble fault update: work item = (instance, first fault point of a

@@ -149,6 +149,12 @@ class Solver : public SolverBase {
   std::vector<bool> tiled_ok;
   std::vector<bool> curv_ok;  // 2-D mode II block on the curvilinear tiled kernel
   std::vector<DBuf<R>> smaps;  // per block and face: the external SAT map of each face point
+  // 2-D problems whose blocks all take the same tiled kernel: one launch per
+  // stage over every block's tiles (merged 1: rectilinear, 2: curvilinear)
+  int merged = 0, cur_stage = 0, mnwork = 0;
+  std::vector<DBuf<Tiled2D>> mdesc;  // [stage][roundtrip] per-block descriptors
+  DBuf<BlockDev> mbd;
+  DBuf<int2> mwork;                  // (block, tile)
   std::vector<Tiled3D> tiles;
   std::vector<Tiled2D> tiles2;
   int tile2_tw = 64, tile2_cx = 2;  // 2-D tile: y points per workgroup, x planes per chunk
@@ -800,6 +806,31 @@ Solver<R>::Solver(const std::string& path, const fdf_opts& o) {
       t2.smap[fc] = smaps[b * 4 + fc].p;
     }
   }
+  {
+    bool allt = nd == 2 && S.blocks.size() > 1, allc = allt;
+    for (size_t b = 0; b < S.blocks.size() && (allt || allc); b++) {
+      allt = allt && tiled_ok[b] && tiles2[b].tw == tiles2[0].tw;
+      allc = allc && curv_ok[b] && tiles2[b].tw == tiles2[0].tw;
+    }
+    const char* em = getenv("FDF_MERGE2D");
+    if ((allt || allc) && !(em && atoi(em) == 0)) {
+      merged = allt ? 1 : 2;
+      std::vector<int2> wl;
+      for (size_t b = 0; b < S.blocks.size(); b++)
+        for (int tl = 0; tl < tiles2[b].ntiles; tl++) wl.push_back(make_int2((int)b, tl));
+      mwork.upload(wl);
+      mnwork = (int)wl.size();
+      mdesc.resize((size_t)S.rk.nstages * 2);
+      for (int st = 0; st < S.rk.nstages; st++)
+        for (int rt = 0; rt < 2; rt++) {
+          const StageDev sd{S.rk.A[st], S.rk.B[st], S.dt, rt};
+          std::vector<Tiled2D> d;
+          for (size_t b = 0; b < S.blocks.size(); b++) d.push_back(stage_tile2((int)b, sd));
+          mdesc[(size_t)st * 2 + rt].upload(d);
+        }
+      if (merged == 2) mbd.upload(bdev);
+    }
+  }
   for (size_t b = 0; b < tiled_ok.size(); b++) {
     ncurv += curv_ok[b] ? 1 : 0;
     ntiled += tiled_ok[b] ? 1 : 0;
@@ -1012,6 +1043,31 @@ static void launch_curv2_tw(const Tiled2D& t2, const BlockDev& D, const R* mt, c
   else launch_curv2_pl<R, TW, false>(t2, D, mt, jac, s, par, st);
 }
 
+// merged launches over every block of a 2-D problem, one workgroup per tile
+template <typename R, int TW>
+static void launch_merged_tw(const Tiled2D* d, const int2* wk, int nw, int s, int par, dim3 grid, hipStream_t st) {
+  if (s == 2) hipLaunchKernelGGL((bulk_tiled2d_batch<R, R, 2, TW>), grid, dim3(TW), 0, st, d, wk, nw, par);
+  else if (s == 3) hipLaunchKernelGGL((bulk_tiled2d_batch<R, R, 3, TW>), grid, dim3(TW), 0, st, d, wk, nw, par);
+  else hipLaunchKernelGGL((bulk_tiled2d_batch<R, R, 4, TW>), grid, dim3(TW), 0, st, d, wk, nw, par);
+}
+
+template <typename R, int TW, bool PL>
+static void launch_mcurv_pl(const Tiled2D* d, const BlockDev* bds, const int2* wk, int nw, const R* mt, const R* jac,
+                            int s, int par, dim3 grid, hipStream_t st) {
+  if (s == 2)
+    hipLaunchKernelGGL((bulk_curv2d_multi<R, R, 2, TW, PL>), grid, dim3(TW), 0, st, d, bds, wk, nw, mt, jac, par);
+  else if (s == 3)
+    hipLaunchKernelGGL((bulk_curv2d_multi<R, R, 3, TW, PL>), grid, dim3(TW), 0, st, d, bds, wk, nw, mt, jac, par);
+  else hipLaunchKernelGGL((bulk_curv2d_multi<R, R, 4, TW, PL>), grid, dim3(TW), 0, st, d, bds, wk, nw, mt, jac, par);
+}
+
+template <typename R, int TW>
+static void launch_mcurv_tw(const Tiled2D* d, const BlockDev* bds, const int2* wk, int nw, const R* mt, const R* jac,
+                            int s, bool plastic, int par, dim3 grid, hipStream_t st) {
+  if (plastic) launch_mcurv_pl<R, TW, true>(d, bds, wk, nw, mt, jac, s, par, grid, st);
+  else launch_mcurv_pl<R, TW, false>(d, bds, wk, nw, mt, jac, s, par, grid, st);
+}
+
 // stage descriptor of a 2-D rectilinear block: RK coefficients and arrays
 template <typename R>
 Tiled2D Solver<R>::stage_tile2(int b, const StageDev& sd) const {
@@ -1055,6 +1111,23 @@ void Solver<R>::launch_bulk(int b, const StageDev& sd) {
   const BlockDev& D = bdev[b];
   const bool diss = S.cdiss > 0., het = S.hetmat;
   const bool tiled = tiled_ok[b];
+  if (merged) {
+    const Tiled2D* dsc = mdesc[(size_t)cur_stage * 2 + (sd.roundtrip ? 1 : 0)].p;
+    const dim3 grid((unsigned)(((mnwork + 7) / 8) * 8));
+    const int tw = tiles2[0].tw, s = S.sbporder;
+    if (merged == 1) {
+      if (tw == 64) launch_merged_tw<R, 64>(dsc, mwork.p, mnwork, s, cur, grid, stream);
+      else if (tw == 128) launch_merged_tw<R, 128>(dsc, mwork.p, mnwork, s, cur, grid, stream);
+      else launch_merged_tw<R, 256>(dsc, mwork.p, mnwork, s, cur, grid, stream);
+    } else {
+      if (tw == 64) launch_mcurv_tw<R, 64>(dsc, mbd.p, mwork.p, mnwork, metric.p, jac.p, s, S.plastic, cur, grid, stream);
+      else if (tw == 128)
+        launch_mcurv_tw<R, 128>(dsc, mbd.p, mwork.p, mnwork, metric.p, jac.p, s, S.plastic, cur, grid, stream);
+      else launch_mcurv_tw<R, 256>(dsc, mbd.p, mwork.p, mnwork, metric.p, jac.p, s, S.plastic, cur, grid, stream);
+    }
+    HIPCHK(hipGetLastError());
+    return;
+  }
   if (tiled && S.ndim == 2) {
     Tiled2D t2 = stage_tile2(b, sd);
     launch_tiled2(t2, t2.tw, cur, stream);
@@ -1124,7 +1197,9 @@ void Solver<R>::do_stage(int istep, int st, bool full_step) {
     nev++;
   }
   StageDev sd{S.rk.A[st], S.rk.B[st], S.dt, (full_step && last && !S.plastic && sizeof(R) == 8) ? 1 : 0};
+  cur_stage = st;
   for (size_t b = 0; b < S.blocks.size(); b++) {
+    if (merged && b > 0) break;  // block 0's launch covers every block
     if (capturing) {
       launch_bulk((int)b, sd);
       continue;
