@@ -209,8 +209,8 @@ def test_kernel_level_random_state_3d(tmp_path, sbp):
         assert _rel(g.array("f"), o.array("f")) <= 1e-12
 
 
-@pytest.mark.parametrize("plastic", [False, True])
-def test_kernel_level_random_state_curvilinear(tmp_path, plastic):
+@pytest.mark.parametrize("plastic,sbp", [(False, 3), (True, 3), (True, 2), (False, 4), (True, 4)])
+def test_kernel_level_random_state_curvilinear(tmp_path, plastic, sbp):
     """One RK stage from a random state on curvilinear 2-D blocks, elastic and
     plastic: the curvilinear tiled kernel (per-point metric, conservative
     velocity rates, per-point external SAT maps, szz rate) against the
@@ -222,7 +222,7 @@ def test_kernel_level_random_state_curvilinear(tmp_path, plastic):
     if plastic:
         kw.update(material="plastic", plastic=dict(mu=0.68, c=0., beta=0.35, eta=0.05 * 32.04),
                   s0=(-100., 52., 0., -120., 0., -55.))
-    p = write_case(str(tmp_path), "rc", nx=nx, ny=ny, dx=0.1, nt=2, **kw)
+    p = write_case(str(tmp_path), "rc", nx=nx, ny=ny, dx=0.1, nt=2, sbporder=sbp, **kw)
     o = O.Oracle(p)
     g = lib.Solver(p)
     assert g.info.curv_blocks == 2 and g.info.tiled_blocks == 0
@@ -240,6 +240,29 @@ def test_kernel_level_random_state_curvilinear(tmp_path, plastic):
         assert edf <= 1e-12 and ef <= 1e-12
     g.close()
     o.close()
+
+
+@pytest.mark.parametrize("curved", [False, True])
+def test_merged_launch_matches_per_block(tmp_path, monkeypatch, curved):
+    """One bulk launch per stage over every block's tiles gives the same bits
+    as one launch per block (rectilinear elastic and curvilinear plastic)."""
+    from fdfault_amd import lib
+    kw = dict(nx=61, ny=31, dx=0.1, nt=12)
+    if curved:
+        kw.update(surf=curved_surfaces(str(tmp_path), 61, 31, 0.1, amp=0.1), material="plastic",
+                  plastic=dict(mu=0.68, c=0., beta=0.35, eta=0.05 * 32.04), s0=(-100., 52., 0., -120., 0., -55.),
+                  loads=NUC2)
+    p = write_case(str(tmp_path), "mg", **kw)
+    res = []
+    for m in ("1", "0"):
+        monkeypatch.setenv("FDF_MERGE2D", m)
+        g = lib.Solver(p, precision=lib.FP32)
+        assert (g.info.curv_blocks if curved else g.info.tiled_blocks) == 2
+        g.step(12)
+        res.append((g.array("f"), g.iface(0, "U"), g.front(0)))
+        g.close()
+    for a, b in zip(res[0], res[1]):
+        assert np.array_equal(a, b)
 
 
 def test_mode2_curved_plastic(tmp_path):
