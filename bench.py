@@ -124,7 +124,14 @@ def single_workload(args):
     B_stage = bytes_per_point(nf, w, curvilinear=rough, ndim=2, plastic_words=2 * (4 + 2) if rough else 0)
     # point updates per bulk launch (blocks may share one launch)
     ppl = npts * nst * args.steps / nl
-    kernel = "bulk_tiled2d" if tiled else ("bulk_curv2d" if curv else "bulk_kernel")
+    # one launch per stage covers every block when the blocks are merged
+    merged = nb > 1 and nl == nst * args.steps
+    if tiled:
+        kernel = "bulk_tiled2d_batch" if merged else "bulk_tiled2d"
+    elif curv:
+        kernel = "bulk_curv2d_multi" if merged else "bulk_curv2d"
+    else:
+        kernel = "bulk_kernel"
     wname = "rough2d_plastic_2x1601x801" if rough else "planar2d_2x801x401"
     from fdfault_amd.profiling import lookup_traffic
     tr = lookup_traffic("%s/%s/sbp%d/%s" % (wname, args.precision, 4 if rough else 3, kernel))

@@ -1249,6 +1249,7 @@ struct Tiled3D {
   int rows_nht;                  // > 0: row-tile kernel with this many halo entries per thread
   int rows_var;                  // row-tile variant: 1 double-buffered LDS tile, 0 single
   int rows_shape;                // row-tile workgroup (FP32): 0 TY = 4 rows, 1 TY = 6 (LDS stride 160), 2 TY = 5
+  int grp_y, grp_x;              // row-tile order: patches of grp_y y tiles x grp_x x chunks
   int face_active[6];            // external SAT faces
   // external SAT of a Cartesian face normal to axis d (other axes a < b) as
   // its 16 non-zero linear-map entries, [6 faces][16], in the bulk precision,
@@ -1694,7 +1695,22 @@ __global__ void __launch_bounds__(NTMAX) __attribute__((amdgpu_waves_per_eu(WPE)
   const int per = (ta.ntiles + nxcd - 1) / nxcd;
   const int tile_id = (blockIdx.x % nxcd) * per + blockIdx.x / nxcd;
   if (tile_id >= ta.ntiles) return;
-  const int yt = tile_id % ta.nyt, xc = tile_id / ta.nyt;
+  // Tile order: strips of grp_x x chunks spanning every y tile, cut into
+  // patches of grp_y y tiles (the last strip and patch may be narrower), y
+  // fastest inside a patch.  The workgroups an XCD runs at one time then
+  // cover a compact patch, so the y-halo rows and the x-window planes that
+  // neighbouring tiles share are read while the first reader's copy is still
+  // in that XCD's L2.  grp_x = 1, grp_y >= nyt is the plain y-fastest order.
+  int yt, xc;
+  {
+    const int gx = ta.grp_x, gy = ta.grp_y;
+    const int sx = tile_id / (ta.nyt * gx), rem = tile_id - sx * ta.nyt * gx;
+    const int wx = min(gx, ta.nxc - sx * gx);
+    const int by = rem / (gy * wx), h = min(gy, ta.nyt - by * gy);
+    const int w = rem - by * gy * wx;
+    yt = by * gy + w % h;
+    xc = sx * gx + w / h;
+  }
   const int ty = t / nz, kk = t - ty * nz;
   const int jb = ta.lo[1] + yt * TY, j = jb + ty, k = ta.lo[2] + kk;
   const bool active = ty < TY && j < ta.hi[1];
@@ -1735,7 +1751,9 @@ __global__ void __launch_bounds__(NTMAX) __attribute__((amdgpu_waves_per_eu(WPE)
   }
   constexpr int tf[8] = {0, 1, 2, 4, 5, 6, 7, 8};  // f index of each tile field
   constexpr int yq[6] = {0, 1, 2, 4, 6, 7};        // vx vy vz sxy syy syz
-  // halo entries: rows 0..r-1 and TY+r..TY+2r-1 of the tile, all nz columns
+  // halo entries: rows 0..r-1 and TY+r..TY+2r-1 of the tile, all nz columns;
+  // only the six fields with a y derivative (tile fields hq) are read there
+  constexpr int hq[6] = {0, 1, 2, 3, 5, 6};
   int hrow[NHT];
   unsigned hoff[NHT];
 #pragma unroll
@@ -1747,7 +1765,7 @@ __global__ void __launch_bounds__(NTMAX) __attribute__((amdgpu_waves_per_eu(WPE)
     const int gj = min(max(jb - r + lr, ta.lo[1]), ta.hi[1] - 1);
     hoff[u] = (unsigned)(((long long)gj * sy + ta.lo[2] + (hr < 2 * r ? hk : 0)) * (long long)sizeof(R));
   }
-  R nwin[6], nown[3], nhal[NHT][8];
+  R nwin[6], nown[3], nhal[NHT][6];
   auto issue = [&](int ip) {
     const unsigned pw = xplane(ip + r), pp = (unsigned)ip * psb;
 #pragma unroll
@@ -1758,7 +1776,7 @@ __global__ void __launch_bounds__(NTMAX) __attribute__((amdgpu_waves_per_eu(WPE)
     for (int u = 0; u < NHT; u++)
       if (hrow[u] >= 0) {
 #pragma unroll
-        for (int q = 0; q < 8; q++) nhal[u][q] = bld<R>(rf, hoff[u], tf[q] * fb + pp);
+        for (int q = 0; q < 6; q++) nhal[u][q] = bld<R>(rf, hoff[u], tf[hq[q]] * fb + pp);
       }
   };
   constexpr int LPL = LY * ZS;  // tile entries per field
@@ -1787,7 +1805,7 @@ __global__ void __launch_bounds__(NTMAX) __attribute__((amdgpu_waves_per_eu(WPE)
     for (int u = 0; u < NHT; u++)
       if (hrow[u] >= 0) {
 #pragma unroll
-        for (int q = 0; q < 8; q++) tile[q * LPL + hrow[u]] = nhal[u][q];
+        for (int q = 0; q < 6; q++) tile[hq[q] * LPL + hrow[u]] = nhal[u][q];
       }
     lds_barrier();
     // every wave issues the next plane's loads and this plane's df loads
@@ -1957,7 +1975,9 @@ __device__ __forceinline__ void tile2d(const Tiled2D& ta, int tile_id, int par) 
   constexpr int NRB = 2 * (S - 1), NTB = 3 * (S - 1);
   constexpr int H = NTB;            // LDS halo per side
   constexpr int LW = TW + 2 * H;
-  __shared__ R row[4][LW];          // 0 vx 1 vy 2 sxy 3 syy
+  // double-buffered by plane parity: plane i writes one buffer while a
+  // slower wave may still read plane i-1's, so one barrier per plane suffices
+  __shared__ R row[2][4][LW];       // 0 vx 1 vy 2 sxy 3 syy
   __shared__ C sfd[NRB][NTB];
   const R* __restrict__ f = reinterpret_cast<const R*>(ta.f[par]);
   R* __restrict__ fn = const_cast<R*>(reinterpret_cast<const R*>(ta.f[par ^ 1]));
@@ -2016,17 +2036,18 @@ __device__ __forceinline__ void tile2d(const Tiled2D& ta, int tile_id, int par) 
   };
   issue(ib);
   for (int i = ib; i < ie; i++) {
+    auto* const rb = row[(i - ib) & 1];  // this plane's LDS row buffer
     const unsigned plb = (unsigned)i * psb;
 #pragma unroll
     for (int q = 0; q < 4; q++) win[q][W - 1] = nwin[q];
     const R own = nown;
-    row[0][H + t] = win[0][r];
-    row[1][H + t] = win[1][r];
-    row[2][H + t] = win[3][r];
-    row[3][H + t] = own;
+    rb[0][H + t] = win[0][r];
+    rb[1][H + t] = win[1][r];
+    rb[2][H + t] = win[3][r];
+    rb[3][H + t] = own;
     if (hal) {
 #pragma unroll
-      for (int q = 0; q < 4; q++) row[q][hpos] = nhal[q];
+      for (int q = 0; q < 4; q++) rb[q][hpos] = nhal[q];
     }
     lds_barrier();
     if (i + 1 < ie) issue(i + 1);
@@ -2057,7 +2078,7 @@ __device__ __forceinline__ void tile2d(const Tiled2D& ta, int tile_id, int par) 
       }
       if (yin) {
 #pragma unroll
-        for (int q = 0; q < 4; q++) dyv[q] = central<C, r>(cd, [&](int n) { return (C)row[q][t + H - r + n]; });
+        for (int q = 0; q < 4; q++) dyv[q] = central<C, r>(cd, [&](int n) { return (C)rb[q][t + H - r + n]; });
       } else {  // one-sided row along y: taps in the LDS row
         const bool left = j - ta.lo[1] < NRB;
         const int rw = left ? j - ta.lo[1] + 1 : ta.hi[1] - j;
@@ -2067,7 +2088,7 @@ __device__ __forceinline__ void tile2d(const Tiled2D& ta, int tile_id, int par) 
 #pragma unroll
           for (int n = 0; n < NTB; n++) {
             const int tj = left ? ta.lo[1] + n : ta.hi[1] - 1 - n;
-            sy += sfd[rw - 1][n] * (C)row[q][tj - jb + H];
+            sy += sfd[rw - 1][n] * (C)rb[q][tj - jb + H];
           }
           dyv[q] = left ? sy : -sy;
         }
@@ -2127,12 +2148,12 @@ __device__ __forceinline__ void tile2d(const Tiled2D& ta, int tile_id, int par) 
         bst<R>(rn, (R)v, colb, q * fb + plb);
       }
     }
-    lds_barrier();  // every wave is done reading this plane's row
 #pragma unroll
     for (int n = 0; n < W - 1; n++)
 #pragma unroll
       for (int q = 0; q < 4; q++) win[q][n] = win[q][n + 1];
   }
+  lds_barrier();  // a persistent workgroup's next tile reuses the buffers
 }
 
 // one block: workgroup b is tile b in XCD-aware order
@@ -2183,7 +2204,7 @@ __device__ __forceinline__ void curv2d_tile(const Tiled2D& ta, const BlockDev& b
   constexpr int NRB = 2 * (S - 1), NTB = 3 * (S - 1);
   constexpr int H = NTB;            // LDS halo per side
   constexpr int LW = TW + 2 * H;
-  __shared__ C row[4][LW];          // vx vy Gx Gy
+  __shared__ C row[2][4][LW];       // vx vy Gx Gy, double-buffered by plane parity
   __shared__ C sfd[NRB][NTB];
   const R* __restrict__ f = reinterpret_cast<const R*>(ta.f[par]);
   R* __restrict__ fn = const_cast<R*>(reinterpret_cast<const R*>(ta.f[par ^ 1]));
@@ -2266,6 +2287,7 @@ __device__ __forceinline__ void curv2d_tile(const Tiled2D& ta, const BlockDev& b
   };
   issue(ib);
   for (int i = ib; i < ie; i++) {
+    auto* const rb = row[(i - ib) & 1];  // this plane's LDS row buffer
     const unsigned plb = (unsigned)i * psb;
     {
       const C J = (C)nlead[5], m0 = (C)nlead[6], m1 = (C)nlead[7];
@@ -2277,16 +2299,16 @@ __device__ __forceinline__ void curv2d_tile(const Tiled2D& ta, const BlockDev& b
     }
     const C sxx = (C)nown[0], sxy = (C)nown[1], syy = (C)nown[2], szz = (C)nown[3];
     const C Ji = (C)nown[4], m00 = (C)nown[5], m01 = (C)nown[6], m10 = (C)nown[7], m11 = (C)nown[8];
-    row[0][H + t] = win[0][r];
-    row[1][H + t] = win[1][r];
-    row[2][H + t] = Ji * (m10 * sxx + m11 * sxy);
-    row[3][H + t] = Ji * (m10 * sxy + m11 * syy);
+    rb[0][H + t] = win[0][r];
+    rb[1][H + t] = win[1][r];
+    rb[2][H + t] = Ji * (m10 * sxx + m11 * sxy);
+    rb[3][H + t] = Ji * (m10 * sxy + m11 * syy);
     if (hal) {
       const C J = (C)nhal[5], m0 = (C)nhal[6], m1 = (C)nhal[7];
-      row[0][hpos] = (C)nhal[0];
-      row[1][hpos] = (C)nhal[1];
-      row[2][hpos] = J * (m0 * (C)nhal[2] + m1 * (C)nhal[3]);
-      row[3][hpos] = J * (m0 * (C)nhal[3] + m1 * (C)nhal[4]);
+      rb[0][hpos] = (C)nhal[0];
+      rb[1][hpos] = (C)nhal[1];
+      rb[2][hpos] = J * (m0 * (C)nhal[2] + m1 * (C)nhal[3]);
+      rb[3][hpos] = J * (m0 * (C)nhal[3] + m1 * (C)nhal[4]);
     }
     lds_barrier();
     if (i + 1 < ie) issue(i + 1);
@@ -2316,7 +2338,7 @@ __device__ __forceinline__ void curv2d_tile(const Tiled2D& ta, const BlockDev& b
       }
       if (yin) {
 #pragma unroll
-        for (int q = 0; q < 4; q++) dyv[q] = central<C, r>(cd, [&](int n) { return row[q][t + H - r + n]; });
+        for (int q = 0; q < 4; q++) dyv[q] = central<C, r>(cd, [&](int n) { return rb[q][t + H - r + n]; });
       } else {  // one-sided row along y: taps in the LDS row
         const bool left = j - ta.lo[1] < NRB;
         const int rw = left ? j - ta.lo[1] + 1 : ta.hi[1] - j;
@@ -2326,7 +2348,7 @@ __device__ __forceinline__ void curv2d_tile(const Tiled2D& ta, const BlockDev& b
 #pragma unroll
           for (int n = 0; n < NTB; n++) {
             const int tj = left ? ta.lo[1] + n : ta.hi[1] - 1 - n;
-            sy += sfd[rw - 1][n] * row[q][tj - jb + H];
+            sy += sfd[rw - 1][n] * rb[q][tj - jb + H];
           }
           dyv[q] = left ? sy : -sy;
         }
@@ -2385,12 +2407,12 @@ __device__ __forceinline__ void curv2d_tile(const Tiled2D& ta, const BlockDev& b
         bst<R>(rn, (R)v, colb, q * fb + plb);
       }
     }
-    lds_barrier();  // every wave is done reading this plane's row
 #pragma unroll
     for (int n = 0; n < W - 1; n++)
 #pragma unroll
       for (int q = 0; q < 4; q++) win[q][n] = win[q][n + 1];
   }
+  lds_barrier();  // a persistent workgroup's next tile reuses the buffers
 }
 
 // one block: workgroup b is tile b in XCD-aware order

@@ -215,6 +215,9 @@ static const void* fault_kernel_for(int mode, int law) {
 // row-kernel variant: 1 = double-buffered LDS tile (FP32; FP64's does not
 // fit the LDS), 0 = single buffer
 static const int rows_var_default = 1;
+// row-tile order: patches of 8 y tiles x 4 x chunks (32 workgroups, one per
+// CU of an XCD)
+static const int rows_grp_y = 8, rows_grp_x = 4;
 
 // row-tile workgroup shapes: 0 = FP32 4 rows (<= 1024 threads, 4 waves per
 // SIMD) or FP64 2 rows (<= 512 threads, 2 waves per SIMD), LDS rows padded to
@@ -275,6 +278,7 @@ static size_t rows_lds(int s, int var, int shape) {
 template <typename R>
 static void launch_rows(const Tiled3D& ta, int s, const R* f, R* fn, R* df, const double* hs, hipStream_t st) {
   const int nz = ta.hi[2] - ta.lo[2];
+  if (ta.grp_y < 1 || ta.grp_x < 1) throw Error{"row tiles: patch shape not set"};
   const int nt = ((rows_ty<R>(ta.rows_shape) * nz + 63) / 64) * 64;
   const size_t lds = rows_lds<R>(s, ta.rows_var, ta.rows_shape);
   const dim3 grid((unsigned)(((ta.ntiles + 7) / 8) * 8));
@@ -539,6 +543,17 @@ Solver<R>::Solver(const std::string& path, const fdf_opts& o) {
           HIPCHK(hipFuncSetAttribute(rows_kernel_for<R>(S.sbporder, ta.rows_nht, var, shape),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
           ta.ntiles = ta.nyt * ta.nxc;
+          // tile patches of grp_y x grp_x: one XCD's worth of workgroups
+          // (FDF_ROWS_GROUP="gy,gx"; "1000,1" is the y-fastest order)
+          ta.grp_y = rows_grp_y;
+          ta.grp_x = rows_grp_x;
+          if (const char* eg = getenv("FDF_ROWS_GROUP")) {
+            int gy = 0, gx = 0;
+            if (sscanf(eg, "%d,%d", &gy, &gx) == 2 && gy > 0 && gx > 0) {
+              ta.grp_y = gy;
+              ta.grp_x = gx;
+            }
+          }
         }
       }
     }
