@@ -390,6 +390,8 @@ def main():
     ap.add_argument("--compare-precision", action="store_true",
                     help="planar3d: FP32 vs FP64 errors after --compare-steps steps")
     ap.add_argument("--compare-steps", type=int, default=1500)
+    ap.add_argument("--no-stream", action="store_true",
+                    help="planar3d: skip the copy / same-shape stream bandwidth references")
     ap.add_argument("--only", action="store_true",
                     help="planar3d: time only --precision (no second precision, no error check)")
     ap.add_argument("--check-steps", type=int, default=200,
@@ -486,13 +488,36 @@ def run_planar(lib, path, pname, args, workload):
     kernel = ("bulk_rows3d" if nrows == nblocks else "bulk_tiled3d") if ntiled == nblocks else "bulk_kernel"
     from fdfault_amd.profiling import lookup_traffic
     tr = lookup_traffic("%s/%s/sbp%d/%s" % (workload, pname, args.sbporder, kernel))
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": tr["traffic"] if tr else None,
+            "traffic_source": tr["summary"] if tr else None, "kernel": kernel,
+            "avg_launch_ms": round(kms, 4), "launches": nlaunch, "bytes_per_point": B,
+            "points_per_launch": int(ppl)}
+    ref = stream_reference(int(ppl), w) if not args.no_stream else None
+    if ref:
+        roof.update(ref)
+        roof["frac_of_stream36"] = round(achieved / ref["stream36_GBs"], 4)
     return {"value": round(npts * nst * args.steps / t / 1e6, 2), "ms_per_step": round(t / args.steps * 1e3, 3),
-            "points": npts, "rk_stages": nst,
-            "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": tr["traffic"] if tr else None,
-                         "traffic_source": tr["summary"] if tr else None, "kernel": kernel,
-                         "avg_launch_ms": round(kms, 4), "launches": nlaunch, "bytes_per_point": B,
-                         "points_per_launch": int(ppl)}}
+            "points": npts, "rk_stages": nst, "roofline": roof}
+
+
+def stream_reference(points, w):
+    """Achievable-bandwidth references measured on this GPU by the copy probe
+    (fdfault_amd/_lib/copy_bw, a separate process): a plain 1 GiB copy, and a
+    stream over the 3-D stage's own array shape (18 arrays read and 18
+    written, `points` words of w bytes each: the algorithmic bytes of one
+    block launch, with no stencil or re-reads); each the best of a small
+    sweep of launch shapes (and, for the copy, non-temporal accesses)."""
+    exe = os.path.join(os.path.dirname(os.path.abspath(__file__)), "fdfault_amd", "_lib", "copy_bw")
+    if not os.path.exists(exe):
+        return None
+    out = {}
+    for key, argv in (("copy_GBs", [exe, "copybest", "1024", "20"]), ("stream36_GBs", [exe, "soa36", str(points * w // 4)])):
+        r = subprocess.run(argv, capture_output=True, text=True, timeout=300)
+        if r.returncode != 0:
+            return None
+        out[key] = float(r.stdout.strip().split("\n")[-1].split()[-2])
+    return out
 
 
 def fp32_error(lib, path, steps):

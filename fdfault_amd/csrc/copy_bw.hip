@@ -5,13 +5,27 @@
 // known exactly (read n + write n bytes).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 
 __global__ void copy_kernel(const double4* __restrict__ a, double4* __restrict__ b, long long n) {
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const long long stride = (long long)gridDim.x * blockDim.x;
   for (; i < n; i += stride) b[i] = a[i];
+}
+
+// the same copy with non-temporal loads and stores
+__global__ void copy_kernel_nt(const double4* __restrict__ a, double4* __restrict__ b, long long n) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) {
+    const double* s = reinterpret_cast<const double*>(a + i);
+    double* d = reinterpret_cast<double*>(b + i);
+#pragma unroll
+    for (int c = 0; c < 4; c++) __builtin_nontemporal_store(__builtin_nontemporal_load(s + c), d + c);
+  }
 }
 
 // SoA stream: `nin` input and `nout` output arrays of n floats (the bulk
@@ -43,15 +57,22 @@ static void soa_run(long long n, int reps) {
   hipEvent_t e0, e1;
   (void)hipEventCreate(&e0);
   (void)hipEventCreate(&e1);
-  hipLaunchKernelGGL((soa_stream<NIN, NOUT>), dim3(256 * 8), dim3(256), 0, 0, (const float* const*)din, dout, n);
-  (void)hipEventRecord(e0, 0);
-  for (int r = 0; r < reps; r++)
-    hipLaunchKernelGGL((soa_stream<NIN, NOUT>), dim3(256 * 8), dim3(256), 0, 0, (const float* const*)din, dout, n);
-  (void)hipEventRecord(e1, 0);
-  (void)hipEventSynchronize(e1);
-  float ms = 0.f;
-  (void)hipEventElapsedTime(&ms, e0, e1);
-  const double per = ms / reps, bytes = (double)(NIN + NOUT) * n * 4;
+  // grid sizes: 8 or 32 workgroups per CU (grid-stride), or one point per thread
+  const long long grids[3] = {256 * 8, 256 * 32, (n + 255) / 256};
+  double per = 1e30;
+  for (long long gb : grids) {
+    const dim3 grid((unsigned)gb);
+    hipLaunchKernelGGL((soa_stream<NIN, NOUT>), grid, dim3(256), 0, 0, (const float* const*)din, dout, n);
+    (void)hipEventRecord(e0, 0);
+    for (int r = 0; r < reps; r++)
+      hipLaunchKernelGGL((soa_stream<NIN, NOUT>), grid, dim3(256), 0, 0, (const float* const*)din, dout, n);
+    (void)hipEventRecord(e1, 0);
+    (void)hipEventSynchronize(e1);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    per = std::min(per, (double)ms / reps);
+  }
+  const double bytes = (double)(NIN + NOUT) * n * 4;
   printf("soa stream %d in + %d out arrays of %lld floats: %.4f ms per launch, %.1f GB/s\n", NIN, NOUT, n, per,
          bytes / (per * 1e-3) / 1e9);
   for (int q = 0; q < NIN; q++) (void)hipFree(hin[q]);
@@ -63,6 +84,11 @@ static void soa_run(long long n, int reps) {
 }
 
 int main(int argc, char** argv) {
+  if (argc > 1 && strcmp(argv[1], "soa36") == 0) {  // "soa36 <n>": the 3-D stage's 18 in + 18 out arrays
+    const long long n = argc > 2 ? atoll(argv[2]) : 16200801;
+    soa_run<18, 18>(n, 10);
+    return 0;
+  }
   if (argc > 1 && argv[1][0] == 's') {  // "soa <npoints>": bulk-stage stream shapes
     const long long n = argc > 2 ? atoll(argv[2]) : 16200801;
     soa_run<2, 1>(n, 10);
@@ -70,6 +96,8 @@ int main(int argc, char** argv) {
     soa_run<18, 18>(n, 10);
     return 0;
   }
+  const bool best = argc > 1 && strcmp(argv[1], "copybest") == 0;  // "copybest <mib>": best of a sweep
+  if (best) { argv++; argc--; }
   const long long mib = argc > 1 ? atoll(argv[1]) : 1024;
   const int reps = argc > 2 ? atoi(argv[2]) : 20;
   const long long bytes = mib << 20;
@@ -92,7 +120,26 @@ int main(int argc, char** argv) {
   (void)hipEventSynchronize(e1);
   float ms = 0.f;
   (void)hipEventElapsedTime(&ms, e0, e1);
-  const double per = ms / reps;
+  double per = ms / reps;
+  if (best) {  // non-temporal or not, 8 or 32 workgroups per CU or one element per thread
+    const long long grids[3] = {256 * 8, 256 * 32, (n + 255) / 256};
+    for (int nt = 0; nt < 2; nt++)
+      for (long long gb : grids) {
+        const dim3 grid((unsigned)gb);
+        if (nt) hipLaunchKernelGGL(copy_kernel_nt, grid, dim3(threads), 0, 0, a, b, n);
+        else hipLaunchKernelGGL(copy_kernel, grid, dim3(threads), 0, 0, a, b, n);
+        (void)hipEventRecord(e0, 0);
+        for (int r = 0; r < reps; r++) {
+          if (nt) hipLaunchKernelGGL(copy_kernel_nt, grid, dim3(threads), 0, 0, a, b, n);
+          else hipLaunchKernelGGL(copy_kernel, grid, dim3(threads), 0, 0, a, b, n);
+        }
+        (void)hipEventRecord(e1, 0);
+        (void)hipEventSynchronize(e1);
+        float m2 = 0.f;
+        (void)hipEventElapsedTime(&m2, e0, e1);
+        per = std::min(per, (double)m2 / reps);
+      }
+  }
   printf("copy %lld MiB: read %lld B + write %lld B per launch, %.4f ms per launch, %.1f GB/s\n", mib, bytes, bytes, per,
          2.0 * bytes / (per * 1e-3) / 1e9);
   (void)hipFree(a);

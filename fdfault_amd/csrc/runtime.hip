@@ -215,9 +215,9 @@ static const void* fault_kernel_for(int mode, int law) {
 // row-kernel variant: 1 = double-buffered LDS tile (FP32; FP64's does not
 // fit the LDS), 0 = single buffer
 static const int rows_var_default = 1;
-// row-tile order: patches of 8 y tiles x 4 x chunks (32 workgroups, one per
-// CU of an XCD)
-static const int rows_grp_y = 8, rows_grp_x = 4;
+// row-tile order: patches of 16 y tiles x 2 x chunks (32 workgroups, one per
+// CU of an XCD; measured best of 8x4, 4x8, 16x2, 8x8, 2x16, 32x1)
+static const int rows_grp_y = 16, rows_grp_x = 2;
 
 // row-tile workgroup shapes: 0 = FP32 4 rows (<= 1024 threads, 4 waves per
 // SIMD) or FP64 2 rows (<= 512 threads, 2 waves per SIMD), LDS rows padded to
