@@ -70,6 +70,7 @@ static void soa_run(long long n, int reps) {
     (void)hipEventSynchronize(e1);
     float ms = 0.f;
     (void)hipEventElapsedTime(&ms, e0, e1);
+    printf("  grid %lld x 256: %.4f ms\n", gb, (double)ms / reps);
     per = std::min(per, (double)ms / reps);
   }
   const double bytes = (double)(NIN + NOUT) * n * 4;
@@ -137,6 +138,7 @@ int main(int argc, char** argv) {
         (void)hipEventSynchronize(e1);
         float m2 = 0.f;
         (void)hipEventElapsedTime(&m2, e0, e1);
+        printf("  %s grid %lld x %d: %.4f ms\n", nt ? "non-temporal" : "plain", gb, threads, (double)m2 / reps);
         per = std::min(per, (double)m2 / reps);
       }
   }
