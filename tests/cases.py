@@ -56,3 +56,38 @@ def curved_surfaces(tmpdir, nx, ny, dx, amp=0.05, ndim=2, nz=1):
             faces.append(pth)
         paths.append(faces)
     return paths
+
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def reference_example(tmpdir: str, name: str, nt: int, datadir: str) -> str:
+    """The reference's example problem `name` (tests/golden/<name>.in, a
+    verbatim copy of problems/<name>.in) with its data directory set to
+    `datadir`, its step count cut to `nt` and each output unit's last step
+    clipped to nt; every other token is unchanged."""
+    lines = open(os.path.join(GOLDEN, name + ".in")).read().split("\n")
+    out, sec, k = [], None, 0
+    for ln in lines:
+        t = ln.strip()
+        if t.startswith("["):
+            sec, k = t, 0
+            out.append(ln)
+            continue
+        if sec == "[fdfault.problem]" and t:
+            k += 1
+            if k == 2:
+                ln = datadir
+            elif k == 3:
+                ln = str(nt)
+        elif sec == "[fdfault.outputlist]" and t:
+            k += 1
+            if k % 6 == 3:  # "tmin tmax tstride" of a unit
+                tm, tM, ts = t.split()
+                ln = "%s %d %s" % (tm, min(int(tM), nt), ts)
+        out.append(ln)
+    os.makedirs(os.path.join(tmpdir, name), exist_ok=True)
+    path = os.path.join(tmpdir, name, name + ".in")
+    open(path, "w").write("\n".join(out))
+    return path
+

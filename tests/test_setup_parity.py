@@ -41,3 +41,11 @@ def test_cartesian_metric_closed_form(tmp_path):
     np.testing.assert_allclose(m[3][:, :101], 1.0 / 2.0, rtol=1e-13)
     np.testing.assert_allclose(m[1], 0.0, atol=1e-13)
     assert (j > 0).all()
+
+
+@pytest.mark.parametrize("name", ["test2d", "tpv4"])
+def test_setup_reference_examples(tmp_path, name):
+    """The reference's own example problems (tests/golden): grid, metric,
+    Jacobian, dt and nt bit-identical to the oracle's."""
+    from cases import reference_example
+    _compare(reference_example(str(tmp_path), name, 10, str(tmp_path) + "/data/"))
