@@ -377,16 +377,27 @@ Solver<R>::Solver(const std::string& path, const fdf_opts& o) {
   // SBP tables
   HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(c_fd), S.fd.c, sizeof(S.fd.c)));
   HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(c_diss), S.fd.d, sizeof(S.fd.d)));
-  // grid arrays in the bulk precision
-  auto cast = [](const std::vector<double>& v) {
-    std::vector<R> out(v.size());
-    for (size_t i = 0; i < v.size(); i++) out[i] = (R)v[i];
-    return out;
+  // grid arrays in the bulk precision: FP64 arrays are copied as they are,
+  // FP32 ones converted on the device (round to nearest, as a host cast)
+  auto upload_bulk = [&](DBuf<R>& dst, const double* h, size_t n) {
+    if constexpr (std::is_same<R, double>::value) {
+      dst.upload(h, n);
+    } else {
+      DBuf<double> tmp;
+      tmp.upload(h, n);
+      dst.alloc(n);
+      if (n) {
+        hipLaunchKernelGGL(convert_from_double<R>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, tmp.p,
+                           dst.p, (long long)n);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(stream));
+      }
+    }
   };
-  metric.upload(cast(S.metric));
-  jac.upload(cast(S.jac));
-  x.upload(S.x);
-  if (S.hetmat) mat.upload(cast(S.mat));
+  upload_bulk(metric, S.metric.data(), S.metric.size());
+  upload_bulk(jac, S.jac.data(), S.jac.size());
+  x.upload(S.x.data(), S.x.size());
+  if (S.hetmat) upload_bulk(mat, S.mat.data(), S.mat.size());
   if (S.hetstress) hs.upload(S.s);
   for (int b = 0; b < 2; b++) {
     f[b].alloc((size_t)S.nf * N);
@@ -460,24 +471,7 @@ Solver<R>::Solver(const std::string& path, const fdf_opts& o) {
                !(getenv("FDF_CURV") && atoi(getenv("FDF_CURV")) == 0);
     for (int d = 0; d < nd && okc; d++) okc = B.mrb[d] > B.mc[d] && B.nx[d] >= 3 * (S.sbporder - 1);
     double mdiag[3] = {0., 0., 0.};
-    if (ok) {
-      // metric diagonal and uniform, Jacobian uniform, to 1e-9 relative (the
-      // SBP derivative of TFI coordinates leaves ~1e-12 rounding noise)
-      const size_t gc = S.idx((B.mlb[0] + B.prb[0]) / 2, (B.mlb[1] + B.prb[1]) / 2, (B.mlb[2] + B.prb[2]) / 2);
-      const double jc = S.jac[gc];
-      for (int d = 0; d < nd; d++) mdiag[d] = S.metric[(size_t)(d * nd + d) * N + gc];
-      for (int i = B.mlb[0]; i < B.prb[0] && ok; i++)
-        for (int j = B.mlb[1]; j < B.prb[1] && ok; j++)
-          for (int k = B.mlb[2]; k < B.prb[2] && ok; k++) {
-            const size_t g = S.idx(i, j, k);
-            if (fabs(S.jac[g] - jc) > 1e-9 * fabs(jc)) ok = false;
-            for (int l = 0; l < nd && ok; l++)
-              for (int m = 0; m < nd && ok; m++) {
-                const double v = S.metric[(size_t)(l * nd + m) * N + g];
-                if (l == m ? fabs(v - mdiag[l]) > 1e-9 * fabs(mdiag[l]) : fabs(v) > 1e-9 * fabs(mdiag[l])) ok = false;
-              }
-          }
-    }
+    if (ok) ok = block_metric_uniform(S, B, mdiag);
     Tiled3D ta{};
     Tiled2D t2{};
     if (ok && ok3) {

@@ -314,9 +314,15 @@ void build_block_grid(SetupH& S, BlockH& B, const SurfView* F) {
   // dX_c / dxi_d along grid lines, one-sided rows near block edges
   const int s = S.fd.s;
   const size_t nb = (size_t)B.nx[0] * B.nx[1] * B.nx[2];
-  std::vector<double> J9(9 * nb, 0.);  // J9[(c*3+d)*nb + local]
+  gvec J9(9 * nb);  // J9[(c*3+d)*nb + local]; entries with c, d < ndim are written below
   for (int c = 0; c < 3; c++)
-    for (size_t q = 0; q < nb; q++) J9[(size_t)(c * 3 + c) * nb + q] = 1.;
+    for (int d = 0; d < 3; d++)
+      if (c >= S.ndim || d >= S.ndim) {
+        const double v = c == d ? 1. : 0.;
+        double* J = J9.data() + (size_t)(c * 3 + d) * nb;
+#pragma omp parallel for schedule(static)
+        for (long long q = 0; q < (long long)nb; q++) J[q] = v;
+      }
   const size_t gstride[3] = {(size_t)S.nx[1] * S.nx[2], (size_t)S.nx[2], 1};
   for (int d = 0; d < S.ndim; d++)
 #pragma omp parallel for schedule(static)
@@ -428,6 +434,28 @@ double min_dx_over_cs(const SetupH& S) {  // block.cpp:368-403, domain.cpp:199-2
 
 }  // namespace
 
+bool block_metric_uniform(const SetupH& S, const BlockH& B, double mdiag[3]) {
+  const int nd = S.ndim;
+  const size_t N = S.N;
+  const size_t gc = S.idx((B.mlb[0] + B.prb[0]) / 2, (B.mlb[1] + B.prb[1]) / 2, (B.mlb[2] + B.prb[2]) / 2);
+  const double jc = S.jac[gc];
+  for (int d = 0; d < nd; d++) mdiag[d] = S.metric[(size_t)(d * nd + d) * N + gc];
+  bool bad = false;
+#pragma omp parallel for schedule(static) reduction(|| : bad)
+  for (int i = B.mlb[0]; i < B.prb[0]; i++)
+    for (int j = B.mlb[1]; j < B.prb[1]; j++)
+      for (int k = B.mlb[2]; k < B.prb[2]; k++) {
+        const size_t g = S.idx(i, j, k);
+        if (fabs(S.jac[g] - jc) > 1e-9 * fabs(jc)) bad = true;
+        for (int l = 0; l < nd; l++)
+          for (int m = 0; m < nd; m++) {
+            const double v = S.metric[(size_t)(l * nd + m) * N + g];
+            if (l == m ? fabs(v - mdiag[l]) > 1e-9 * fabs(mdiag[l]) : fabs(v) > 1e-9 * fabs(mdiag[l])) bad = true;
+          }
+      }
+  return !bad;
+}
+
 void compile_problem(const std::string& path, SetupH& S, bool outputs, bool quiet) {
   Scanner sc(path);
   // [fdfault.problem], problem.cpp:27-44
@@ -481,9 +509,10 @@ void compile_problem(const std::string& path, SetupH& S, bool outputs, bool quie
     if (S.ndim == 2 && S.mode == 2) { S.nf = 6; S.ns = 4; S.sidx[3] = 5; }
   }
   S.N = (size_t)S.nx[0] * S.nx[1] * S.nx[2];
-  S.x.assign((size_t)S.ndim * S.N, 0.);
-  S.metric.assign((size_t)S.ndim * S.ndim * S.N, 0.);
-  S.jac.assign(S.N, 0.);
+  // (blocks tile the grid, so build_block_grid writes every entry)
+  S.x.resize((size_t)S.ndim * S.N);
+  S.metric.resize((size_t)S.ndim * S.ndim * S.N);
+  S.jac.resize(S.N);
   if (loadfile != "none") { S.hetstress = true; S.s = load_f8(loadfile, (size_t)S.ns * S.N, "heterogeneous stress"); }
   if (matfile != "none") { S.hetmat = true; S.mat = load_f8(matfile, (size_t)S.nmat * S.N, "heterogeneous material"); }
   if (S.hetmat && S.ndim == 2 && S.mode == 3)

@@ -12,10 +12,36 @@
 #pragma once
 
 #include <cstdint>
+#include <new>
 #include <string>
+#include <utility>
 #include <vector>
 
 namespace fdf {
+
+// Allocator whose value-initialisation leaves doubles uninitialised: the
+// grid arrays (gigabytes on large grids) are first written by the parallel
+// set-up loops, which then also place their pages, instead of being zeroed
+// by one thread on resize.
+template <typename T>
+struct uninit_alloc : std::allocator<T> {
+  template <typename U>
+  struct rebind {
+    using other = uninit_alloc<U>;
+  };
+  uninit_alloc() = default;
+  template <typename U>
+  uninit_alloc(const uninit_alloc<U>&) noexcept {}
+  template <typename U>
+  void construct(U* p) noexcept {
+    ::new ((void*)p) U;
+  }
+  template <typename U, typename... A>
+  void construct(U* p, A&&... a) {
+    ::new ((void*)p) U(std::forward<A>(a)...);
+  }
+};
+using gvec = std::vector<double, uninit_alloc<double>>;
 
 struct Error {
   std::string msg;
@@ -108,7 +134,8 @@ struct SetupH {
   bool hetstress = false, hetmat = false;
   double cdiss = 0.;
   size_t N = 0;
-  std::vector<double> x, metric, jac, s, mat;
+  gvec x, metric, jac;  // every entry is written by the block set-up
+  std::vector<double> s, mat;
   std::vector<BlockH> blocks;
   std::vector<IfaceH> ifaces;
   std::vector<OutputH> outputs;
@@ -121,5 +148,10 @@ struct SetupH {
 
 // Parse + build; throws Error.  `outputs` parses the output list (else skipped).
 void compile_problem(const std::string& path, SetupH& S, bool outputs, bool quiet);
+
+// Whether block B's metric is diagonal and uniform and its Jacobian uniform,
+// to 1e-9 relative (the SBP derivative of TFI coordinates leaves ~1e-12
+// rounding noise); mdiag receives the diagonal at the block's centre point.
+bool block_metric_uniform(const SetupH& S, const BlockH& B, double mdiag[3]);
 
 }  // namespace fdf
