@@ -353,6 +353,10 @@ def ensemble(args):
                    "max_front_err_steps": worst_front, "front_points_ruptured_in_one_only": unmatched,
                    "reference": "FP64 bit-faithful single runs", "per_instance": per}
         b.close()
+    from fdfault_amd.profiling import lookup_traffic
+    tr = lookup_traffic("ensemble_2d_slipweak_%d_instances/%s/sbp3/bulk_tiled2d_batch_tw%d_cx%d"
+                        % (len(paths), args.precision, tun["tile_width"], tun["chunk"])) \
+        if not tun["persistent"] else None
     line = {
         "metric": "ruptures/hour (batched ensemble of uneven 2-D ruptures)",
         "value": round(rph, 1), "unit": "ruptures/hour", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -365,7 +369,10 @@ def ensemble(args):
                    "tuned_from_cache": tun["from_cache"]},
         "mpt_updates_per_s": round(npts * nst * args.steps / t / 1e6, 1),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": tr["traffic"] if tr else None, "traffic_source": tr["summary"] if tr else None,
+                     "traffic_note": "HBM bytes of one bulk_tiled2d_batch launch over every instance "
+                                     "(the stage's fault kernel adds about 1.6 B per point)" if tr else None,
                      "kernel": "bulk_tiled2d_batch + fault_batch_kernel (whole stage)",
                      "stage_ms": round(stage_ms, 4), "bytes_per_point": B},
         "error_vs_fp64": err,
