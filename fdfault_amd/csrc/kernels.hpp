@@ -1811,10 +1811,20 @@ __global__ void __launch_bounds__(NTMAX) __attribute__((amdgpu_waves_per_eu(WPE)
     // every wave issues the next plane's loads and this plane's df loads
     // before any wave starts the arithmetic (measured: 2.5 % faster FP32)
     __builtin_amdgcn_s_setprio(3);
-    if (i + 1 < ie) issue(i + 1);
     R odf[9];
+    if constexpr (sizeof(R) == 4) {
+      if (i + 1 < ie) issue(i + 1);
 #pragma unroll
-    for (int q = 0; q < 9; q++) odf[q] = bld<R>(rd, colb, q * fb + plb);
+      for (int q = 0; q < 9; q++) odf[q] = bld<R>(rd, colb, q * fb + plb);
+    } else {
+      // FP64: this plane's df first, so that waiting for it does not wait
+      // for the next plane's loads too (vmcnt counts in issue order;
+      // measured 1.243 -> 1.232 ms per block launch, FP32 unchanged)
+#pragma unroll
+      for (int q = 0; q < 9; q++) odf[q] = bld<R>(rd, colb, q * fb + plb);
+      asm volatile("" ::: "memory");
+      if (i + 1 < ie) issue(i + 1);
+    }
     __builtin_amdgcn_s_setprio(0);
     if (active) {
       C dxv[6], dyv[6], dzv[6];
