@@ -490,7 +490,8 @@ def run_planar(lib, path, pname, args, workload):
     ntiled, nrows = g.info.tiled_blocks, g.info.row_blocks
     g.close()
     B = bytes_per_point(9, w)
-    ppl = npts / nblocks
+    # point updates per bulk launch (both blocks may share one launch)
+    ppl = npts * nst * args.steps / nlaunch
     achieved = B * ppl / (kms * 1e-3) / 1e9
     kernel = ("bulk_rows3d" if nrows == nblocks else "bulk_tiled3d") if ntiled == nblocks else "bulk_kernel"
     from fdfault_amd.profiling import lookup_traffic

@@ -152,6 +152,13 @@ class Solver : public SolverBase {
   // 2-D problems whose blocks all take the same tiled kernel: one launch per
   // stage over every block's tiles (merged 1: rectilinear, 2: curvilinear)
   int merged = 0, cur_stage = 0, mnwork = 0;
+  // 3-D problems of two blocks on the tiled kernels: the second block's
+  // launch runs on `side`, forked after the stage's fault solve and joined
+  // before the next stage, so its workgroups fill the CUs that the first
+  // block's tail leaves idle
+  bool pair3 = false;
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   std::vector<DBuf<Tiled2D>> mdesc;  // [stage][roundtrip] per-block descriptors
   DBuf<BlockDev> mbd;
   DBuf<int2> mwork;                  // (block, tile)
@@ -840,6 +847,23 @@ Solver<R>::Solver(const std::string& path, const fdf_opts& o) {
       if (merged == 2) mbd.upload(bdev);
     }
   }
+  {
+    // by default only for blocks of fewer than 8 rounds of one workgroup per
+    // CU, where a launch's tail is a sizeable part of it (FDF_PAIR3D: 0 off,
+    // 1 always)
+    const char* ep = getenv("FDF_PAIR3D");
+    int dev = 0, ncu = 0;
+    HIPCHK(hipGetDevice(&dev));
+    HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    const bool two = nd == 3 && S.blocks.size() == 2 && tiled_ok[0] && tiled_ok[1];
+    const bool small = two && tiles[0].ntiles < 8 * ncu && tiles[1].ntiles < 8 * ncu;
+    if (two && (ep ? atoi(ep) != 0 : small)) {
+      pair3 = true;
+      HIPCHK(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+      HIPCHK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+      HIPCHK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
+    }
+  }
   for (size_t b = 0; b < tiled_ok.size(); b++) {
     ncurv += curv_ok[b] ? 1 : 0;
     ntiled += tiled_ok[b] ? 1 : 0;
@@ -936,10 +960,14 @@ Solver<R>::~Solver() {
     if (o.time) fclose(o.time);
   }
   if (stream) (void)hipStreamSynchronize(stream);
+  if (side) (void)hipStreamSynchronize(side);
   for (int e = 0; e < 2 * NEV; e++)
     if (evs[e]) (void)hipEventDestroy(evs[e]);
   for (int q = 0; q < 2; q++)
     if (graphs[q]) (void)hipGraphExecDestroy(graphs[q]);
+  if (ev_fork) (void)hipEventDestroy(ev_fork);
+  if (ev_join) (void)hipEventDestroy(ev_join);
+  if (side) (void)hipStreamDestroy(side);
   if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -1207,18 +1235,29 @@ void Solver<R>::do_stage(int istep, int st, bool full_step) {
   }
   StageDev sd{S.rk.A[st], S.rk.B[st], S.dt, (full_step && last && !S.plastic && sizeof(R) == 8) ? 1 : 0};
   cur_stage = st;
+  if (pair3) {  // the second block on the side stream, after the fault solve
+    HIPCHK(hipEventRecord(ev_fork, stream));
+    HIPCHK(hipStreamWaitEvent(side, ev_fork, 0));
+  }
   for (size_t b = 0; b < S.blocks.size(); b++) {
     if (merged && b > 0) break;  // block 0's launch covers every block
+    const hipStream_t main = stream;
+    if (pair3 && b == 1) stream = side;  // launch_bulk issues on `stream`
     if (capturing) {
       launch_bulk((int)b, sd);
-      continue;
+    } else {
+      if (nev == NEV) flush_events();
+      HIPCHK(hipEventRecord(evs[2 * nev], stream));
+      evkind[nev] = 0;
+      launch_bulk((int)b, sd);
+      HIPCHK(hipEventRecord(evs[2 * nev + 1], stream));
+      nev++;
     }
-    if (nev == NEV) flush_events();
-    HIPCHK(hipEventRecord(evs[2 * nev], stream));
-    evkind[nev] = 0;
-    launch_bulk((int)b, sd);
-    HIPCHK(hipEventRecord(evs[2 * nev + 1], stream));
-    nev++;
+    stream = main;
+  }
+  if (pair3) {
+    HIPCHK(hipEventRecord(ev_join, side));
+    HIPCHK(hipStreamWaitEvent(stream, ev_join, 0));
   }
   cur ^= 1;
   if (last && S.plastic) {
