@@ -1,6 +1,6 @@
 """FP32-bulk / FP64-fault path against the FP64 oracle.
 
-Bar (north star): final slip within 1e-4 relative (max-norm over the fault)
+Accuracy bar: final slip within 1e-4 relative (max-norm over the fault)
 and rupture-front times within one time step, with every result reporting
 its error against the FP64 oracle."""
 import numpy as np
