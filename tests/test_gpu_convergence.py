@@ -55,3 +55,17 @@ def test_gpu_boundary_order_2d(tmp_path):
     p = PW.order(*e)
     print("2-D boundary sbporder 3 errors", e, "order %.2f" % p, flush=True)
     assert p >= 2.5
+
+
+@pytest.mark.parametrize("s,ndim", [(3, 2), (4, 2), (3, 3)])
+def test_gpu_interior_order_varying_lambda(tmp_path, s, ndim):
+    """Per-point material (lambda varying in x and y, uniform rho and G; the
+    generic kernel's per-point branch) keeps the interior order of the plane
+    S wave."""
+    kw = dict(ndim=3, Lz=9.0) if ndim == 3 else {}
+    dxs = (0.2, 0.1) if ndim == 3 else (0.1, 0.05)
+    e = [PW.error(_open(lib.FP64), str(tmp_path / ("g%d" % k)), dx, s, T=0.25, Lx=6.0, Ly=6.0, lam_var=0.5, **kw)
+         for k, dx in enumerate(dxs)]
+    p = PW.order(*e)
+    print("%d-D varying lambda sbporder" % ndim, s, "errors", e, "order %.2f" % p, flush=True)
+    assert p >= 2 * (s - 1) - 0.3

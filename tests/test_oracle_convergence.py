@@ -35,3 +35,22 @@ def test_boundary_order(tmp_path, s):
     p = PW.order(*e)
     print("sbporder", s, "errors", e, "order %.2f" % p)
     assert p >= s - 0.5
+
+
+@pytest.mark.parametrize("s", [2, 3, 4])
+def test_interior_order_varying_lambda(tmp_path, s):
+    """Per-point material (lambda varying in x and y, uniform rho and G): the
+    plane S wave is unchanged, and the per-point branch keeps the order."""
+    e = [PW.error(O.Oracle, str(tmp_path / ("g%d" % k)), dx, s, T=0.25, Lx=6.0, Ly=6.0, lam_var=0.5)
+         for k, dx in enumerate((0.1, 0.05))]
+    p = PW.order(*e)
+    print("sbporder", s, "varying lambda, errors", e, "order %.2f" % p)
+    assert p >= 2 * (s - 1) - 0.3
+
+
+def test_boundary_order_varying_lambda(tmp_path):
+    e = [PW.error(O.Oracle, str(tmp_path / ("g%d" % k)), dx, 3, T=0.95, Lx=6.0, Ly=10.0, lam_var=0.5)
+         for k, dx in enumerate((0.1, 0.05))]
+    p = PW.order(*e)
+    print("sbporder 3 varying lambda, boundary errors", e, "order %.2f" % p)
+    assert p >= 2.5
