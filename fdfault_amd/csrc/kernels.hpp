@@ -81,9 +81,18 @@ struct Boxes {
   long long start[7];
 };
 
+// Which of the stage's df/f traffic a bulk launch moves (bits): DF_LOAD reads
+// df and scales it by A (fields::scale_df, fields.cpp:293-300; skipped where
+// A = 0, the first stage of every low-storage scheme of rk.cpp:42-91),
+// DF_STORE writes the new df (skipped on the last stage of a full time step,
+// whose df only the next step's first stage would read, times A = 0), F_STORE
+// writes the updated f (off when a skipped df is recomputed for a reader).
+enum { DF_LOAD = 1, DF_STORE = 2, F_STORE = 4, DF_ALL = 7 };
+
 struct StageDev {
   double A, B, dt;
   int roundtrip;  // end of an elastic step: the set_stress/remove_stress rounding of problem.cpp:160-164
+  int dfmode = DF_ALL;
 };
 
 // rotated boundary fields, boundary.cpp:505-568
@@ -377,9 +386,14 @@ __global__ void __launch_bounds__(256) bulk_kernel(BlockDev bd, Boxes bx, GridDe
   }
 
   C acc[NFE + 1];
+  if (st.dfmode & DF_LOAD) {
 #pragma unroll
-  for (int q = 0; q < NFE; q++) acc[q] = (C)df[q * N + id] * (C)st.A;
-  acc[NFE] = (NF > NFE) ? (C)df[NFE * N + id] * (C)st.A : (C)0;
+    for (int q = 0; q < NFE; q++) acc[q] = (C)df[q * N + id] * (C)st.A;
+    acc[NFE] = (NF > NFE) ? (C)df[NFE * N + id] * (C)st.A : (C)0;
+  } else {
+#pragma unroll
+    for (int q = 0; q <= NFE; q++) acc[q] = (C)0;
+  }
   const C J1 = (C)jac[id];
 
 #pragma unroll
@@ -451,9 +465,12 @@ __global__ void __launch_bounds__(256) bulk_kernel(BlockDev bd, Boxes bx, GridDe
       if (p < NFE) acc[p] = v; else acc[NFE] = v;
     }
   }
+  if (st.dfmode & DF_STORE) {
 #pragma unroll
-  for (int q = 0; q < NFE; q++) df[q * N + id] = (R)acc[q];
-  if (NF > NFE) df[NFE * N + id] = (R)acc[NFE];
+    for (int q = 0; q < NFE; q++) df[q * N + id] = (R)acc[q];
+    if (NF > NFE) df[NFE * N + id] = (R)acc[NFE];
+  }
+  if (!(st.dfmode & F_STORE)) return;
   // RK update into the other f buffer, plus the per-step stress round trip
   C upd[NFE + 1];
 #pragma unroll
@@ -1221,9 +1238,10 @@ namespace fdf {
 //   * y and z derivatives from an LDS tile of the current plane with an r-point
 //     halo (8 fields: all but sxx);
 //   * one-sided SBP rows near block edges read their taps directly (cached);
-//   * df = A*df + ..., f_new = f + B*df, written once.
-// External and interface SAT terms of face points are added afterwards by
-// face_sat_kernel.  Not bit-identical to the reference (derivative sums are
+//   * df = A*df + ..., f_new = f + B*df, written once;
+//   * external SAT as the face's 16 non-zero linear-map entries, interface
+//     SAT terms from the fault kernel's buffer.
+// Not bit-identical to the reference (derivative sums are
 // formed before scaling, the metric is the block constant); the FP64 result
 // stays within 1e-10 of the bit-faithful path over long runs.
 // Workgroup barrier that orders LDS only: __syncthreads() also waits for every
@@ -1245,6 +1263,7 @@ struct Tiled3D {
   double A, B;
   float fcv[3], fcg2[3], fcl[3], fcg[3], fA, fB;  // the same, rounded once for the FP32 path
   int roundtrip, hetstress;
+  int dfmode = DF_ALL;           // DF_LOAD / DF_STORE / F_STORE bits of this launch
   double s0[6];
   int rows_nht;                  // > 0: row-tile kernel with this many halo entries per thread
   int rows_var;                  // row-tile variant: 1 double-buffered LDS tile, 0 single
@@ -1508,8 +1527,13 @@ __global__ void __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(WP
     // df of this plane is first needed at the end of the update: loading it
     // here rather than a plane ahead keeps it out of the loop-carried state
     R odf[9];
+    if (ta.dfmode & DF_LOAD) {
 #pragma unroll
-    for (int q = 0; q < 9; q++) odf[q] = bld<R>(rd, colb, q * fb + plb);
+      for (int q = 0; q < 9; q++) odf[q] = bld<R>(rd, colb, q * fb + plb);
+    } else {
+#pragma unroll
+      for (int q = 0; q < 9; q++) odf[q] = (R)0;
+    }
     if (active) {
       C dxv[6], dyv[6], dzv[6];
       const int jw = __builtin_amdgcn_readfirstlane(j);  // a wave is one y row
@@ -1622,7 +1646,7 @@ __global__ void __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(WP
       }
 #pragma unroll
       for (int q = 0; q < 9; q++) {
-        bst<R>(rd, (R)acc[q], colb, q * fb + plb);
+        if (ta.dfmode & DF_STORE) bst<R>(rd, (R)acc[q], colb, q * fb + plb);
         C v = fo[q] + kB * acc[q];
         if (q >= 3 && ta.roundtrip) {  // set_stress / remove_stress rounding (FP64 only)
           double w = (double)v;
@@ -1632,7 +1656,7 @@ __global__ void __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(WP
           if (ta.hetstress) w -= hs[(q - 3) * N + id];
           v = (C)w;
         }
-        bst<R>(rn, (R)v, colb, q * fb + plb);
+        if (ta.dfmode & F_STORE) bst<R>(rn, (R)v, colb, q * fb + plb);
       }
     }
     lds_barrier();  // every wave is done reading this plane's tile
@@ -1812,16 +1836,17 @@ __global__ void __launch_bounds__(NTMAX) __attribute__((amdgpu_waves_per_eu(WPE)
     // before any wave starts the arithmetic (measured: 2.5 % faster FP32)
     __builtin_amdgcn_s_setprio(3);
     R odf[9];
+    const bool ldf = ta.dfmode & DF_LOAD;
     if constexpr (sizeof(R) == 4) {
       if (i + 1 < ie) issue(i + 1);
 #pragma unroll
-      for (int q = 0; q < 9; q++) odf[q] = bld<R>(rd, colb, q * fb + plb);
+      for (int q = 0; q < 9; q++) odf[q] = ldf ? bld<R>(rd, colb, q * fb + plb) : (R)0;
     } else {
       // FP64: this plane's df first, so that waiting for it does not wait
       // for the next plane's loads too (vmcnt counts in issue order;
       // measured 1.243 -> 1.232 ms per block launch, FP32 unchanged)
 #pragma unroll
-      for (int q = 0; q < 9; q++) odf[q] = bld<R>(rd, colb, q * fb + plb);
+      for (int q = 0; q < 9; q++) odf[q] = ldf ? bld<R>(rd, colb, q * fb + plb) : (R)0;
       asm volatile("" ::: "memory");
       if (i + 1 < ie) issue(i + 1);
     }
@@ -1918,7 +1943,7 @@ __global__ void __launch_bounds__(NTMAX) __attribute__((amdgpu_waves_per_eu(WPE)
       }
 #pragma unroll
       for (int q = 0; q < 9; q++) {
-        bst<R>(rd, (R)acc[q], colb, q * fb + plb);
+        if (ta.dfmode & DF_STORE) bst<R>(rd, (R)acc[q], colb, q * fb + plb);
         C v = fo[q] + kB * acc[q];
         if (q >= 3 && ta.roundtrip) {
           double w = (double)v;
@@ -1928,7 +1953,7 @@ __global__ void __launch_bounds__(NTMAX) __attribute__((amdgpu_waves_per_eu(WPE)
           if (ta.hetstress) w -= hs[(q - 3) * N + id];
           v = (C)w;
         }
-        bst<R>(rn, (R)v, colb, q * fb + plb);
+        if (ta.dfmode & F_STORE) bst<R>(rn, (R)v, colb, q * fb + plb);
       }
     }
     if constexpr (NBUF == 1) lds_barrier();  // every wave is done reading this plane's tile
@@ -1965,6 +1990,7 @@ struct Tiled2D {
   double cv[2], cg2[2], cl[2], cg[2], A, B;
   float fcv[2], fcg2[2], fcl[2], fcg[2], fA, fB;
   int roundtrip, hetstress;
+  int dfmode = DF_ALL;  // DF_LOAD / DF_STORE / F_STORE bits (single-block launch)
   double s0[3];       // sxx sxy syy
   int face_active[4];
   const void* fcoef;  // [4 faces][10] external SAT entries in the bulk precision
@@ -1979,7 +2005,7 @@ struct Tiled2D {
 };
 
 template <typename R, typename C, int S, int TW>
-__device__ __forceinline__ void tile2d(const Tiled2D& ta, int tile_id, int par) {
+__device__ __forceinline__ void tile2d(const Tiled2D& ta, int tile_id, int par, int dfm) {
 #pragma clang fp contract(fast)
   constexpr int r = S - 1, W = 2 * r + 1;
   constexpr int NRB = 2 * (S - 1), NTB = 3 * (S - 1);
@@ -2063,7 +2089,7 @@ __device__ __forceinline__ void tile2d(const Tiled2D& ta, int tile_id, int par) 
     if (i + 1 < ie) issue(i + 1);
     R odf[5];
 #pragma unroll
-    for (int q = 0; q < 5; q++) odf[q] = bld<R>(rd, colb, q * fb + plb);
+    for (int q = 0; q < 5; q++) odf[q] = (dfm & DF_LOAD) ? bld<R>(rd, colb, q * fb + plb) : (R)0;
     if (active) {
       C dxv[4], dyv[4];  // d/dx of vx vy sxx sxy; d/dy of vx vy sxy syy
       if (i >= ta.mc[0] && i < ta.mrb[0]) {
@@ -2144,7 +2170,7 @@ __device__ __forceinline__ void tile2d(const Tiled2D& ta, int tile_id, int par) 
       constexpr int s0i[5] = {-1, -1, 0, 1, 2};
 #pragma unroll
       for (int q = 0; q < 5; q++) {
-        bst<R>(rd, (R)acc[q], colb, q * fb + plb);
+        if (dfm & DF_STORE) bst<R>(rd, (R)acc[q], colb, q * fb + plb);
         C v = fo[q] + kB * acc[q];
         if (q >= 2 && ta.roundtrip) {  // set_stress / remove_stress rounding (FP64 only)
           const long long id = (long long)i * ta.Ny + j;
@@ -2155,7 +2181,7 @@ __device__ __forceinline__ void tile2d(const Tiled2D& ta, int tile_id, int par) 
           if (ta.hetstress) w -= ta.hs[(q - 2) * N + id];
           v = (C)w;
         }
-        bst<R>(rn, (R)v, colb, q * fb + plb);
+        if (dfm & F_STORE) bst<R>(rn, (R)v, colb, q * fb + plb);
       }
     }
 #pragma unroll
@@ -2173,7 +2199,7 @@ __global__ void __launch_bounds__(TW) bulk_tiled2d(Tiled2D ta, int par) {
   const int per = (ta.ntiles + nxcd - 1) / nxcd;
   const int tile_id = (blockIdx.x % nxcd) * per + blockIdx.x / nxcd;
   if (tile_id >= ta.ntiles) return;
-  tile2d<R, C, S, TW>(ta, tile_id, par);
+  tile2d<R, C, S, TW>(ta, tile_id, par, ta.dfmode);
 }
 
 // ensemble: a persistent grid walks the work list (block descriptor, tile)
@@ -2183,11 +2209,12 @@ __global__ void __launch_bounds__(TW) bulk_tiled2d(Tiled2D ta, int par) {
 // run of them, neighbouring y tiles sharing that XCD's L2)
 template <typename R, typename C, int S, int TW>
 __global__ void __launch_bounds__(TW) bulk_tiled2d_batch(const Tiled2D* __restrict__ descs,
-                                                        const int2* __restrict__ work, int nwork, int par) {
+                                                        const int2* __restrict__ work, int nwork, int par,
+                                                        int dfm) {
   const int first = (blockIdx.x % 8) * (gridDim.x / 8) + blockIdx.x / 8;
   for (int w = first; w < nwork; w += gridDim.x) {
     const int2 it = work[w];
-    tile2d<R, C, S, TW>(descs[it.x], it.y, par);
+    tile2d<R, C, S, TW>(descs[it.x], it.y, par, dfm);
   }
 }
 
@@ -2207,7 +2234,7 @@ __global__ void __launch_bounds__(TW) bulk_tiled2d_batch(const Tiled2D* __restri
 // at that point (ta.smap, built by face_map_kernel from ext_sat).
 template <typename R, typename C, int S, int TW, bool PL>
 __device__ __forceinline__ void curv2d_tile(const Tiled2D& ta, const BlockDev& bd, const R* __restrict__ mt,
-                                            const R* __restrict__ jac, int tile_id, int par) {
+                                            const R* __restrict__ jac, int tile_id, int par, int dfm) {
 #pragma clang fp contract(fast)
   constexpr int NF = PL ? 6 : 5;
   constexpr int r = S - 1, W = 2 * r + 1;
@@ -2324,7 +2351,7 @@ __device__ __forceinline__ void curv2d_tile(const Tiled2D& ta, const BlockDev& b
     if (i + 1 < ie) issue(i + 1);
     R odf[5];  // (the szz rate is assigned, so its old value is not read)
 #pragma unroll
-    for (int q = 0; q < 5; q++) odf[q] = bld<R>(rd, colb, q * fb + plb);
+    for (int q = 0; q < 5; q++) odf[q] = (dfm & DF_LOAD) ? bld<R>(rd, colb, q * fb + plb) : (R)0;
     if (active) {
       C dxv[4], dyv[4];  // d/dx of vx vy Fx Fy; d/dy of vx vy Gx Gy
       if (i >= ta.mc[0] && i < ta.mrb[0]) {
@@ -2404,7 +2431,7 @@ __device__ __forceinline__ void curv2d_tile(const Tiled2D& ta, const BlockDev& b
       constexpr int s0i[5] = {-1, -1, 0, 1, 2};
 #pragma unroll
       for (int q = 0; q < NF; q++) {
-        bst<R>(rd, (R)acc[q], colb, q * fb + plb);
+        if (dfm & DF_STORE) bst<R>(rd, (R)acc[q], colb, q * fb + plb);
         C v = fo[q] + kB * acc[q];
         if (!PL && q >= 2 && ta.roundtrip) {  // set_stress / remove_stress rounding (FP64 elastic only)
           double w = (double)v;
@@ -2414,7 +2441,7 @@ __device__ __forceinline__ void curv2d_tile(const Tiled2D& ta, const BlockDev& b
           if (ta.hetstress) w -= ta.hs[(q - 2) * N + id];
           v = (C)w;
         }
-        bst<R>(rn, (R)v, colb, q * fb + plb);
+        if (dfm & F_STORE) bst<R>(rn, (R)v, colb, q * fb + plb);
       }
     }
 #pragma unroll
@@ -2433,7 +2460,7 @@ __global__ void __launch_bounds__(TW) bulk_curv2d(Tiled2D ta, BlockDev bd, const
   const int per = (ta.ntiles + nxcd - 1) / nxcd;
   const int tile_id = (blockIdx.x % nxcd) * per + blockIdx.x / nxcd;
   if (tile_id >= ta.ntiles) return;
-  curv2d_tile<R, C, S, TW, PL>(ta, bd, mt, jac, tile_id, par);
+  curv2d_tile<R, C, S, TW, PL>(ta, bd, mt, jac, tile_id, par, ta.dfmode);
 }
 
 // every block of a problem in one launch: work item (block, tile), one
@@ -2443,11 +2470,11 @@ template <typename R, typename C, int S, int TW, bool PL>
 __global__ void __launch_bounds__(TW) bulk_curv2d_multi(const Tiled2D* __restrict__ descs,
                                                         const BlockDev* __restrict__ bds, const int2* __restrict__ work,
                                                         int nwork, const R* __restrict__ mt, const R* __restrict__ jac,
-                                                        int par) {
+                                                        int par, int dfm) {
   const int w = (blockIdx.x % 8) * (gridDim.x / 8) + blockIdx.x / 8;
   if (w >= nwork) return;
   const int2 it = work[w];
-  curv2d_tile<R, C, S, TW, PL>(descs[it.x], bds[it.x], mt, jac, it.y, par);
+  curv2d_tile<R, C, S, TW, PL>(descs[it.x], bds[it.x], mt, jac, it.y, par, dfm);
 }
 
 // ensemble fault update: work item = (instance, first fault point of a
@@ -2476,10 +2503,6 @@ __global__ void __launch_bounds__(256) fault_batch_kernel(const FaultBatchDev* _
                         err, nullptr, p, nullptr);
   }
 }
-
-struct FaceList {
-  int start[7];  // prefix offsets of the face point counts, face order
-};
 
 // Linear map of the external SAT on one face point: column q is ext_sat
 // applied to the unit state e_q (9 fields), so M . f reproduces the penalty
@@ -2514,80 +2537,6 @@ __global__ void face_map_kernel(FaceDev F, GridDev g, double dt, int n0, int nr,
   double acc[10] = {0., 0., 0., 0., 0., 0., 0., 0., 0., 0.};
   ext_sat<M, double, double>(acc, F, unit, nullptr, gg, q, u, 0, dt);
   for (int p = 0; p < nr; p++) out[((long long)u * nr + p) * NC + q] = acc[p];
-}
-
-// SAT terms of the face points of a block whose bulk update came from
-// bulk_tiled3d: external boundaries in face order, then interface terms in
-// interface order (the reference's order per point), added to df and, times
-// B, to the new f.  A point on several faces is owned by its first face.
-template <int M, typename R, typename C>
-__global__ void __launch_bounds__(256) face_sat_kernel(BlockDev bd, GridDev g, StageDev st, const R* __restrict__ f,
-                                                       R* __restrict__ fn, R* __restrict__ df, const R* __restrict__ mat,
-                                                       const double* __restrict__ hs, FaceList fl) {
-  constexpr int ND = (M == 2) ? 3 : 2;
-  constexpr int NFE = Nf<M>::v;
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= fl.start[2 * ND]) return;
-  int fc = 0;
-  while (t >= fl.start[fc + 1]) fc++;
-  const int loc = t - fl.start[fc];
-  const int d = fc / 2, a = (d == 0) ? 1 : 0, b = (d == 2) ? 1 : 2;
-  int c3[3];
-  c3[d] = (fc & 1) ? bd.hi[d] - 1 : bd.lo[d];
-  c3[a] = bd.lo[a] + loc / bd.n[b];
-  c3[b] = bd.lo[b] + loc % bd.n[b];
-  // owned by the first face (in face order) that contains the point
-  for (int e = 0; e < fc; e++) {
-    const int de = e / 2;
-    const int edge = (e & 1) ? bd.hi[de] - 1 : bd.lo[de];
-    if (c3[de] == edge) return;
-  }
-  const long long N = g.N;
-  const long long id = ((long long)c3[0] * g.Ny + c3[1]) * g.Nz + c3[2];
-  C acc[NFE + 1];
-#pragma unroll
-  for (int q = 0; q <= NFE; q++) acc[q] = (C)0;
-  bool any = false;
-#pragma unroll
-  for (int e = 0; e < 2 * ND; e++) {
-    const FaceDev& F = bd.face[e];
-    if (!F.active) continue;
-    const int de = e / 2;
-    const int edge = (e & 1) ? bd.hi[de] - 1 : bd.lo[de];
-    if (c3[de] != edge) continue;
-    const int ae = (de == 0) ? 1 : 0, be = (de == 2) ? 1 : 2;
-    ext_sat<M, R, C>(acc, F, f, mat, g, id, c3[ae] - bd.lo[ae], c3[be] - bd.lo[be], st.dt);
-    any = true;
-  }
-  const int NF = g.nf;
-  for (int q = 0; q < bd.nif; q++) {
-    const IfFaceDev& I = bd.ifc[q];
-    if (c3[I.dir] != I.fixed) continue;
-    const long long o = (long long)(c3[I.a] - I.org_a) * I.n1 + (c3[I.b] - I.org_b);
-    const double* sn = I.sat + (long long)(I.side * 2 + 0) * NF * I.n;
-    const double* sv = I.sat + (long long)(I.side * 2 + 1) * NF * I.n;
-    for (int p = 0; p < NFE; p++) {
-      acc[p] = (C)((double)acc[p] - sn[(long long)p * I.n + o]);
-      acc[p] = (C)((double)acc[p] - sv[(long long)p * I.n + o]);
-    }
-    any = true;
-  }
-  if (!any) return;
-#pragma unroll
-  for (int q = 0; q < NFE; q++) {
-    df[q * N + id] = (R)((C)df[q * N + id] + acc[q]);
-    C v = (C)fn[q * N + id] + (C)st.B * acc[q];
-    if (st.roundtrip && q >= g.nv) {
-      const int i = q - g.nv;
-      double w = (double)v;
-      w += g.s0[g.sidx[i]];
-      if (g.hetstress) w += hs[i * N + id];
-      w -= g.s0[g.sidx[i]];
-      if (g.hetstress) w -= hs[i * N + id];
-      v = (C)w;
-    }
-    fn[q * N + id] = (R)v;
-  }
 }
 
 }  // namespace fdf

@@ -122,6 +122,15 @@ class Solver : public SolverBase {
  private:
   int mode_id() const { return S.ndim == 3 ? 2 : (S.mode == 2 ? 0 : 1); }
   void do_stage(int istep, int st, bool full_step);
+  // The last stage of a full time step leaves df unwritten when the next
+  // step's first stage scales it by A = 0 (every scheme of rk.cpp:42-91):
+  // df_stale marks that, and materialize_df() recomputes it, bit-identically,
+  // for a caller that reads df (the last stage's input f is still in the
+  // other f buffer, df holds the previous stage's, and the interface SAT
+  // buffers hold the last stage's terms until the next stage).
+  bool df_stale = false;
+  int stage_dfmode(int st, bool full_step) const;
+  void materialize_df();
   void launch_fault(int i, const FaultStageDev& fs);
   void launch_bulk(int b, const StageDev& sd);
   void launch_tiled(Tiled3D ta, int ty);
@@ -1082,27 +1091,29 @@ static void launch_curv2_tw(const Tiled2D& t2, const BlockDev& D, const R* mt, c
 
 // merged launches over every block of a 2-D problem, one workgroup per tile
 template <typename R, int TW>
-static void launch_merged_tw(const Tiled2D* d, const int2* wk, int nw, int s, int par, dim3 grid, hipStream_t st) {
-  if (s == 2) hipLaunchKernelGGL((bulk_tiled2d_batch<R, R, 2, TW>), grid, dim3(TW), 0, st, d, wk, nw, par);
-  else if (s == 3) hipLaunchKernelGGL((bulk_tiled2d_batch<R, R, 3, TW>), grid, dim3(TW), 0, st, d, wk, nw, par);
-  else hipLaunchKernelGGL((bulk_tiled2d_batch<R, R, 4, TW>), grid, dim3(TW), 0, st, d, wk, nw, par);
+static void launch_merged_tw(const Tiled2D* d, const int2* wk, int nw, int s, int par, int dfm, dim3 grid,
+                             hipStream_t st) {
+  if (s == 2) hipLaunchKernelGGL((bulk_tiled2d_batch<R, R, 2, TW>), grid, dim3(TW), 0, st, d, wk, nw, par, dfm);
+  else if (s == 3) hipLaunchKernelGGL((bulk_tiled2d_batch<R, R, 3, TW>), grid, dim3(TW), 0, st, d, wk, nw, par, dfm);
+  else hipLaunchKernelGGL((bulk_tiled2d_batch<R, R, 4, TW>), grid, dim3(TW), 0, st, d, wk, nw, par, dfm);
 }
 
 template <typename R, int TW, bool PL>
 static void launch_mcurv_pl(const Tiled2D* d, const BlockDev* bds, const int2* wk, int nw, const R* mt, const R* jac,
-                            int s, int par, dim3 grid, hipStream_t st) {
+                            int s, int par, int dfm, dim3 grid, hipStream_t st) {
   if (s == 2)
-    hipLaunchKernelGGL((bulk_curv2d_multi<R, R, 2, TW, PL>), grid, dim3(TW), 0, st, d, bds, wk, nw, mt, jac, par);
+    hipLaunchKernelGGL((bulk_curv2d_multi<R, R, 2, TW, PL>), grid, dim3(TW), 0, st, d, bds, wk, nw, mt, jac, par, dfm);
   else if (s == 3)
-    hipLaunchKernelGGL((bulk_curv2d_multi<R, R, 3, TW, PL>), grid, dim3(TW), 0, st, d, bds, wk, nw, mt, jac, par);
-  else hipLaunchKernelGGL((bulk_curv2d_multi<R, R, 4, TW, PL>), grid, dim3(TW), 0, st, d, bds, wk, nw, mt, jac, par);
+    hipLaunchKernelGGL((bulk_curv2d_multi<R, R, 3, TW, PL>), grid, dim3(TW), 0, st, d, bds, wk, nw, mt, jac, par, dfm);
+  else
+    hipLaunchKernelGGL((bulk_curv2d_multi<R, R, 4, TW, PL>), grid, dim3(TW), 0, st, d, bds, wk, nw, mt, jac, par, dfm);
 }
 
 template <typename R, int TW>
 static void launch_mcurv_tw(const Tiled2D* d, const BlockDev* bds, const int2* wk, int nw, const R* mt, const R* jac,
-                            int s, bool plastic, int par, dim3 grid, hipStream_t st) {
-  if (plastic) launch_mcurv_pl<R, TW, true>(d, bds, wk, nw, mt, jac, s, par, grid, st);
-  else launch_mcurv_pl<R, TW, false>(d, bds, wk, nw, mt, jac, s, par, grid, st);
+                            int s, bool plastic, int par, int dfm, dim3 grid, hipStream_t st) {
+  if (plastic) launch_mcurv_pl<R, TW, true>(d, bds, wk, nw, mt, jac, s, par, dfm, grid, st);
+  else launch_mcurv_pl<R, TW, false>(d, bds, wk, nw, mt, jac, s, par, dfm, grid, st);
 }
 
 // stage descriptor of a 2-D rectilinear block: RK coefficients and arrays
@@ -1114,6 +1125,7 @@ Tiled2D Solver<R>::stage_tile2(int b, const StageDev& sd) const {
   t2.fA = (float)sd.A;
   t2.fB = (float)sd.B;
   t2.roundtrip = sd.roundtrip;
+  t2.dfmode = sd.dfmode;
   t2.f[0] = f[0].p;
   t2.f[1] = f[1].p;
   t2.df = df.p;
@@ -1152,15 +1164,17 @@ void Solver<R>::launch_bulk(int b, const StageDev& sd) {
     const Tiled2D* dsc = mdesc[(size_t)cur_stage * 2 + (sd.roundtrip ? 1 : 0)].p;
     const dim3 grid((unsigned)(((mnwork + 7) / 8) * 8));
     const int tw = tiles2[0].tw, s = S.sbporder;
+    const int dm = sd.dfmode;
     if (merged == 1) {
-      if (tw == 64) launch_merged_tw<R, 64>(dsc, mwork.p, mnwork, s, cur, grid, stream);
-      else if (tw == 128) launch_merged_tw<R, 128>(dsc, mwork.p, mnwork, s, cur, grid, stream);
-      else launch_merged_tw<R, 256>(dsc, mwork.p, mnwork, s, cur, grid, stream);
+      if (tw == 64) launch_merged_tw<R, 64>(dsc, mwork.p, mnwork, s, cur, dm, grid, stream);
+      else if (tw == 128) launch_merged_tw<R, 128>(dsc, mwork.p, mnwork, s, cur, dm, grid, stream);
+      else launch_merged_tw<R, 256>(dsc, mwork.p, mnwork, s, cur, dm, grid, stream);
     } else {
-      if (tw == 64) launch_mcurv_tw<R, 64>(dsc, mbd.p, mwork.p, mnwork, metric.p, jac.p, s, S.plastic, cur, grid, stream);
+      if (tw == 64)
+        launch_mcurv_tw<R, 64>(dsc, mbd.p, mwork.p, mnwork, metric.p, jac.p, s, S.plastic, cur, dm, grid, stream);
       else if (tw == 128)
-        launch_mcurv_tw<R, 128>(dsc, mbd.p, mwork.p, mnwork, metric.p, jac.p, s, S.plastic, cur, grid, stream);
-      else launch_mcurv_tw<R, 256>(dsc, mbd.p, mwork.p, mnwork, metric.p, jac.p, s, S.plastic, cur, grid, stream);
+        launch_mcurv_tw<R, 128>(dsc, mbd.p, mwork.p, mnwork, metric.p, jac.p, s, S.plastic, cur, dm, grid, stream);
+      else launch_mcurv_tw<R, 256>(dsc, mbd.p, mwork.p, mnwork, metric.p, jac.p, s, S.plastic, cur, dm, grid, stream);
     }
     HIPCHK(hipGetLastError());
     return;
@@ -1188,6 +1202,7 @@ void Solver<R>::launch_bulk(int b, const StageDev& sd) {
     ta.fA = (float)sd.A;
     ta.fB = (float)sd.B;
     ta.roundtrip = sd.roundtrip;
+    ta.dfmode = sd.dfmode;
     launch_tiled(ta, tile_ty);
     HIPCHK(hipGetLastError());
     return;
@@ -1214,10 +1229,40 @@ void Solver<R>::launch_plastic(int b) {
   HIPCHK(hipGetLastError());
 }
 
+template <typename R>
+int Solver<R>::stage_dfmode(int st, bool full_step) const {
+  if (opts.exact_fp64) return DF_ALL;  // the reference's scale_df and update, unchanged
+  int m = F_STORE;
+  if (S.rk.A[st] != 0.) m |= DF_LOAD;
+  if (!(full_step && st + 1 == S.rk.nstages && S.rk.A[0] == 0.)) m |= DF_STORE;
+  return m;
+}
+
+template <typename R>
+void Solver<R>::materialize_df() {
+  if (!df_stale) return;
+  const int st = S.rk.nstages - 1;
+  StageDev sd{S.rk.A[st], S.rk.B[st], S.dt, 0};
+  sd.dfmode = (S.rk.A[st] != 0. ? DF_LOAD : 0) | DF_STORE;
+  const int save_cur = cur, save_stage = cur_stage;
+  cur ^= 1;  // f[cur] is again the last stage's input
+  cur_stage = st;
+  for (size_t b = 0; b < S.blocks.size(); b++) {
+    if (merged && b > 0) break;
+    launch_bulk((int)b, sd);
+  }
+  cur = save_cur;
+  cur_stage = save_stage;
+  HIPCHK(hipStreamSynchronize(stream));
+  df_stale = false;
+}
+
 // domain::do_rk_stage, domain.cpp:228-305
 template <typename R>
 void Solver<R>::do_stage(int istep, int st, bool full_step) {
   const bool last = (st + 1 == S.rk.nstages);
+  const int dfm = stage_dfmode(st, full_step);
+  if (!capturing && df_stale && (dfm & DF_LOAD)) materialize_df();
   const double t = (double)istep * S.dt;
   const double tst = t + S.rk.C[st] * S.dt;
   const double tfront = (double)(istep + 1) * S.dt;
@@ -1233,7 +1278,7 @@ void Solver<R>::do_stage(int istep, int st, bool full_step) {
     HIPCHK(hipEventRecord(evs[2 * nev + 1], stream));
     nev++;
   }
-  StageDev sd{S.rk.A[st], S.rk.B[st], S.dt, (full_step && last && !S.plastic && sizeof(R) == 8) ? 1 : 0};
+  StageDev sd{S.rk.A[st], S.rk.B[st], S.dt, (full_step && last && !S.plastic && sizeof(R) == 8) ? 1 : 0, dfm};
   cur_stage = st;
   if (pair3) {  // the second block on the side stream, after the fault solve
     HIPCHK(hipEventRecord(ev_fork, stream));
@@ -1260,6 +1305,7 @@ void Solver<R>::do_stage(int istep, int st, bool full_step) {
     HIPCHK(hipStreamWaitEvent(stream, ev_join, 0));
   }
   cur ^= 1;
+  if (!capturing) df_stale = !(dfm & DF_STORE);
   if (last && S.plastic) {
     for (size_t b = 0; b < S.blocks.size(); b++) launch_plastic((int)b);
     FaultStageDev fs2{0., 0., S.dt, tst, 1, full_step ? tfront : -1., capturing ? graph_step_ptr : nullptr, S.rk.C[st],
@@ -1313,6 +1359,7 @@ void Solver<R>::graph_step() {
   }
   HIPCHK(hipGraphLaunch(graphs[par], stream));
   cur ^= graph_flip;
+  df_stale = !(stage_dfmode(S.rk.nstages - 1, true) & DF_STORE);
 }
 
 template <typename R>
@@ -1430,6 +1477,7 @@ void Solver<R>::read_array(int which, double* h, long n) {
     }
     case 1:
       if ((size_t)n != (size_t)S.nf * N) throw Error{"read_array: size mismatch"};
+      materialize_df();
       fetch(df.p, (size_t)n, h);
       break;
     case 2:
@@ -1465,6 +1513,7 @@ void Solver<R>::write_array(int which, const double* h, long n) {
   } else if (which == 1) {
     if ((size_t)n != (size_t)S.nf * N) throw Error{"write_array: size mismatch"};
     put(df.p, h, (size_t)n);
+    df_stale = false;
   } else {
     throw Error{"write_array: only f and df are writable"};
   }
@@ -1618,7 +1667,8 @@ class Batch : public BatchBase {
   double time_config(int tw_, int cx_, int persist_);
   void autotune();
   template <int TW>
-  void launch_bulk_tw(int st);
+  void launch_bulk_tw(int st, int dfm);
+  int stage_dfmode(int st) const;
 
   fdf_opts opts{};
   std::vector<std::unique_ptr<Solver<R>>> inst;
@@ -1744,11 +1794,22 @@ void Batch<R>::build(int tw_, int cx_, int persist_) {
 
 template <typename R>
 template <int TW>
-void Batch<R>::launch_bulk_tw(int st) {
+void Batch<R>::launch_bulk_tw(int st, int dfm) {
   const Tiled2D* d = descs[st].p;
-  if (sbp == 2) hipLaunchKernelGGL((bulk_tiled2d_batch<R, R, 2, TW>), dim3(grid), dim3(TW), 0, stream, d, work.p, nwork, cur);
-  else if (sbp == 3) hipLaunchKernelGGL((bulk_tiled2d_batch<R, R, 3, TW>), dim3(grid), dim3(TW), 0, stream, d, work.p, nwork, cur);
-  else hipLaunchKernelGGL((bulk_tiled2d_batch<R, R, 4, TW>), dim3(grid), dim3(TW), 0, stream, d, work.p, nwork, cur);
+  const dim3 g(grid), b(TW);
+  if (sbp == 2) hipLaunchKernelGGL((bulk_tiled2d_batch<R, R, 2, TW>), g, b, 0, stream, d, work.p, nwork, cur, dfm);
+  else if (sbp == 3) hipLaunchKernelGGL((bulk_tiled2d_batch<R, R, 3, TW>), g, b, 0, stream, d, work.p, nwork, cur, dfm);
+  else hipLaunchKernelGGL((bulk_tiled2d_batch<R, R, 4, TW>), g, b, 0, stream, d, work.p, nwork, cur, dfm);
+}
+
+// df traffic of stage st in a full step (as Solver::stage_dfmode; a batch
+// always runs the tiled kernels, never the bit-faithful mode)
+template <typename R>
+int Batch<R>::stage_dfmode(int st) const {
+  int m = F_STORE;
+  if (rk.A[st] != 0.) m |= DF_LOAD;
+  if (!(st + 1 == nst && rk.A[0] == 0.)) m |= DF_STORE;
+  return m;
 }
 
 // one RK stage of every instance: the stage of domain::do_rk_stage
@@ -1764,9 +1825,10 @@ void Batch<R>::launch_stage(int st) {
   if (nfwork)
     hipLaunchKernelGGL((fault_batch_kernel<0, R>), dim3(fgrid), dim3(256), 0, stream, fdescs.p, fwork.p, nfwork, fs, cur,
                        err.p);
-  if (tw == 64) launch_bulk_tw<64>(st);
-  else if (tw == 128) launch_bulk_tw<128>(st);
-  else launch_bulk_tw<256>(st);
+  const int dfm = stage_dfmode(st);
+  if (tw == 64) launch_bulk_tw<64>(st, dfm);
+  else if (tw == 128) launch_bulk_tw<128>(st, dfm);
+  else launch_bulk_tw<256>(st, dfm);
   HIPCHK(hipGetLastError());
   cur ^= 1;
 }
@@ -1781,15 +1843,16 @@ double Batch<R>::time_config(int tw_, int cx_, int persist_) {
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
   const int save = cur;
-  if (tw == 64) launch_bulk_tw<64>(0);
-  else if (tw == 128) launch_bulk_tw<128>(0);
-  else launch_bulk_tw<256>(0);
+  // a middle stage's full traffic (df loaded and stored)
+  if (tw == 64) launch_bulk_tw<64>(0, DF_ALL);
+  else if (tw == 128) launch_bulk_tw<128>(0, DF_ALL);
+  else launch_bulk_tw<256>(0, DF_ALL);
   const int reps = 3;
   HIPCHK(hipEventRecord(e0, stream));
   for (int k = 0; k < reps; k++) {
-    if (tw == 64) launch_bulk_tw<64>(0);
-    else if (tw == 128) launch_bulk_tw<128>(0);
-    else launch_bulk_tw<256>(0);
+    if (tw == 64) launch_bulk_tw<64>(0, DF_ALL);
+    else if (tw == 128) launch_bulk_tw<128>(0, DF_ALL);
+    else launch_bulk_tw<256>(0, DF_ALL);
   }
   HIPCHK(hipEventRecord(e1, stream));
   HIPCHK(hipEventSynchronize(e1));
@@ -1917,6 +1980,8 @@ void Batch<R>::step(int n) {
       hipLaunchKernelGGL(advance_step, dim3(1), dim3(1), 0, stream, dstep.p);
     }
     cur_step++;
+    const bool stale = !(stage_dfmode(nst - 1) & DF_STORE);
+    for (auto& I : inst) I->df_stale = stale;
     if (opts.write_outputs) {
       sync();
       for (auto& I : inst) I->write_outputs(cur_step);

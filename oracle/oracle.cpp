@@ -1442,6 +1442,27 @@ static double sw_mu_value(double u, double t, double dct, double must, double mu
   return (f1 >= f2) ? must + (mudt - must) * f1 : must + (mudt - must) * f2;
 }
 
+// slipweak::solve_fs for one point, slipweak.cpp:206-251: compressive normal
+// stress -> strength c0 + mu |sn|, locked if it exceeds phi; tensile -> slip
+// once phi exceeds the cohesion alone
+static void sw_solve_point(double phi, double eta, double snc, double c0t, double mu, double& V, double& S) {
+  if (snc < 0.) {
+    if (c0t + mu * fabs(snc) > phi) { V = 0.; S = phi; }
+    else { S = c0t + mu * fabs(snc); V = (phi - S) / eta; }
+  } else {
+    if (phi > c0t) { S = c0t; V = (phi - S) / eta; }
+    else { V = 0.; S = phi; }
+  }
+}
+
+// stz::calc_dstatedt + chihat, stz.cpp:311-356
+static double stz_dstatedt(double V, double S, double chi, double c0t, double Rt, double betat, double chiwt,
+                           double v1t) {
+  if (V > v1t) throw Err("Slip velocity exceeds melting value in stz.cpp");
+  double chihat = (V == 0.) ? chiwt : chiwt / log(v1t / V);
+  return V * S / c0t * (1. - chi / chihat) - Rt * exp(-betat / chi);
+}
+
 // friction law for one point: (phi, eta, snc) -> (V, S); slipweak.cpp:206-251,
 // friction.cpp:400-435, stz.cpp:263-356
 static void solve_fs(Iface& I, int i, int j, double phi, double eta, double snc, double t, double& V, double& S) {
@@ -1467,11 +1488,9 @@ static void solve_fs(Iface& I, int i, int j, double phi, double eta, double snc,
         tct += I.prm[5][index];
       }
       double mu = sw_mu_value(I.u[index], t, dct, must, mudt, trupt, tct);
-      if (c0t + mu * fabs(snc) > phi) { V = 0.; S = phi; }
-      else { S = c0t + mu * fabs(snc); V = (phi - S) / eta; }
+      sw_solve_point(phi, eta, snc, c0t, mu, V, S);
     } else {
-      if (phi > c0t) { S = c0t; V = (phi - S) / eta; }
-      else { V = 0.; S = phi; }
+      sw_solve_point(phi, eta, snc, c0t, 0., V, S);
     }
     return;
   }
@@ -1517,9 +1536,7 @@ static void solve_fs(Iface& I, int i, int j, double phi, double eta, double snc,
       chiwt += I.prm[7][index];
       v1t += I.prm[8][index];
     }
-    if (V > v1t) throw Err("Slip velocity exceeds melting value in stz.cpp");
-    double chihat = (V == 0.) ? chiwt : chiwt / log(v1t / V);
-    I.dstatedt[index] = V * S / c0t * (1. - I.state[index] / chihat) - Rt * exp(-betat / I.state[index]);
+    I.dstatedt[index] = stz_dstatedt(V, S, I.state[index], c0t, Rt, betat, chiwt, v1t);
   }
 }
 
@@ -2380,6 +2397,23 @@ void orc_plastic_flow(double dt, const double* in, double k, double g, double mu
   PMat m{mu, c, beta, eta};
   PS so = plastic_flow(dt, si, k, g, m);
   memcpy(out, &so, sizeof(PS));
+}
+
+double orc_stz_vpl(double mu, const double* p) { return calc_vpl(mu, p); }
+
+double orc_stz_dstatedt(double V, double S, double chi, double c0, double R, double beta, double chiw, double v1,
+                        int* status) {
+  *status = 0;
+  try {
+    return stz_dstatedt(V, S, chi, c0, R, beta, chiw, v1);
+  } catch (const Err&) {
+    *status = -1;
+    return 0.;
+  }
+}
+
+void orc_sw_solve(double phi, double eta, double snc, double c0, double mu, double* vs) {
+  sw_solve_point(phi, eta, snc, c0, mu, vs[0], vs[1]);
 }
 
 double orc_sw_mu(double u, double t, double dc, double mus, double mud, double trup, double tc) {

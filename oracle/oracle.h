@@ -60,6 +60,14 @@ double orc_stz_func(double mu, const double* params8);
 /* plastic_flow on (sxx sxy sxz syy syz szz gammap lambda epxx epxy epxz epyy epyz epzz) */
 void orc_plastic_flow(double dt, const double* s_in14, double k, double g,
                       double mu, double c, double beta, double eta, double* s_out14);
+/* STZ plastic slip rate V(mu) (stz.cpp:436-460), params as orc_stz_mu */
+double orc_stz_vpl(double mu, const double* params8);
+/* STZ effective-temperature rate (stz.cpp:311-356); *status = -1 when V > v1 */
+double orc_stz_dstatedt(double V, double S, double chi, double c0, double R, double beta, double chiw, double v1,
+                        int* status);
+/* slip-weakening solve for one point (slipweak.cpp:206-251) given the
+ * friction coefficient mu and cohesion c0: vs = (V, S) */
+void orc_sw_solve(double phi, double eta, double snc, double c0, double mu, double* vs);
 /* slip-weakening friction coefficient (slipweak.cpp:253-308) */
 double orc_sw_mu(double u, double t, double dc, double mus, double mud, double trup, double tc);
 /* solve_locked on one characteristic (interface.cpp:1017-1028): io = s1 s2 v1 v2 */

@@ -13,7 +13,7 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_DIR = os.path.join(ROOT, "oracle")
-LIB_PATH = os.path.join(ORACLE_DIR, "_build", "liboracle.so")
+LIB_PATH = os.environ.get("FDF_ORACLE_LIB") or os.path.join(ORACLE_DIR, "_build", "liboracle.so")
 EXE_PATH = os.path.join(ORACLE_DIR, "_build", "fdfault_oracle")
 
 _lib = None
@@ -59,6 +59,11 @@ def lib():
                                        C.c_double, C.c_double, dp]
         L.orc_sw_mu.restype = C.c_double
         L.orc_sw_mu.argtypes = [C.c_double] * 7
+        L.orc_stz_vpl.restype = C.c_double
+        L.orc_stz_vpl.argtypes = [C.c_double, dp]
+        L.orc_stz_dstatedt.restype = C.c_double
+        L.orc_stz_dstatedt.argtypes = [C.c_double] * 8 + [C.POINTER(C.c_int)]
+        L.orc_sw_solve.argtypes = [C.c_double] * 5 + [dp]
         L.orc_solve_locked.argtypes = [dp, C.c_double, C.c_double]
         L.orc_rotate.argtypes = [dp, dp, dp, dp, C.c_int, dp]
         L.orc_mlp_forward.argtypes = [dp, dp, C.c_long, dp]

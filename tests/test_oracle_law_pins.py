@@ -1,0 +1,479 @@
+"""Pins of the oracle's friction laws, load ramp, mode-III and szz rates,
+interface penalty and fault input files against what the mathematics fixes.
+
+* STZ flow law V(mu) (stz.cpp:436-460): V = v0 exp(-f0 + mu/a - 1/chi)
+  (1 - muy/mu) checked at known answers -- with f0 = muy = 0 and 1/chi -> 0,
+  mu = a ln 2 doubles v0; mu = 2 muy halves the exponential; each of f0 and
+  chi scales V by its own exponential factor; V = 0 at or below muy.
+* STZ effective-temperature rate (stz.cpp:311-356): V = 0 leaves only the
+  relaxation -R exp(-beta/chi); with R = 0 the rate vanishes at
+  chi = chihat(V) = chiw / ln(v1/V), and at V = v1/e (chihat = chiw) with
+  chi = chiw/2 it is V S / (2 c0); V > v1 is the abort path.
+* Slip-weakening mu(U, t) (slipweak.cpp:253-308): the slip ramp and the
+  time ramp each reach the midpoint (mus + mud)/2 halfway, mud at their
+  ends, and the larger of the two weakenings wins.
+* Slip-weakening strength (slipweak.cpp:206-251): locked below strength
+  c0 + mu |sn|; sliding at strength with V = (phi - S)/eta; tensile normal
+  stress keeps only the cohesion.
+* Mode-III bulk rates (block.cpp:1384-1539) and the plane-strain szz rate of
+  plastic mode II (block.cpp:1280-1295) on linear fields: exact.
+* Interface penalty (interface.cpp:573-1028): a plane pulse meeting a locked
+  interface between two materials splits into the reflection and
+  transmission coefficients (Z1 - Z2)/(Z1 + Z2) and 2 Z1/(Z1 + Z2) of the
+  impedances Z = rho c (S waves in mode II and III, P waves in mode II).
+* Load time ramp and stage time (pert.cpp:45-57, domain.cpp:254): amplitude
+  times min(t/t0, 1) at t = (step + C[stage]) dt.
+* Fault input files (friction.cpp:505-567, slipweak.cpp:310-383,
+  stz.cpp:358-434): a file holding a non-uniform perturbation's values at
+  every fault point gives a run bit-identical to the perturbation itself
+  (catches a transposed or misordered layout); the STZ state file is read
+  into the state array as written.
+"""
+import ctypes as C
+import math
+import os
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from cases import STZ_PARAMS, write_case
+from fdfault_amd.problems import Pert, two_block_fault
+
+L = O.lib()
+dp = C.POINTER(C.c_double)
+
+
+def _d(a):
+    return a.ctypes.data_as(dp)
+
+
+def vpl(mu, chi, v0, f0, a, muy):
+    p = np.array([0., 1., -100., chi, v0, f0, a, muy])
+    return L.orc_stz_vpl(mu, _d(p))
+
+
+def test_stz_flow_law_known_answers():
+    a, v0 = 0.01, 1e-6
+    big = 1e300  # 1/chi below every rounding
+    assert vpl(a * math.log(2.), big, v0, 0., a, 0.) == pytest.approx(2 * v0, rel=1e-13)
+    assert vpl(a * math.log(8.), big, v0, 0., a, 0.) == pytest.approx(8 * v0, rel=1e-13)
+    muy = 0.2
+    assert vpl(2 * muy, big, v0, 0., a, muy) == pytest.approx(0.5 * v0 * math.exp(2 * muy / a), rel=1e-13)
+    assert vpl(4 * muy, big, v0, 0., a, muy) == pytest.approx(0.75 * v0 * math.exp(4 * muy / a), rel=1e-13)
+    mu = 0.35
+    base = vpl(mu, big, v0, 0., a, muy)
+    assert vpl(mu, big, v0, 0.3, a, muy) == pytest.approx(base * math.exp(-0.3), rel=1e-13)
+    assert vpl(mu, 1.0, v0, 0., a, muy) == pytest.approx(base * math.exp(-1.0), rel=1e-13)
+    assert vpl(mu, 0.5, v0, 0., a, muy) == pytest.approx(base * math.exp(-2.0), rel=1e-13)
+    assert vpl(muy, 0.07, v0, 0.3, a, muy) == 0.0
+    assert vpl(0.5 * muy, 0.07, v0, 0.3, a, muy) == 0.0
+
+
+def dchi(V, S, chi, c0, R, beta, chiw, v1):
+    st = C.c_int()
+    r = L.orc_stz_dstatedt(V, S, chi, c0, R, beta, chiw, v1, C.byref(st))
+    return r, st.value
+
+
+def test_stz_state_rate_known_answers():
+    c0, beta, chiw, v1 = 1e4, 0.1, 0.4, 1e4
+    for chi in (0.05, 0.07, 0.3):
+        for R in (1e-3, 2.5):
+            r, st = dchi(0., 70., chi, c0, R, beta, chiw, v1)
+            assert st == 0 and r == pytest.approx(-R * math.exp(-beta / chi), rel=1e-14)
+    for V in (1e-6, 1e-2, 3.0):
+        chihat = chiw / math.log(v1 / V)
+        r, st = dchi(V, 60., chihat, c0, 0., beta, chiw, v1)
+        assert st == 0 and abs(r) <= 1e-14 * V * 60. / c0
+        # below chihat the state heats up, above it the state cools
+        assert dchi(V, 60., 0.5 * chihat, c0, 0., beta, chiw, v1)[0] > 0
+        assert dchi(V, 60., 1.5 * chihat, c0, 0., beta, chiw, v1)[0] < 0
+    V = v1 / math.e
+    r, st = dchi(V, 60., 0.5 * chiw, c0, 0., beta, chiw, v1)
+    assert st == 0 and r == pytest.approx(V * 60. / (2 * c0), rel=1e-13)
+    assert dchi(2 * v1, 60., 0.07, c0, 0., beta, chiw, v1)[1] == -1
+
+
+def sw_mu(u, t, dc=0.4, mus=0.677, mud=0.525, trup=0., tc=0.):
+    return L.orc_sw_mu(u, t, dc, mus, mud, trup, tc)
+
+
+def test_slipweak_mu_known_answers():
+    mus, mud, dc = 0.677, 0.525, 0.4
+    mid = 0.5 * (mus + mud)
+    assert sw_mu(0., 0.) == mus
+    assert sw_mu(0.5 * dc, 3.) == pytest.approx(mid, rel=1e-15)
+    assert sw_mu(0.25 * dc, 3.) == pytest.approx(mus + 0.25 * (mud - mus), rel=1e-15)
+    assert sw_mu(dc, 3.) == mud and sw_mu(3 * dc, 3.) == mud
+    assert sw_mu(0., 0., dc=0.) == mud  # dc = 0: weakened at once
+    trup, tc = 1.0, 0.5
+    assert sw_mu(0., 0.99, trup=trup, tc=tc) == mus
+    assert sw_mu(0., trup + 0.5 * tc, trup=trup, tc=tc) == pytest.approx(mid, rel=1e-15)
+    assert sw_mu(0., trup + tc, trup=trup, tc=tc) == mud
+    assert sw_mu(0., trup, trup=trup, tc=0.) == mud  # tc = 0: weakened at trup
+    # the larger weakening wins, in either order
+    assert sw_mu(0.25 * dc, trup + 0.75 * tc, trup=trup, tc=tc) == pytest.approx(mus + 0.75 * (mud - mus), rel=1e-15)
+    assert sw_mu(0.75 * dc, trup + 0.25 * tc, trup=trup, tc=tc) == pytest.approx(mus + 0.75 * (mud - mus), rel=1e-15)
+    # trup <= 0 switches time weakening off
+    assert sw_mu(0., 100., trup=0., tc=0.5) == mus
+
+
+def sw_solve(phi, eta, snc, c0, mu):
+    vs = np.zeros(2)
+    L.orc_sw_solve(phi, eta, snc, c0, mu, _d(vs))
+    return vs
+
+
+def test_slipweak_strength_branches():
+    eta, mu = 4.6, 0.6
+    # compressive, below strength: locked, traction = phi
+    V, S = sw_solve(70., eta, -120., 5., mu)
+    assert V == 0. and S == 70.
+    # compressive, above strength c0 + mu |sn|: sliding at strength
+    V, S = sw_solve(90., eta, -120., 5., 0.525)
+    assert S == pytest.approx(5. + 0.525 * 120., rel=1e-15)
+    assert V == pytest.approx((90. - S) / eta, rel=1e-14) and V > 0
+    # cohesion alone can hold the fault
+    V, S = sw_solve(70., eta, -120., 10., 0.5)
+    assert V == 0. and S == 70.
+    # tensile (and zero) normal stress: only the cohesion resists
+    for sn in (30., 0.):
+        V, S = sw_solve(12., eta, sn, 10., mu)
+        assert S == 10. and V == pytest.approx(2. / eta, rel=1e-14)
+        V, S = sw_solve(8., eta, sn, 10., mu)
+        assert V == 0. and S == 8.
+
+
+def quiet(tmp_path, name, **kw):
+    """Locked interface, no loads, no boundary penalty: df after one stage
+    from df = 0 is dt times the bulk right-hand side."""
+    kw.setdefault("law", "locked")
+    kw.setdefault("bc", "none")
+    kw.setdefault("front", None)
+    kw.setdefault("nt", 1)
+    kw.setdefault("rkorder", 1)
+    kw.setdefault("loads", [])
+    return write_case(str(tmp_path), name, **kw)
+
+
+def quiet_mat(tmp_path, name, rho, lam, g, **kw):
+    """quiet() with both blocks of material (rho, lam, g): lam != G makes a
+    swapped modulus visible."""
+    kw.setdefault("law", "locked")
+    kw.setdefault("bc", "none")
+    kw.setdefault("front", None)
+    kw.setdefault("nt", 1)
+    kw.setdefault("rkorder", 1)
+    kw.setdefault("loads", [])
+    d = os.path.join(str(tmp_path), name)
+    os.makedirs(os.path.join(d, "data"), exist_ok=True)
+    p = two_block_fault(datadir=os.path.join(d, "data") + "/", name=name, **kw)
+    for b in p.blocks:
+        b.update(rho=rho, lam=lam, g=g)
+    return p.write(os.path.join(d, name + ".in"))
+
+
+@pytest.mark.parametrize("sbp", [2, 3, 4])
+def test_mode3_linear_fields_exact_rates(tmp_path, sbp):
+    G, rho = 20.0, 2.67
+    p = quiet_mat(tmp_path, "m3lin", rho, 45.0, G, mode=3, nx=31, ny=16, dx=0.1, sbporder=sbp)
+    o = O.Oracle(p)
+    X, Y = o.array("x")[:2]
+    dt = o.dt
+    f = np.zeros(o.array("f").shape)  # vz sxz syz
+    f[0] = 0.3 * X - 0.8 * Y
+    o.set_array("f", f)
+    o.set_array("df", np.zeros(o.array("df").shape))
+    o.stage(0, 0)
+    df = o.array("df")
+    np.testing.assert_allclose(df[1], dt * G * 0.3, rtol=1e-10)
+    np.testing.assert_allclose(df[2], dt * G * -0.8, rtol=1e-10)
+    assert np.abs(df[0]).max() <= 1e-10
+    f[:] = 0.
+    f[1] = 1.5 * X + 0.2 * Y  # d/dx sxz = 1.5
+    f[2] = -0.5 * X + 2.5 * Y  # d/dy syz = 2.5
+    o.set_array("f", f)
+    o.set_array("df", np.zeros(o.array("df").shape))
+    o.stage(0, 0)
+    df = o.array("df")
+    np.testing.assert_allclose(df[0], dt / rho * (1.5 + 2.5), rtol=1e-10)
+    assert np.abs(df[1:]).max() <= 1e-10
+
+
+@pytest.mark.parametrize("sbp", [2, 4])
+def test_plastic_szz_rate_plane_strain(tmp_path, sbp):
+    """Plane strain: the szz rate is nu (rate sxx + rate syy) = lambda div v."""
+    plastic = dict(mu=0.68, c=0., beta=0.35, eta=0.05 * 32.04)
+    lam, G = 45.0, 20.0
+    p = quiet_mat(tmp_path, "szz", 2.67, lam, G, nx=31, ny=16, dx=0.1, sbporder=sbp, material="plastic",
+                  plastic=plastic)
+    o = O.Oracle(p)
+    X, Y = o.array("x")[:2]
+    dt = o.dt
+    a, b, c, d = 0.3, -0.7, 1.1, 0.4
+    f = np.zeros(o.array("f").shape)  # vx vy sxx sxy syy szz lambda gammap
+    f[0] = a * X + b * Y
+    f[1] = c * X + d * Y
+    o.set_array("f", f)
+    o.set_array("df", np.zeros(o.array("df").shape))
+    o.stage(0, 0)
+    df = o.array("df")
+    assert df.shape[0] == 6
+    np.testing.assert_allclose(df[5], dt * lam * (a + d), rtol=1e-10)
+    np.testing.assert_allclose(df[2], dt * ((lam + 2 * G) * a + lam * d), rtol=1e-10)
+
+
+RHO1, LAM1, G1 = 2.67, 32.04, 32.04
+RHO2, LAM2, G2 = 3.0, 10.0, 20.0
+
+
+def bimaterial(tmp_path, name, mode):
+    """Two blocks glued by a locked interface at y = 4 km, material 1 below,
+    material 2 above; plane fields (uniform in x), so the x faces carry no
+    penalty and the outer y faces absorb."""
+    d = os.path.join(str(tmp_path), name)
+    os.makedirs(os.path.join(d, "data"), exist_ok=True)
+    p = two_block_fault(ndim=2, mode=mode, nx=13, ny=161, dx=0.025, nt=1, sbporder=4, law="locked", bc="none",
+                        loads=[], front=None, datadir=os.path.join(d, "data") + "/", name=name)
+    p.blocks[1].update(rho=RHO2, lam=LAM2, g=G2)
+    for b, face in ((0, 2), (1, 3)):
+        bc = list(p.blocks[b]["bc"])
+        bc[face] = "absorbing"
+        p.blocks[b]["bc"] = bc
+    return p.write(os.path.join(d, name + ".in"))
+
+
+@pytest.mark.parametrize("mode,wave", [(3, "S"), (2, "S"), (2, "P")])
+def test_interface_reflection_transmission(tmp_path, mode, wave):
+    o = O.Oracle(bimaterial(tmp_path, "bm%d%s" % (mode, wave), mode))
+    x = o.array("x")
+    Y = x[1]
+    ny = o.nx[1] // 2
+    w, y0 = 0.25, 2.5
+    g = np.exp(-((Y - y0) / w) ** 2)
+    g[:, ny:] = 0.
+    f = np.zeros(o.array("f").shape)
+    if wave == "S":
+        c1, c2 = math.sqrt(G1 / RHO1), math.sqrt(G2 / RHO2)
+        vi, si = (0, 2) if mode == 3 else (0, 3)  # vz/syz or vx/sxy
+    else:
+        c1, c2 = math.sqrt((LAM1 + 2 * G1) / RHO1), math.sqrt((LAM2 + 2 * G2) / RHO2)
+        vi, si = 1, 4  # vy/syy; sxx = lam/(lam+2G) syy in a plane P wave
+        f[2] = -LAM1 / (LAM1 + 2 * G1) * RHO1 * c1 * g
+    Z1, Z2 = RHO1 * c1, RHO2 * c2
+    f[vi] = g
+    f[si] = -Z1 * g  # travelling towards +y
+    o.set_array("f", f)
+    o.set_array("df", np.zeros(o.array("df").shape))
+    T = (4.0 - y0 + 4 * w) / c1 + 0.5 * w / c2  # the incident pulse has left block 1
+    o.step(int(round(T / o.dt)))
+    v = o.array("f")[vi][o.nx[0] // 2, :, 0]
+    y = Y[o.nx[0] // 2, :, 0]
+    I0 = w * math.sqrt(math.pi)
+    refl = np.trapezoid(v[:ny], y[:ny]) / I0
+    trans = np.trapezoid(v[ny:], y[ny:]) / I0 * c1 / c2  # the pulse is compressed by c2/c1
+    R, Tt = (Z1 - Z2) / (Z1 + Z2), 2 * Z1 / (Z1 + Z2)
+    print(mode, wave, "R %.7f (%.7f)  T %.7f (%.7f)" % (refl, R, trans, Tt))
+    assert abs(refl - R) <= 1e-4 and abs(trans - Tt) <= 1e-4
+    assert abs(R) > 0.05  # the materials differ enough for the check to bite
+
+
+def sbp_norm(s, n):
+    """Diagonal SBP norm weights (in grid units): h0 at the ends, the other
+    boundary weights recovered from the symmetry of H D off the diagonal."""
+    from test_oracle_pins import d_matrix, fd_tables
+    D = d_matrix(s, n)
+    h0 = fd_tables(s)[2]
+    h = np.ones(n)
+    h[0] = h[-1] = h0
+    nb = 2 * (s - 1)
+    for i in range(1, nb):
+        j = next(j for j in range(n) if j != i and D[i, j] != 0 and D[j, i] != 0 and (j >= nb or h[j] != 1.0 or j == 0))
+        h[i] = h[n - 1 - i] = -h[j] * D[j, i] / D[i, j]
+    return h
+
+
+@pytest.mark.parametrize("sbp", [2, 3, 4])
+@pytest.mark.parametrize("bc", ["absorbing", "free", "rigid"])
+def test_energy_rate_nonpositive(tmp_path, sbp, bc):
+    """SBP-SAT energy estimate: with a locked interface between two different
+    materials and absorbing, free or rigid outer faces, the discrete energy
+    sum_H (rho |v|^2 + s : C^-1 : s) / 2 cannot grow, for any state.  One
+    Euler stage from a random state gives df = dt * rate; the H-weighted
+    energy rate must be <= 0 (interior terms cancel by summation by parts,
+    boundary and interface terms are dissipated by the penalties)."""
+    mats = ((2.67, 32.04, 32.04), (3.0, 10.0, 20.0))
+    nx, ny, dx = 41, 21, 0.1
+    d = os.path.join(str(tmp_path), "en")
+    os.makedirs(os.path.join(d, "data"), exist_ok=True)
+    p = two_block_fault(ndim=2, mode=2, nx=nx, ny=ny, dx=dx, nt=1, sbporder=sbp, rkorder=1, law="locked", bc=bc,
+                        loads=[], front=None, datadir=os.path.join(d, "data") + "/", name="en")
+    for b in (0, 1):
+        p.blocks[b].update(rho=mats[b][0], lam=mats[b][1], g=mats[b][2])
+    o = O.Oracle(p.write(os.path.join(d, "en.in")))
+    hx, hy = sbp_norm(sbp, nx) * dx, sbp_norm(sbp, ny) * dx
+    H = hx[:, None] * hy[None, :]
+    for seed in range(3):
+        f = np.random.default_rng(seed).normal(size=o.array("f").shape)
+        o.set_array("f", f)
+        o.set_array("df", np.zeros(o.array("df").shape))
+        o.stage(0, 0)
+        df = o.array("df") / o.dt
+        rate, scale = 0.0, 0.0
+        for b in (0, 1):
+            sl = slice(0, ny) if b == 0 else slice(ny, 2 * ny)
+            rho, lam, G = mats[b]
+            vx, vy, sxx, sxy, syy = (f[q][:, sl, 0] for q in range(5))
+            dvx, dvy, dsxx, dsxy, dsyy = (df[q][:, sl, 0] for q in range(5))
+            Ci = np.linalg.inv(np.array([[lam + 2 * G, lam], [lam, lam + 2 * G]]))
+            exx, eyy = Ci[0, 0] * sxx + Ci[0, 1] * syy, Ci[1, 0] * sxx + Ci[1, 1] * syy
+            term = rho * (vx * dvx + vy * dvy) + exx * dsxx + eyy * dsyy + sxy * dsxy / G
+            rate += np.sum(H * term)
+            scale += np.sum(H * np.abs(term))
+        print(sbp, bc, seed, "energy rate %.3e of %.3e" % (rate, scale))
+        assert rate <= 1e-12 * scale
+
+
+def test_load_time_ramp_and_stage_time(tmp_path):
+    """Frictionless fault, zero bulk fields: the hat normal traction is the
+    ramped normal load, sn min(t/t0, 1), and the slip rate is |s2| / eta,
+    both at the stage time t = (step + C[stage]) dt."""
+    t0, sn, s2 = 0.05, -100., 30.
+    loads = [Pert("constant", t0, 0., 0., 0., 0., (sn, s2, 0.))]
+    p = write_case(str(tmp_path), "ramp", nx=41, ny=13, dx=0.1, law="frictionless", loads=loads, front=None,
+                   rkorder=4)
+    o = O.Oracle(p)
+    ns = C.c_int()
+    A, B, Cc = np.zeros(5), np.zeros(5), np.zeros(6)
+    L.orc_rk_coeffs(4, C.byref(ns), _d(A), _d(B), _d(Cc))
+    zs = math.sqrt(2.67 * 32.04)
+    eta = zs * zs / (zs + zs)
+    zero_f, zero_df = np.zeros(o.array("f").shape), np.zeros(o.array("df").shape)
+    checked = 0
+    for istep in (0, 3, 7, 20):
+        for st in range(ns.value):
+            o.set_array("f", zero_f)
+            o.set_array("df", zero_df)
+            o.stage(istep, st)
+            t = istep * o.dt + Cc[st] * o.dt
+            ramp = 1.0 if t >= t0 else t / t0
+            np.testing.assert_allclose(o.iface(0, "Sn"), sn * ramp, rtol=1e-14, atol=1e-300)
+            np.testing.assert_allclose(o.iface(0, "V"), abs(s2) * ramp / eta, rtol=1e-13, atol=1e-300)
+            checked += ramp < 1.0
+    assert checked >= 6  # stages inside the ramp, where the time argument matters
+
+
+# --------------------------------------------------------------------------
+# fault input files: bit-identical to the perturbation whose values they hold
+
+
+def linear_values(nx, nz, dx, x0, sx, z0, sz, ndim):
+    """Values of pert.cpp's linear shape, (x(i)/dx + x0)(z(j)/dz + z0), at
+    every point of a y-face of the two-block case (face axes x and z)."""
+    Lx, Lz = (nx - 1) * dx, (nz - 1) * dx
+    b = Lx / float(nx - 1)
+    xi = b * np.arange(nx, dtype=float) + 0.0
+    xv = xi / sx + x0
+    if ndim == 2:
+        return xv[:, None] * 1.0
+    d = Lz / float(nz - 1)
+    zj = d * np.arange(nz, dtype=float) + 0.0
+    zv = zj / sz + z0
+    return xv[:, None] * zv[None, :]
+
+
+SHAPE2 = dict(x0=0.5, sx=3.0, z0=0.0, sz=0.0)
+SHAPE3 = dict(x0=0.5, sx=3.0, z0=1.5, sz=-1.7)
+
+
+def _shape(ndim):
+    return SHAPE2 if ndim == 2 else SHAPE3
+
+
+def _pert(kind_amps, ndim):
+    s = _shape(ndim)
+    return Pert("linear", 0., s["x0"], s["sx"], s["z0"], abs(s["sz"]), kind_amps)
+
+
+def _geom(ndim):
+    return dict(nx=33, ny=14, nz=19, dx=0.1) if ndim == 3 else dict(nx=61, ny=21, dx=0.1)
+
+
+def _xy(ndim):
+    g = _geom(ndim)
+    s = _shape(ndim)
+    return linear_values(g["nx"], g.get("nz", 1), g["dx"], s["x0"], s["sx"], s["z0"], abs(s["sz"]), ndim)
+
+
+def file_cases(tmp, ndim, which):
+    """Two problem files: perturbation (A) and input file (B) carrying the
+    same non-uniform values.  Returns (pathA, pathB)."""
+    base = [Pert("constant", 0., 0., 0., 0., 0., (-120., 70., 0.)),
+            Pert("boxcar", 0., 3.0 if ndim == 2 else 1.6, 0.6, 0.9, 0.6 if ndim == 3 else 0., (0., 11.6, 0.))]
+    kw = dict(ndim=ndim, nt=25, **_geom(ndim))
+    xy = _xy(ndim)
+    d = os.path.join(str(tmp), "%s%d" % (which, ndim))
+    os.makedirs(d, exist_ok=True)
+    if which == "load":
+        amps = (-4.0, 3.0, 1.5 if ndim == 3 else 0.)
+        a = write_case(str(tmp), "A" + which + str(ndim), loads=[_pert(amps, ndim)] + base, **kw)
+        vals = np.stack([amp * xy * 1.0 for amp in amps])  # [3][n0][n1]
+        lf = os.path.join(d, "load.bin")
+        vals.tofile(lf)
+        b = write_case(str(tmp), "B" + which + str(ndim), loads=base, loadfile=lf, **kw)
+    elif which == "sw":
+        sw_base = Pert("constant", 0., 0., 0., 0., 0., (0.4, 0.677, 0.525, 0., 0., 0.))
+        amps = (0.02, 0.01, -0.02, 0.3, 0.004, 0.002)  # dc mus mud c0 trup tc
+        a = write_case(str(tmp), "A" + which + str(ndim), loads=base, sw=[sw_base, _pert(amps, ndim)], **kw)
+        vals = np.stack([amp * xy * 1.0 for amp in amps])  # [6][n0][n1]
+        pf = os.path.join(d, "sw.bin")
+        vals.tofile(pf)
+        b = write_case(str(tmp), "B" + which + str(ndim), loads=base, sw=[sw_base], paramfile=pf, **kw)
+    else:  # stz parameter file
+        amps = (1e-8, 0.01, 0.0005, 0.01, 100., 1e-5, 0.002, 0.01, 100.)  # v0 f0 a muy c0 R beta chiw v1
+        sb = STZ_PARAMS["perts"][0]
+        stz_a = dict(STZ_PARAMS, chi0=0.07, perts=[sb, _pert(amps, ndim)])
+        stz_b = dict(STZ_PARAMS, chi0=0.07, perts=[sb], paramfile=None)
+        vals = np.stack([amp * xy * 1.0 for amp in amps])  # [9][n0][n1]
+        pf = os.path.join(d, "stz.bin")
+        vals.tofile(pf)
+        stz_b["paramfile"] = pf
+        loads = [Pert("constant", 0., 0., 0., 0., 0., (-120., 70., 0.))]
+        a = write_case(str(tmp), "A" + which + str(ndim), law="stz", stz=stz_a, loads=loads, **kw)
+        b = write_case(str(tmp), "B" + which + str(ndim), law="stz", stz=stz_b, loads=loads, **kw)
+    return a, b
+
+
+FILE_FIELDS = ("U", "V", "S", "Sn", "Ux", "Vx", "Sx")
+
+
+@pytest.mark.parametrize("ndim", [2, 3])
+@pytest.mark.parametrize("which", ["load", "sw", "stz"])
+def test_fault_input_file_equals_perturbation(tmp_path, ndim, which):
+    pa, pb = file_cases(tmp_path, ndim, which)
+    oa, ob = O.Oracle(pa), O.Oracle(pb)
+    nt = 6 if ndim == 3 else 25
+    oa.step(nt)
+    ob.step(nt)
+    fields = FILE_FIELDS + (("state",) if which == "stz" else ())
+    for w in fields:
+        np.testing.assert_array_equal(oa.iface(0, w), ob.iface(0, w), err_msg=w)
+    np.testing.assert_array_equal(oa.array("f"), ob.array("f"))
+    V = oa.iface(0, "V")
+    assert np.ptp(V) > 0  # the files change something along the fault
+
+
+def test_stz_state_file_layout(tmp_path):
+    """The state array after set-up is the file (+ chi0) in [n0][n1] order."""
+    nx, nz = 33, 19
+    rng = np.random.default_rng(9)
+    chi = rng.uniform(0.06, 0.08, size=(nx, nz))
+    sp = os.path.join(str(tmp_path), "state.bin")
+    chi.tofile(sp)
+    p = write_case(str(tmp_path), "st3", ndim=3, nx=nx, ny=14, nz=nz, dx=0.1, law="stz",
+                   stz=dict(STZ_PARAMS, chi0=0.01, statefile=sp),
+                   loads=[Pert("constant", 0., 0., 0., 0., 0., (-120., 70., 0.))])
+    o = O.Oracle(p)
+    assert o.iface_shape(0)[:2] == (nx, nz)
+    np.testing.assert_array_equal(o.iface(0, "state"), (chi + 0.01).ravel())
