@@ -113,10 +113,11 @@ def single_workload(args):
         g.close()
         cpu = None
         if not args.no_cpu:
-            threads = min(os.cpu_count() or 1, 16)
+            threads, host = cpu_threads()
             cv, cn = cpu_baseline(path, args.cpu_steps, threads)
             cpu = {"value": round(cv, 3), "unit": "Mpt-updates/s", "cores": threads, "kind": "port",
-                   "sample": "the same problem (%d pts), %d steps, FP64 CPU oracle, OpenMP" % (cn, args.cpu_steps)}
+                   "sample": "the same problem (%d pts), %d steps, FP64 CPU oracle, OpenMP on %s"
+                             % (cn, args.cpu_steps, host)}
     nf = 6 if rough else 5
     # the bulk kernel's own algorithmic bytes (the plastic fields are read and
     # written once per step by the plasticity kernel, not by this kernel)
@@ -283,11 +284,36 @@ def stz_proposal(args):
     print(json.dumps(line), flush=True)
 
 
+def stage_field_words(nstages=5):
+    """Words per evolved field per point, averaged over the stages of a time
+    step, that a low-storage RK stage must move: read f and write f always,
+    read df except where A = 0 (the first stage of every scheme of
+    rk.cpp:42-91), write df except on the last stage (its only reader is the
+    next step's first stage, which scales it by A = 0).  RK4 (5 stages):
+    (3 + 4 + 4 + 4 + 3) / 5 = 3.6."""
+    if nstages == 1:
+        return 2.0
+    return (3 + 4 * (nstages - 2) + 3) / nstages
+
+
 def bytes_per_point(nf, w, curvilinear=False, ndim=3, plastic_words=0, nstages=5):
-    """Algorithmic bytes per point per RK stage: read f, read df, write df,
-    write f (4 nf words) + geometry words (curvilinear) + plastic words/step."""
+    """Algorithmic bytes per point per RK stage (average over a step): the
+    f/df words of stage_field_words per field + geometry words (curvilinear)
+    + plastic words per step."""
     G = (ndim * ndim + 1) if curvilinear else 0
-    return w * (4 * nf + G) + w * plastic_words / nstages
+    return w * (stage_field_words(nstages) * nf + G) + w * plastic_words / nstages
+
+
+def cpu_threads():
+    """Threads for the CPU baseline: every core this process may run on (its
+    affinity mask, what `nproc` counts), limited by OMP_NUM_THREADS when the
+    host sets it (this job's CPU share); returns the count and a description
+    of the host."""
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    omp = os.environ.get("OMP_NUM_THREADS")
+    n = min(aff, int(omp)) if omp and omp.isdigit() and int(omp) > 0 else aff
+    return n, "%d threads (affinity %d cores, %d logical CPUs on the host, OMP_NUM_THREADS=%s)" % (
+        n, aff, os.cpu_count() or 0, omp)
 
 
 def cpu_baseline(sample_in, steps, threads):
@@ -451,11 +477,11 @@ def main():
             os.makedirs(sd)
             snx, sny, snz = 201, 101, 101
             sp = workload_3d(sd, snx, sny, snz, sbporder=args.sbporder)
-            threads = min(os.cpu_count() or 1, 16)
+            threads, host = cpu_threads()
             cv, cn = cpu_baseline(sp, args.cpu_steps, threads)
             cpu = {"value": round(cv, 3), "unit": "Mpt-updates/s", "cores": threads, "kind": "port",
                    "sample": "same 3-D workload at 2x%dx%dx%d points (%d pts), %d steps, FP64 CPU oracle, "
-                             "OpenMP" % (snx, sny, snz, cn, args.cpu_steps)}
+                             "OpenMP on %s" % (snx, sny, snz, cn, args.cpu_steps, host)}
     h, o = res[headline], res.get(other)
     line = {
         "metric": "Mpt-updates/s per RK stage & % HBM peak; FP32 vs FP64 at <=1e-4 slip err",
@@ -500,7 +526,11 @@ def run_planar(lib, path, pname, args, workload):
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": tr["traffic"] if tr else None,
             "traffic_source": tr["summary"] if tr else None, "kernel": kernel,
             "avg_launch_ms": round(kms, 4), "launches": nlaunch, "bytes_per_point": B,
-            "points_per_launch": int(ppl)}
+            "points_per_launch": int(ppl),
+            # the same launches counted at 4 nf words per point and stage (read
+            # f and df, write df and f), the per-stage bytes of a low-storage
+            # update that does not exploit A = 0
+            "frac_at_4nf_words": round(w * 4 * 9 * ppl / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
     ref = stream_reference(int(ppl), w) if not args.no_stream else None
     if ref:
         roof.update(ref)

@@ -1429,7 +1429,12 @@ __global__ void __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(WP
   const unsigned psb = (unsigned)(sx * (long long)sizeof(R));    // plane size in bytes
   // one descriptor per array (the host keeps 9 N elements under 4 GiB);
   // field q of plane i lies at scalar byte offset q*fb + i*psb
-  const rsrc_t rf = buf_rsrc(f, 9 * N), rd = buf_rsrc(df, 9 * N), rn = buf_rsrc(fn, 9 * N);
+  // df/f traffic a stage does not need goes through a descriptor with no
+  // records: out-of-range buffer loads return 0 and stores are dropped,
+  // without a memory request, and the instruction stream stays the same
+  const rsrc_t rf = buf_rsrc(f, 9 * N), rd = buf_rsrc(df, (ta.dfmode & DF_LOAD) ? 9 * N : 0),
+               rds = buf_rsrc(df, (ta.dfmode & DF_STORE) ? 9 * N : 0),
+               rn = buf_rsrc(fn, (ta.dfmode & F_STORE) ? 9 * N : 0);
   const unsigned fb = (unsigned)(N * (long long)sizeof(R));
   const bool yin = j >= ta.mc[1] && j < ta.mrb[1];
   const bool zin = k >= ta.mc[2] && k < ta.mrb[2];
@@ -1527,13 +1532,8 @@ __global__ void __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(WP
     // df of this plane is first needed at the end of the update: loading it
     // here rather than a plane ahead keeps it out of the loop-carried state
     R odf[9];
-    if (ta.dfmode & DF_LOAD) {
 #pragma unroll
-      for (int q = 0; q < 9; q++) odf[q] = bld<R>(rd, colb, q * fb + plb);
-    } else {
-#pragma unroll
-      for (int q = 0; q < 9; q++) odf[q] = (R)0;
-    }
+    for (int q = 0; q < 9; q++) odf[q] = bld<R>(rd, colb, q * fb + plb);
     if (active) {
       C dxv[6], dyv[6], dzv[6];
       const int jw = __builtin_amdgcn_readfirstlane(j);  // a wave is one y row
@@ -1646,7 +1646,7 @@ __global__ void __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(WP
       }
 #pragma unroll
       for (int q = 0; q < 9; q++) {
-        if (ta.dfmode & DF_STORE) bst<R>(rd, (R)acc[q], colb, q * fb + plb);
+        bst<R>(rds, (R)acc[q], colb, q * fb + plb);
         C v = fo[q] + kB * acc[q];
         if (q >= 3 && ta.roundtrip) {  // set_stress / remove_stress rounding (FP64 only)
           double w = (double)v;
@@ -1656,7 +1656,7 @@ __global__ void __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(WP
           if (ta.hetstress) w -= hs[(q - 3) * N + id];
           v = (C)w;
         }
-        if (ta.dfmode & F_STORE) bst<R>(rn, (R)v, colb, q * fb + plb);
+        bst<R>(rn, (R)v, colb, q * fb + plb);
       }
     }
     lds_barrier();  // every wave is done reading this plane's tile
@@ -1745,7 +1745,12 @@ __global__ void __launch_bounds__(NTMAX) __attribute__((amdgpu_waves_per_eu(WPE)
   const int jc = min(j, ta.hi[1] - 1), tyc = min(ty, TY - 1);
   const unsigned colb = (unsigned)(((long long)jc * sy + k) * (long long)sizeof(R));
   const unsigned psb = (unsigned)(sx * (long long)sizeof(R));
-  const rsrc_t rf = buf_rsrc(f, 9 * N), rd = buf_rsrc(df, 9 * N), rn = buf_rsrc(fn, 9 * N);
+  // df/f traffic a stage does not need goes through a descriptor with no
+  // records: out-of-range buffer loads return 0 and stores are dropped,
+  // without a memory request, and the instruction stream stays the same
+  const rsrc_t rf = buf_rsrc(f, 9 * N), rd = buf_rsrc(df, (ta.dfmode & DF_LOAD) ? 9 * N : 0),
+               rds = buf_rsrc(df, (ta.dfmode & DF_STORE) ? 9 * N : 0),
+               rn = buf_rsrc(fn, (ta.dfmode & F_STORE) ? 9 * N : 0);
   const unsigned fb = (unsigned)(N * (long long)sizeof(R));
   const bool yin = j >= ta.mc[1] && j < ta.mrb[1];
   // z row of this lane: coefficient row and first tap column
@@ -1836,17 +1841,16 @@ __global__ void __launch_bounds__(NTMAX) __attribute__((amdgpu_waves_per_eu(WPE)
     // before any wave starts the arithmetic (measured: 2.5 % faster FP32)
     __builtin_amdgcn_s_setprio(3);
     R odf[9];
-    const bool ldf = ta.dfmode & DF_LOAD;
     if constexpr (sizeof(R) == 4) {
       if (i + 1 < ie) issue(i + 1);
 #pragma unroll
-      for (int q = 0; q < 9; q++) odf[q] = ldf ? bld<R>(rd, colb, q * fb + plb) : (R)0;
+      for (int q = 0; q < 9; q++) odf[q] = bld<R>(rd, colb, q * fb + plb);
     } else {
       // FP64: this plane's df first, so that waiting for it does not wait
       // for the next plane's loads too (vmcnt counts in issue order;
       // measured 1.243 -> 1.232 ms per block launch, FP32 unchanged)
 #pragma unroll
-      for (int q = 0; q < 9; q++) odf[q] = ldf ? bld<R>(rd, colb, q * fb + plb) : (R)0;
+      for (int q = 0; q < 9; q++) odf[q] = bld<R>(rd, colb, q * fb + plb);
       asm volatile("" ::: "memory");
       if (i + 1 < ie) issue(i + 1);
     }
@@ -1943,7 +1947,7 @@ __global__ void __launch_bounds__(NTMAX) __attribute__((amdgpu_waves_per_eu(WPE)
       }
 #pragma unroll
       for (int q = 0; q < 9; q++) {
-        if (ta.dfmode & DF_STORE) bst<R>(rd, (R)acc[q], colb, q * fb + plb);
+        bst<R>(rds, (R)acc[q], colb, q * fb + plb);
         C v = fo[q] + kB * acc[q];
         if (q >= 3 && ta.roundtrip) {
           double w = (double)v;
@@ -1953,7 +1957,7 @@ __global__ void __launch_bounds__(NTMAX) __attribute__((amdgpu_waves_per_eu(WPE)
           if (ta.hetstress) w -= hs[(q - 3) * N + id];
           v = (C)w;
         }
-        if (ta.dfmode & F_STORE) bst<R>(rn, (R)v, colb, q * fb + plb);
+        bst<R>(rn, (R)v, colb, q * fb + plb);
       }
     }
     if constexpr (NBUF == 1) lds_barrier();  // every wave is done reading this plane's tile
@@ -2029,7 +2033,10 @@ __device__ __forceinline__ void tile2d(const Tiled2D& ta, int tile_id, int par, 
   const unsigned colb = (unsigned)jc * (unsigned)sizeof(R);
   const unsigned psb = (unsigned)ta.Ny * (unsigned)sizeof(R);
   const unsigned fb = (unsigned)(N * (long long)sizeof(R));
-  const rsrc_t rf = buf_rsrc(f, 5 * N), rd = buf_rsrc(df, 5 * N), rn = buf_rsrc(fn, 5 * N);
+  // df/f traffic a stage does not need: descriptors with no records (loads
+  // return 0, stores are dropped, no memory request)
+  const rsrc_t rf = buf_rsrc(f, 5 * N), rd = buf_rsrc(df, (dfm & DF_LOAD) ? 5 * N : 0),
+               rds = buf_rsrc(df, (dfm & DF_STORE) ? 5 * N : 0), rn = buf_rsrc(fn, (dfm & F_STORE) ? 5 * N : 0);
   const bool yin = j >= ta.mc[1] && j < ta.mrb[1];
   C kcv[2], kcg2[2], kcl[2], kcg[2], kA, kB;
 #pragma unroll
@@ -2089,7 +2096,7 @@ __device__ __forceinline__ void tile2d(const Tiled2D& ta, int tile_id, int par, 
     if (i + 1 < ie) issue(i + 1);
     R odf[5];
 #pragma unroll
-    for (int q = 0; q < 5; q++) odf[q] = (dfm & DF_LOAD) ? bld<R>(rd, colb, q * fb + plb) : (R)0;
+    for (int q = 0; q < 5; q++) odf[q] = bld<R>(rd, colb, q * fb + plb);
     if (active) {
       C dxv[4], dyv[4];  // d/dx of vx vy sxx sxy; d/dy of vx vy sxy syy
       if (i >= ta.mc[0] && i < ta.mrb[0]) {
@@ -2170,7 +2177,7 @@ __device__ __forceinline__ void tile2d(const Tiled2D& ta, int tile_id, int par, 
       constexpr int s0i[5] = {-1, -1, 0, 1, 2};
 #pragma unroll
       for (int q = 0; q < 5; q++) {
-        if (dfm & DF_STORE) bst<R>(rd, (R)acc[q], colb, q * fb + plb);
+        bst<R>(rds, (R)acc[q], colb, q * fb + plb);
         C v = fo[q] + kB * acc[q];
         if (q >= 2 && ta.roundtrip) {  // set_stress / remove_stress rounding (FP64 only)
           const long long id = (long long)i * ta.Ny + j;
@@ -2181,7 +2188,7 @@ __device__ __forceinline__ void tile2d(const Tiled2D& ta, int tile_id, int par, 
           if (ta.hetstress) w -= ta.hs[(q - 2) * N + id];
           v = (C)w;
         }
-        if (dfm & F_STORE) bst<R>(rn, (R)v, colb, q * fb + plb);
+        bst<R>(rn, (R)v, colb, q * fb + plb);
       }
     }
 #pragma unroll
@@ -2257,7 +2264,10 @@ __device__ __forceinline__ void curv2d_tile(const Tiled2D& ta, const BlockDev& b
   const unsigned colb = (unsigned)jc * (unsigned)sizeof(R);
   const unsigned psb = (unsigned)ta.Ny * (unsigned)sizeof(R);
   const unsigned fb = (unsigned)(N * (long long)sizeof(R));
-  const rsrc_t rf = buf_rsrc(f, NF * N), rd = buf_rsrc(df, NF * N), rn = buf_rsrc(fn, NF * N);
+  // df/f traffic a stage does not need: descriptors with no records (loads
+  // return 0, stores are dropped, no memory request)
+  const rsrc_t rf = buf_rsrc(f, NF * N), rd = buf_rsrc(df, (dfm & DF_LOAD) ? NF * N : 0),
+               rds = buf_rsrc(df, (dfm & DF_STORE) ? NF * N : 0), rn = buf_rsrc(fn, (dfm & F_STORE) ? NF * N : 0);
   const rsrc_t rm = buf_rsrc(mt, 4 * N), rj = buf_rsrc(jac, N);
   const bool yin = j >= ta.mc[1] && j < ta.mrb[1];
   C kv[2], kg2l[2], kg[2], kl[2];
@@ -2351,7 +2361,7 @@ __device__ __forceinline__ void curv2d_tile(const Tiled2D& ta, const BlockDev& b
     if (i + 1 < ie) issue(i + 1);
     R odf[5];  // (the szz rate is assigned, so its old value is not read)
 #pragma unroll
-    for (int q = 0; q < 5; q++) odf[q] = (dfm & DF_LOAD) ? bld<R>(rd, colb, q * fb + plb) : (R)0;
+    for (int q = 0; q < 5; q++) odf[q] = bld<R>(rd, colb, q * fb + plb);
     if (active) {
       C dxv[4], dyv[4];  // d/dx of vx vy Fx Fy; d/dy of vx vy Gx Gy
       if (i >= ta.mc[0] && i < ta.mrb[0]) {
@@ -2431,7 +2441,7 @@ __device__ __forceinline__ void curv2d_tile(const Tiled2D& ta, const BlockDev& b
       constexpr int s0i[5] = {-1, -1, 0, 1, 2};
 #pragma unroll
       for (int q = 0; q < NF; q++) {
-        if (dfm & DF_STORE) bst<R>(rd, (R)acc[q], colb, q * fb + plb);
+        bst<R>(rds, (R)acc[q], colb, q * fb + plb);
         C v = fo[q] + kB * acc[q];
         if (!PL && q >= 2 && ta.roundtrip) {  // set_stress / remove_stress rounding (FP64 elastic only)
           double w = (double)v;
@@ -2441,7 +2451,7 @@ __device__ __forceinline__ void curv2d_tile(const Tiled2D& ta, const BlockDev& b
           if (ta.hetstress) w -= ta.hs[(q - 2) * N + id];
           v = (C)w;
         }
-        if (dfm & F_STORE) bst<R>(rn, (R)v, colb, q * fb + plb);
+        bst<R>(rn, (R)v, colb, q * fb + plb);
       }
     }
 #pragma unroll

@@ -14,6 +14,7 @@ known byte counts (the copy kernel of fdfault_amd/csrc/copy_bw.hip moving
 
 Usage (after the rocprofv3 runs):
   python -m fdfault_amd.profiling stats  <kernel_stats.csv> <out.md> <title> <command>
+  python -m fdfault_amd.profiling stages <kernel_trace.csv> <kernel-substring> <launches-per-stage> <out.md>
   python -m fdfault_amd.profiling traffic <key> <kernel-substring> <fetch.csv> <write.csv>
          <copy_fetch.csv> <copy_write.csv> <copy_mib> <points_per_launch> <out.md> <command...>
 The traffic mode also records the result under <key> in profiles/traffic.json.
@@ -98,6 +99,28 @@ def write_traffic(key, kernel_sub, fetch_csv, write_csv, cfetch_csv, cwrite_csv,
     return rec
 
 
+def append_stage_times(trace_csv, kernel_sub, per_stage, out_md):
+    """Append the kernel's launch durations grouped by RK stage (from a
+    rocprofv3 kernel_trace.csv, launches in order, `per_stage` launches per
+    stage, stages of a step in order) to a summary: the first and last
+    stages of a step move less df traffic than the middle ones."""
+    d = []
+    with open(trace_csv, newline="") as fh:
+        for row in csv.DictReader(fh):
+            if kernel_sub in row["Kernel_Name"]:
+                d.append((int(row["End_Timestamp"]) - int(row["Start_Timestamp"])) / 1e3)
+    nst = 5
+    per_step = nst * per_stage
+    steps = len(d) // per_step
+    with open(out_md, "a") as fh:
+        fh.write("\nLaunch durations by RK stage (us, mean over %d steps x %d launches per stage):\n\n" % (steps, per_stage))
+        fh.write("| stage | df traffic | mean (us) |\n|---|---|---|\n")
+        for st in range(nst):
+            v = [d[k * per_step + st * per_stage + b] for k in range(steps) for b in range(per_stage)]
+            what = "no df load (A = 0)" if st == 0 else ("no df store" if st == nst - 1 else "df load + store")
+            fh.write("| %d | %s | %.1f |\n" % (st, what, sum(v) / len(v)))
+
+
 def lookup_traffic(key):
     """Recorded traffic for a workload key, or None."""
     if not os.path.exists(TRAFFIC_JSON):
@@ -110,6 +133,8 @@ if __name__ == "__main__":
     a = sys.argv[1:]
     if a and a[0] == "stats" and len(a) == 5:
         write_stats(a[1], a[2], a[3], a[4])
+    elif a and a[0] == "stages" and len(a) == 5:
+        append_stage_times(a[1], a[2], int(a[3]), a[4])
     elif a and a[0] == "traffic" and len(a) >= 11:
         print(json.dumps(write_traffic(a[1], a[2], a[3], a[4], a[5], a[6], float(a[7]), int(a[8]), a[9],
                                        " ".join(a[10:]))))
