@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <ctime>
 #include <memory>
+#include <mutex>
 #include <random>
 #include <type_traits>
 #include <string>
@@ -2081,6 +2082,49 @@ void Solver<R>::stz_stats(unsigned long long out[4]) {
 }
 
 // ---- MLP training on the GPU ---------------------------------------------
+// ---- Allocation-free gradient of the proposal network ----------------------
+// Scratch of fdf_mlp_grad_noalloc, a file-scope device array each GPU holds
+// until the process ends: per-sample activations and deltas (H1, H2, D1, D2:
+// MLP_H floats each; DY: one) for up to MLP_NOALLOC_MAXN samples, the MLP_GS
+// weight-gradient slices, and the loss sum.  Host code never passes the
+// array's name to a kernel: it is a host-side stand-in, and the device
+// address comes from hipGetSymbolAddress once per device.
+constexpr long MLP_NOALLOC_MAXN = 4096;
+constexpr size_t MLP_SCR_SAMPLE = 4 * MLP_H + 1;
+constexpr size_t MLP_SCR_GP = (size_t)MLP_NOALLOC_MAXN * MLP_SCR_SAMPLE;
+constexpr size_t MLP_SCR_LOSS = MLP_SCR_GP + (size_t)MLP_GS * MLP_NW;
+constexpr size_t MLP_SCR_FLOATS = MLP_SCR_LOSS + 64;  // 4.36 MB
+static __device__ __align__(256) float scratch[MLP_SCR_FLOATS];
+
+static std::mutex g_scratch_mu;
+static float* g_scratch_dev[64] = {};
+
+// the scratch array's address on the current device (looked up on the
+// first call there)
+static float* scratch_address() {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) throw Error{"fdf_mlp_grad_noalloc: device ordinal out of range"};
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      throw Error{std::string("fdf_mlp_grad_noalloc: hipGetSymbolAddress failed: ") + hipGetErrorString(e)};
+    g_scratch_dev[dev] = static_cast<float*>(p);
+  }
+  return g_scratch_dev[dev];
+}
+
+__global__ void mlp_fill_kernel(float* __restrict__ p, int n, float v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void mlp_loss_mean_kernel(const float* __restrict__ sum, float* __restrict__ out, float n) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) out[0] = sum[0] / n;
+}
+
 struct MlpBufs {
   DBuf<float> P, G, GP, M, V, X, T, H1, H2, D1, D2, DY, loss;
   DBuf<int> idx;
@@ -2415,6 +2459,37 @@ int fdf_mlp_grad(const float* params, const float* X, const float* T, long n, fl
       HIPCHK(hipMemcpy(&l, b.loss.p, sizeof(float), hipMemcpyDeviceToHost));
       *loss = l / (float)n;
     }
+  });
+}
+
+int fdf_mlp_grad_noalloc(int device, const float* params, const float* X, const float* T, long n, float* grad,
+                         float* loss, void* stream) {
+  return guarded([&] {
+    if (!params || !X || !T || !grad) throw Error{"fdf_mlp_grad_noalloc: null argument"};
+    if (n < 1 || n > fdf::MLP_NOALLOC_MAXN)
+      throw Error{"fdf_mlp_grad_noalloc: batch of " + std::to_string(n) + " samples outside 1.." +
+                  std::to_string(fdf::MLP_NOALLOC_MAXN) + " (the fixed scratch holds at most that many)"};
+    HIPCHK(hipSetDevice(device));
+    float* scr = fdf::scratch_address();
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    const int B = (int)n;
+    float* H1 = scr;
+    float* H2 = H1 + (size_t)B * fdf::MLP_H;
+    float* D1 = H2 + (size_t)B * fdf::MLP_H;
+    float* D2 = D1 + (size_t)B * fdf::MLP_H;
+    float* DY = D2 + (size_t)B * fdf::MLP_H;
+    float* GP = scr + fdf::MLP_SCR_GP;
+    float* lsum = scr + fdf::MLP_SCR_LOSS;
+    // the loss sum starts at zero (a fill kernel, not a memset, so that the
+    // sequence can be captured in a graph)
+    hipLaunchKernelGGL(fdf::mlp_fill_kernel, dim3(1), dim3(64), 0, st, lsum, 1, 0.f);
+    hipLaunchKernelGGL(fdf::mlp_fwd_bwd_kernel, dim3((B + 255) / 256), dim3(256), 0, st, params, X, T, nullptr, B, H1,
+                       H2, D1, D2, DY, lsum);
+    hipLaunchKernelGGL(fdf::mlp_grad_kernel, dim3(2 * fdf::MLP_H + 1, fdf::MLP_GS), dim3(256), 0, st, X, nullptr, B,
+                       H1, H2, D1, D2, DY, GP);
+    hipLaunchKernelGGL(fdf::mlp_gsum_kernel, dim3((fdf::MLP_NW + 255) / 256), dim3(256), 0, st, GP, grad);
+    if (loss) hipLaunchKernelGGL(fdf::mlp_loss_mean_kernel, dim3(1), dim3(64), 0, st, lsum, loss, (float)n);
+    HIPCHK(hipGetLastError());
   });
 }
 

@@ -46,7 +46,7 @@ EXPORTS = ["fdf_open", "fdf_step", "fdf_stage", "fdf_run", "fdf_read_field", "fd
            "fdf_host_setup", "fdf_last_error", "fdf_close", "fdf_bulk_kernel_ms", "fdf_fault_kernel_ms",
            "fdf_open_batch", "fdf_batch_step", "fdf_batch_size", "fdf_batch_get_info", "fdf_batch_read_iface",
            "fdf_batch_read_front", "fdf_batch_read_array", "fdf_batch_tuning", "fdf_batch_persistent", "fdf_batch_finish",
-           "fdf_batch_close", "fdf_mlp_param_count", "fdf_mlp_train", "fdf_mlp_grad", "fdf_mlp_predict", "fdf_mlp_predict_mfma",
+           "fdf_batch_close", "fdf_mlp_param_count", "fdf_mlp_train", "fdf_mlp_grad", "fdf_mlp_grad_noalloc", "fdf_mlp_predict", "fdf_mlp_predict_mfma",
            "fdf_stz_set_model", "fdf_stz_log", "fdf_stz_log_read", "fdf_stz_stats"]
 
 _lib = None
@@ -98,6 +98,8 @@ def load():
     fp = C.POINTER(C.c_float)
     L.fdf_mlp_train.argtypes = [dp, C.c_long, C.c_int, C.c_int, C.c_float, C.c_uint, fp, fp]
     L.fdf_mlp_grad.argtypes = [fp, fp, fp, C.c_long, fp, fp]
+    vp = C.c_void_p
+    L.fdf_mlp_grad_noalloc.argtypes = [C.c_int, vp, vp, vp, C.c_long, vp, vp, vp]
     L.fdf_mlp_predict.argtypes = [fp, fp, C.c_long, fp]
     L.fdf_mlp_predict_mfma.argtypes = [fp, fp, C.c_long, fp]
     L.fdf_stz_set_model.argtypes = [C.c_void_p, fp, C.c_int]
@@ -303,6 +305,14 @@ def mlp_grad(P, X, T):
     if load().fdf_mlp_grad(_fp(P), _fp(X), _fp(T), X.shape[0], _fp(G), _fp(loss)) != 0:
         raise _err()
     return G, float(loss[0])
+
+
+def mlp_grad_noalloc(device, params_ptr, X_ptr, T_ptr, n, grad_ptr, loss_ptr=None, stream=None):
+    """fdf_mlp_grad_noalloc on device buffers given by address (params 4555,
+    X n x 4, T n, grad 4545, loss 1 floats); queued on `stream` (a
+    hipStream_t address, None for the null stream)."""
+    if load().fdf_mlp_grad_noalloc(device, params_ptr, X_ptr, T_ptr, n, grad_ptr, loss_ptr, stream) != 0:
+        raise _err()
 
 
 def mlp_predict(P, X, mfma=False):

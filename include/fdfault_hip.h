@@ -169,6 +169,17 @@ int fdf_mlp_train(const double* rows, long n, int epochs, int batch, float lr, u
 /* Gradient (4545 floats) and mean loss of one batch of n normalised samples,
  * X[n][4], T[n], computed by the GPU training kernels. */
 int fdf_mlp_grad(const float* params, const float* X, const float* T, long n, float* grad, float* loss);
+/* The gradient of fdf_mlp_grad without allocating: no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight training or serving loops).  params, X[n][4], T[n],
+ * grad[4545] and loss[1] (mean loss; may be NULL) are device pointers on
+ * `device`; the kernels are queued on `stream` (a hipStream_t, NULL for the
+ * null stream) and the call returns without waiting.  Working memory is a
+ * fixed 4.36 MB array that each GPU holds until the process ends, so n is
+ * limited to 1..4096 (larger batches return an error); calls that overlap on
+ * one GPU share that array and race. */
+int fdf_mlp_grad_noalloc(int device, const float* params, const float* X, const float* T, long n, float* grad,
+                         float* loss, void* stream);
 /* Normalised network outputs for n normalised inputs (GPU). */
 int fdf_mlp_predict(const float* params, const float* X, long n, float* Y);
 /* The same with the evaluation the fault kernel uses: the hidden 64 x 64
