@@ -418,7 +418,7 @@ def main():
     ap.add_argument("--nz", type=int, default=201)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=20)
-    ap.add_argument("--workload", choices=["planar3d", "ensemble", "stz3d", "planar2d", "rough2d"],
+    ap.add_argument("--workload", choices=["planar3d", "ensemble", "stz3d", "planar2d", "rough2d", "drift"],
                     default="planar3d")
     ap.add_argument("--compare-precision", action="store_true",
                     help="planar3d: FP32 vs FP64 errors after --compare-steps steps")
@@ -445,6 +445,9 @@ def main():
     if args.workload in ("planar2d", "rough2d"):
         single_workload(args)
         return
+    if args.workload == "drift":
+        fp64_drift(args)
+        return
     if args.workload == "planar3d" and args.compare_precision:
         compare_precision(args)
         return
@@ -462,13 +465,26 @@ def main():
     headline = args.precision
     other = "fp32" if headline == "fp64" else "fp64"
     workload = "3d_planar_fault_slipweak_2x%dx%dx%d" % (args.nx, args.ny, args.nz)
-    res, err, cpu = {}, None, None
+    res, err, cpu, exact = {}, None, None, None
     with tempfile.TemporaryDirectory() as d:
         path = workload_3d(d, args.nx, args.ny, args.nz, sbporder=args.sbporder)
         print("workload written", flush=True)
         for pname in ((headline,) if args.only else (headline, other)):
             res[pname] = run_planar(lib, path, pname, args, workload)
             print(pname, {k: v for k, v in res[pname].items() if k != "roofline"}, flush=True)
+        exact = None
+        if not args.only:
+            g = lib.Solver(path, exact=True)
+            assert g.info.tiled_blocks == 0
+            g.step(2)
+            t0 = time.perf_counter()
+            g.step(10)
+            te = time.perf_counter() - t0
+            exact = {"value": round(g.N * g.nstages * 10 / te / 1e6, 2), "ms_per_step": round(te / 10 * 1e3, 3),
+                     "steps": 10, "note": "FP64 with the reference's per-point association in every kernel "
+                                          "(bit-faithful mode), eager launches"}
+            g.close()
+            print("exact fp64", exact, flush=True)
         if args.check_steps > 0 and not args.only:
             err = fp32_error(lib, path, args.check_steps)
             print("fp32 vs fp64 after %d steps: %s" % (args.check_steps, err), flush=True)
@@ -495,6 +511,8 @@ def main():
         "cpu_baseline": cpu,
         "%s_path" % other: {"value": o["value"], "ms_per_step": o["ms_per_step"], "roofline": o["roofline"]} if o else None,
         "fp32_over_fp64": round(res["fp32"]["value"] / res["fp64"]["value"], 3) if o else None,
+        "fp64_bit_faithful_path": exact,
+        "fp64_fast_vs_bit_faithful_drift": "profiles/fp64_fast_vs_exact_drift_full_runs.md",
         "error_fp32_vs_fp64": err,
     }
     print(json.dumps(line), flush=True)
@@ -537,6 +555,53 @@ def run_planar(lib, path, pname, args, workload):
         roof["frac_of_stream36"] = round(achieved / ref["stream36_GBs"], 4)
     return {"value": round(npts * nst * args.steps / t / 1e6, 2), "ms_per_step": round(t / args.steps * 1e3, 3),
             "points": npts, "rk_stages": nst, "roofline": roof}
+
+
+def fp64_drift(args):
+    """FP64 fast kernels (block-constant metric, fused multiply-adds) against
+    the bit-faithful FP64 path over the full run of the 2-D planar workload
+    (2x801x401, 2000 steps) and the 3-D planar workload (2x401x201x201, 1500
+    steps): max relative difference of the fields, slip, slip rate and
+    traction, and of the rupture times in time steps; also the bit-faithful
+    path's throughput (whole steps, eager launches)."""
+    from fdfault_amd import lib
+    out = {}
+    rel = lambda a, b: float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+    with tempfile.TemporaryDirectory() as d:
+        cases = (("2d_planar_2x801x401", workload_2d(d, nt=2000), 2000),
+                 ("3d_planar_2x401x201x201", workload_3d(d, sbporder=3), 1500))
+        for name, path, nt in cases:
+            res = {}
+            for exact in (False, True):
+                g = lib.Solver(path, exact=exact, graph=not exact)
+                t0 = time.perf_counter()
+                done = 0
+                while done < nt:
+                    k = min(100, nt - done)
+                    g.step(k)
+                    done += k
+                    print(name, "exact" if exact else "fast", "step", done, flush=True)
+                t = time.perf_counter() - t0
+                res[exact] = dict(f=g.array("f"), U=g.iface(0, "U"), V=g.iface(0, "V"), S=g.iface(0, "S"),
+                                  front=g.front(0), dt=g.dt, mpts=g.N * g.nstages * nt / t / 1e6,
+                                  tiled=g.info.tiled_blocks, blocks=g.info.nblocks)
+                g.close()
+            fa, ex = res[False], res[True]
+            assert fa["tiled"] == fa["blocks"] and ex["tiled"] == 0
+            both = (fa["front"] >= 0) & (ex["front"] >= 0)
+            out[name] = {
+                "steps": nt,
+                "max_field_rel_diff": max(rel(fa["f"][q], ex["f"][q]) for q in range(ex["f"].shape[0])),
+                "slip_rel_diff": rel(fa["U"], ex["U"]), "slip_rate_rel_diff": rel(fa["V"], ex["V"]),
+                "traction_rel_diff": rel(fa["S"], ex["S"]),
+                "front_diff_steps": float(np.abs(fa["front"][both] - ex["front"][both]).max() / ex["dt"])
+                if both.any() else 0.,
+                "front_points_ruptured_in_one_only": int(((fa["front"] >= 0) != (ex["front"] >= 0)).sum()),
+                "ruptured_points": int(both.sum()),
+                "fast_mpts": round(fa["mpts"], 1), "exact_mpts": round(ex["mpts"], 1),
+            }
+            print(name, json.dumps(out[name]), flush=True)
+    print(json.dumps({"workload": "fp64_fast_vs_exact_drift", "results": out}), flush=True)
 
 
 def stream_reference(points, w):
