@@ -418,7 +418,7 @@ def main():
     ap.add_argument("--nz", type=int, default=201)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=20)
-    ap.add_argument("--workload", choices=["planar3d", "ensemble", "stz3d", "planar2d", "rough2d", "drift"],
+    ap.add_argument("--workload", choices=["planar3d", "ensemble", "stz3d", "planar2d", "rough2d", "drift", "setup"],
                     default="planar3d")
     ap.add_argument("--compare-precision", action="store_true",
                     help="planar3d: FP32 vs FP64 errors after --compare-steps steps")
@@ -447,6 +447,9 @@ def main():
         return
     if args.workload == "drift":
         fp64_drift(args)
+        return
+    if args.workload == "setup":
+        setup_times(args)
         return
     if args.workload == "planar3d" and args.compare_precision:
         compare_precision(args)
@@ -555,6 +558,49 @@ def run_planar(lib, path, pname, args, workload):
         roof["frac_of_stream36"] = round(achieved / ref["stream36_GBs"], 4)
     return {"value": round(npts * nst * args.steps / t / 1e6, 2), "ms_per_step": round(t / args.steps * 1e3, 3),
             "points": npts, "rk_stages": nst, "roofline": roof}
+
+
+def setup_times(args):
+    """Open time (fdf_open / fdf_open_batch: parse, grid, metric, Jacobian,
+    normals, dt, upload, t = 0 interface solve) with the grid built on the
+    device and on the host, for the 3-D planar workload in FP64 and FP32 and
+    for the 64-instance ensemble (bulk tile fixed, so no autotuning is
+    timed)."""
+    from fdfault_amd import lib
+    from fdfault_amd.ensemble import ensemble_workload
+    out = {}
+
+    def timed(fn, host):
+        if host:
+            os.environ["FDF_HOST_SETUP"] = "1"
+        else:
+            os.environ.pop("FDF_HOST_SETUP", None)
+        best = 1e30
+        for _ in range(2):
+            t0 = time.perf_counter()
+            h = fn()
+            best = min(best, time.perf_counter() - t0)
+            h.close()
+        os.environ.pop("FDF_HOST_SETUP", None)
+        return round(best, 3)
+
+    threads, host_desc = cpu_threads()
+    with tempfile.TemporaryDirectory() as d:
+        path = workload_3d(d, args.nx, args.ny, args.nz, sbporder=args.sbporder)
+        lib.Solver(path).close()  # first HIP use (runtime and code-object load) outside the timing
+        for pname, prec in (("fp64", lib.FP64), ("fp32", lib.FP32)):
+            for host in (False, True):
+                key = "3d_planar_2x%dx%dx%d_%s_%s" % (args.nx, args.ny, args.nz, pname, "host" if host else "device")
+                out[key] = timed(lambda: lib.Solver(path, precision=prec), host)
+                print(key, out[key], flush=True)
+        ed = os.path.join(d, "ens")
+        paths, shapes = ensemble_workload(ed, n=args.instances, nt=args.ens_nt)
+        os.environ["FDF_BATCH_TW"], os.environ["FDF_BATCH_CX"] = "128", "4"
+        for host in (False, True):
+            key = "ensemble_%d_instances_fp32_%s" % (args.instances, "host" if host else "device")
+            out[key] = timed(lambda: lib.Batch(paths, precision=lib.FP32, graph=True), host)
+            print(key, out[key], flush=True)
+    print(json.dumps({"workload": "open_time_seconds", "host": host_desc, "results": out}), flush=True)
 
 
 def fp64_drift(args):

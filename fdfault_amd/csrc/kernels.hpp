@@ -2549,4 +2549,166 @@ __global__ void face_map_kernel(FaceDev F, GridDev g, double dt, int n0, int nr,
   for (int p = 0; p < nr; p++) out[((long long)u * nr + p) * NC + q] = acc[p];
 }
 
+// ---------------------------------------------------------------------------
+// Device-side grid set-up (block.cpp:769-1046, block.cpp:368-403): the
+// transfinite interpolation of a block's face surfaces, the derivatives
+// dX_c/dxi_d by the SBP operator along grid lines (one-sided rows near block
+// edges), the Jacobian and the inverse metric.  The expressions and their
+// order are those of the host set-up (FP64, no contraction), so the arrays
+// are bit-identical to it.
+struct SurfDev {  // a face surface [ndim][n0][n1]
+  const double* p;
+  int n0, n1;
+};
+struct GridBlockDev {
+  SurfDev F[6];
+  int nx[3], xm[3];
+  double dx[3];
+  int ndim, Ny, Nz;
+  long long N;
+};
+
+__device__ __forceinline__ double surf_at(const SurfDev& s, int c, int i, int j) {
+  return s.p[((long long)c * s.n0 + i) * s.n1 + j];
+}
+
+__global__ void __launch_bounds__(256) grid_tfi_kernel(GridBlockDev g, double* __restrict__ x) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long nb = (long long)g.nx[0] * g.nx[1] * g.nx[2];
+  if (t >= nb) return;
+  const int k = (int)(t % g.nx[2]);
+  const int j = (int)((t / g.nx[2]) % g.nx[1]);
+  const int i = (int)(t / ((long long)g.nx[1] * g.nx[2]));
+  const int ny = g.nx[1], nz = g.nx[2];
+  const SurfDev* F = g.F;
+  const long long gi = ((long long)(g.xm[0] + i) * g.Ny + (g.xm[1] + j)) * g.Nz + (g.xm[2] + k);
+  for (int c = 0; c < g.ndim; c++) {
+    const double p = (double)i * g.dx[0], q = (double)j * g.dx[1], r = (double)k * g.dx[2];
+    double v = ((1. - p) * surf_at(F[0], c, j, k) + p * surf_at(F[1], c, j, k) + (1. - q) * surf_at(F[2], c, i, k) +
+                q * surf_at(F[3], c, i, k));
+    if (g.ndim == 3) v += (1. - r) * surf_at(F[4], c, i, j) + r * surf_at(F[5], c, i, j);
+    v -= ((1. - q) * (1. - p) * surf_at(F[0], c, 0, k) + (1. - q) * p * surf_at(F[1], c, 0, k) +
+          q * (1. - p) * surf_at(F[0], c, ny - 1, k) + q * p * surf_at(F[1], c, ny - 1, k));
+    if (g.ndim == 3) {
+      v -= ((1. - p) * (1. - r) * surf_at(F[0], c, j, 0) + p * (1. - r) * surf_at(F[1], c, j, 0) +
+            (1. - q) * (1. - r) * surf_at(F[2], c, i, 0) + q * (1. - r) * surf_at(F[3], c, i, 0) +
+            (1. - p) * r * surf_at(F[0], c, j, nz - 1) + p * r * surf_at(F[1], c, j, nz - 1) +
+            (1. - q) * r * surf_at(F[2], c, i, nz - 1) + q * r * surf_at(F[3], c, i, nz - 1));
+      v += ((1. - p) * (1. - q) * (1. - r) * surf_at(F[0], c, 0, 0) + p * (1. - q) * (1. - r) * surf_at(F[1], c, 0, 0) +
+            (1. - p) * q * (1. - r) * surf_at(F[0], c, ny - 1, 0) + (1. - p) * (1. - q) * r * surf_at(F[0], c, 0, nz - 1) +
+            p * q * (1. - r) * surf_at(F[1], c, ny - 1, 0) + p * (1. - q) * r * surf_at(F[1], c, 0, nz - 1) +
+            (1. - p) * q * r * surf_at(F[0], c, ny - 1, nz - 1) + p * q * r * surf_at(F[1], c, ny - 1, nz - 1));
+    }
+    x[(long long)c * g.N + gi] = v;
+  }
+}
+
+// metric derivatives, Jacobian and inverse metric of one block (SBP table
+// from c_fd, sbporder s); *bad is set where the Jacobian is not positive
+__global__ void __launch_bounds__(256) grid_metric_kernel(GridBlockDev g, int s, const double* __restrict__ x,
+                                                          double* __restrict__ metric, double* __restrict__ jac,
+                                                          int* __restrict__ bad) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long nb = (long long)g.nx[0] * g.nx[1] * g.nx[2];
+  if (t >= nb) return;
+  const int loc[3] = {(int)(t / ((long long)g.nx[1] * g.nx[2])), (int)((t / g.nx[2]) % g.nx[1]), (int)(t % g.nx[2])};
+  const long long gstride[3] = {(long long)g.Ny * g.Nz, (long long)g.Nz, 1};
+  const long long gi = (g.xm[0] + loc[0]) * gstride[0] + (long long)(g.xm[1] + loc[1]) * gstride[1] + (g.xm[2] + loc[2]);
+  double J9[3][3];
+  for (int c = 0; c < 3; c++)
+    for (int d = 0; d < 3; d++) J9[c][d] = (c == d) ? 1. : 0.;
+  const int edge = 2 * (s - 1);
+  for (int d = 0; d < g.ndim; d++) {
+    const int pos = loc[d], n = g.nx[d];
+    const long long line0 = gi - (long long)pos * gstride[d];
+    for (int c = 0; c < g.ndim; c++) {
+      const double* X = x + (long long)c * g.N + line0;
+      double acc = 0.;
+      if (pos < edge) {
+        for (int tp = 0; tp < 3 * (s - 1); tp++) acc += c_fd[pos + 1][tp] * X[(long long)tp * gstride[d]] / g.dx[d];
+      } else if (pos < n - edge) {
+        for (int tp = 0; tp < 2 * s - 1; tp++)
+          acc += c_fd[0][tp] * X[(long long)(pos - s + 1 + tp) * gstride[d]] / g.dx[d];
+      } else {
+        for (int tp = 0; tp < 3 * (s - 1); tp++) acc -= c_fd[n - pos][tp] * X[(long long)(n - 1 - tp) * gstride[d]] / g.dx[d];
+      }
+      J9[c][d] = acc;
+    }
+  }
+  auto X = [&](int c, int d) { return J9[c][d]; };
+  const double jc = (X(0, 0) * (X(1, 1) * X(2, 2) - X(1, 2) * X(2, 1)) - X(1, 0) * (X(0, 1) * X(2, 2) - X(0, 2) * X(2, 1)) +
+                     X(2, 0) * (X(0, 1) * X(1, 2) - X(0, 2) * X(1, 1)));
+  if (!(jc > 0.)) atomicOr(bad, 1);
+  jac[gi] = jc;
+  for (int l = 0; l < g.ndim; l++)
+    for (int m = 0; m < g.ndim; m++)
+      metric[(long long)(l * g.ndim + m) * g.N + gi] =
+          ((X((m + 1) % 3, (l + 1) % 3) * X((m + 2) % 3, (l + 2) % 3) - X((m + 1) % 3, (l + 2) % 3) * X((m + 2) % 3, (l + 1) % 3)) / jc);
+}
+
+// grid values at n global points: x[q][ndim], metric[q][ndim^2], jac[q]
+__global__ void grid_gather_kernel(const long long* __restrict__ idx, int n, int ndim, long long N,
+                                   const double* __restrict__ x, const double* __restrict__ metric,
+                                   const double* __restrict__ jac, double* __restrict__ ox, double* __restrict__ om,
+                                   double* __restrict__ oj) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const long long g = idx[q];
+  for (int c = 0; c < ndim; c++) ox[(long long)q * ndim + c] = x[c * N + g];
+  for (int lm = 0; lm < ndim * ndim; lm++) om[(long long)q * ndim * ndim + lm] = metric[lm * N + g];
+  oj[q] = jac[g];
+}
+
+// max over the block box [lo, hi) and directions l of cs |grad xi_l| / dx_l
+// (block.cpp:368-403): the values are non-negative, so the largest bit
+// pattern is the largest value
+__global__ void __launch_bounds__(256) grid_worst_kernel(int lo0, int lo1, int lo2, int n0, int n1, int n2, int ndim,
+                                                         int Ny, int Nz, long long N, double dx0, double dx1, double dx2,
+                                                         double cs, const double* __restrict__ metric,
+                                                         const double* __restrict__ mat,
+                                                         unsigned long long* __restrict__ out) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long nb = (long long)n0 * n1 * n2;
+  double w = 0.;
+  if (t < nb) {
+    const int k = (int)(t % n2), j = (int)((t / n2) % n1), i = (int)(t / ((long long)n1 * n2));
+    const long long g = ((long long)(lo0 + i) * Ny + (lo1 + j)) * Nz + (lo2 + k);
+    const double dx[3] = {dx0, dx1, dx2};
+    for (int l = 0; l < ndim; l++) {
+      double v = 0.;
+      for (int m = 0; m < ndim; m++) {
+        const double q = metric[(long long)(l * ndim + m) * N + g];
+        v += q * q;
+      }
+      const double c = mat ? sqrt(mat[2 * N + g] / mat[g]) : cs;
+      v = c * sqrt(v) / dx[l];
+      if (v > w) w = v;
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) w = fmax(w, __shfl_xor(w, o, 64));
+  if ((threadIdx.x & 63) == 0) atomicMax(out, (unsigned long long)__double_as_longlong(w));
+}
+
+// whether a block's metric is diagonal and uniform and its Jacobian uniform
+// (the host check of block_metric_uniform: 1e-9 relative of the centre
+// point's values); *bad is set otherwise
+__global__ void __launch_bounds__(256) grid_uniform_kernel(int lo0, int lo1, int lo2, int n0, int n1, int n2, int ndim,
+                                                           int Ny, int Nz, long long N, double jc, double md0,
+                                                           double md1, double md2, const double* __restrict__ metric,
+                                                           const double* __restrict__ jac, int* __restrict__ bad) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long nb = (long long)n0 * n1 * n2;
+  if (t >= nb) return;
+  const int k = (int)(t % n2), j = (int)((t / n2) % n1), i = (int)(t / ((long long)n1 * n2));
+  const long long g = ((long long)(lo0 + i) * Ny + (lo1 + j)) * Nz + (lo2 + k);
+  const double mdiag[3] = {md0, md1, md2};
+  bool b = fabs(jac[g] - jc) > 1e-9 * fabs(jc);
+  for (int l = 0; l < ndim; l++)
+    for (int m = 0; m < ndim; m++) {
+      const double v = metric[(long long)(l * ndim + m) * N + g];
+      if (l == m ? fabs(v - mdiag[l]) > 1e-9 * fabs(mdiag[l]) : fabs(v) > 1e-9 * fabs(mdiag[l])) b = true;
+    }
+  if (b) atomicOr(bad, 1);
+}
+
 }  // namespace fdf

@@ -78,6 +78,124 @@ struct OutFile {
   FILE* time = nullptr;
 };
 
+// Device-side grid set-up (block.cpp:769-1046, problem.cpp:94-134): the
+// problem compiler hands each block's face surfaces here; the grid, metric
+// and Jacobian are built by kernels into FP64 device arrays that the solver
+// then adopts, and the host reads back only the points it needs.
+class DeviceGrid : public GridBuilder {
+ public:
+  DBuf<double> x, metric, jac;  // [ndim][N], [ndim^2][N], [N]
+  DBuf<double> mat;             // heterogeneous material (dt reduction)
+  DBuf<int> bad;
+  hipStream_t stream = nullptr;
+  bool table_set = false;
+
+  explicit DeviceGrid(hipStream_t s) : stream(s) {}
+
+  void build_block(const SetupH& S, const BlockH& B, const std::vector<double>* surf, const int* n0,
+                   const int* n1) override {
+    if (!x.p) {
+      x.alloc((size_t)S.ndim * S.N);
+      metric.alloc((size_t)S.ndim * S.ndim * S.N);
+      jac.alloc(S.N);
+      bad.alloc(1);
+      bad.zero(stream);
+    }
+    if (!table_set) {
+      HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(c_fd), S.fd.c, sizeof(S.fd.c)));
+      table_set = true;
+    }
+    GridBlockDev g{};
+    DBuf<double> fb[6];
+    for (int f = 0; f < 2 * S.ndim; f++) {
+      fb[f].upload(surf[f]);
+      g.F[f] = SurfDev{fb[f].p, n0[f], n1[f]};
+    }
+    for (int d = 0; d < 3; d++) {
+      g.nx[d] = B.nx[d];
+      g.xm[d] = B.xm[d];
+      g.dx[d] = B.dx[d];
+    }
+    g.ndim = S.ndim;
+    g.Ny = S.nx[1];
+    g.Nz = S.nx[2];
+    g.N = (long long)S.N;
+    const long long nb = (long long)B.nx[0] * B.nx[1] * B.nx[2];
+    const dim3 grid((unsigned)((nb + 255) / 256));
+    hipLaunchKernelGGL(grid_tfi_kernel, grid, dim3(256), 0, stream, g, x.p);
+    HIPCHK(hipGetLastError());
+    // the metric reads x along the block's own lines only, so one block's
+    // grid is complete before its metric is formed (same stream)
+    hipLaunchKernelGGL(grid_metric_kernel, grid, dim3(256), 0, stream, g, S.fd.s, x.p, metric.p, jac.p, bad.p);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(stream));  // the surface buffers are freed on return
+  }
+
+  bool jacobians_positive() override {
+    int b = 0;
+    HIPCHK(hipStreamSynchronize(stream));
+    HIPCHK(hipMemcpy(&b, bad.p, sizeof(int), hipMemcpyDeviceToHost));
+    return b == 0;
+  }
+
+  void gather(const SetupH& S, const std::vector<size_t>& idx, double* ox, double* om, double* oj) override {
+    const int n = (int)idx.size();
+    if (!n) return;
+    std::vector<long long> id(idx.begin(), idx.end());
+    DBuf<long long> di;
+    di.upload(id);
+    DBuf<double> dx, dm, dj;
+    dx.alloc((size_t)n * S.ndim);
+    dm.alloc((size_t)n * S.ndim * S.ndim);
+    dj.alloc((size_t)n);
+    hipLaunchKernelGGL(grid_gather_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, di.p, n, S.ndim,
+                       (long long)S.N, x.p, metric.p, jac.p, dx.p, dm.p, dj.p);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(stream));
+    HIPCHK(hipMemcpy(ox, dx.p, dx.n * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(om, dm.p, dm.n * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(oj, dj.p, dj.n * sizeof(double), hipMemcpyDeviceToHost));
+  }
+
+  double block_worst(const SetupH& S, const BlockH& B) override {
+    if (S.hetmat && !mat.p) mat.upload(S.mat);
+    DBuf<unsigned long long> w;
+    w.alloc(1);
+    w.zero(stream);
+    const long long nb = (long long)B.nx[0] * B.nx[1] * B.nx[2];
+    hipLaunchKernelGGL(grid_worst_kernel, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, stream, B.mlb[0],
+                       B.mlb[1], B.mlb[2], B.prb[0] - B.mlb[0], B.prb[1] - B.mlb[1], B.prb[2] - B.mlb[2], S.ndim,
+                       S.nx[1], S.nx[2], (long long)S.N, B.dx[0], B.dx[1], B.dx[2], B.cs, metric.p,
+                       S.hetmat ? mat.p : nullptr, w.p);
+    HIPCHK(hipGetLastError());
+    unsigned long long bits = 0;
+    HIPCHK(hipStreamSynchronize(stream));
+    HIPCHK(hipMemcpy(&bits, w.p, sizeof(bits), hipMemcpyDeviceToHost));
+    double v;
+    memcpy(&v, &bits, sizeof(v));
+    return v;
+  }
+
+  // block_metric_uniform (setup.cpp) on the device arrays
+  bool uniform(const SetupH& S, const BlockH& B, double mdiag[3], const double* metric_p, const double* jac_p) {
+    const size_t gc = S.idx((B.mlb[0] + B.prb[0]) / 2, (B.mlb[1] + B.prb[1]) / 2, (B.mlb[2] + B.prb[2]) / 2);
+    const double jc = S.jac_at(gc);
+    for (int d = 0; d < 3; d++) mdiag[d] = d < S.ndim ? S.metric_at(d * S.ndim + d, gc) : 0.;
+    DBuf<int> flag;
+    flag.alloc(1);
+    flag.zero(stream);
+    const long long nb = (long long)B.nx[0] * B.nx[1] * B.nx[2];
+    hipLaunchKernelGGL(grid_uniform_kernel, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, stream, B.mlb[0],
+                       B.mlb[1], B.mlb[2], B.prb[0] - B.mlb[0], B.prb[1] - B.mlb[1], B.prb[2] - B.mlb[2], S.ndim,
+                       S.nx[1], S.nx[2], (long long)S.N, jc, mdiag[0], mdiag[1], mdiag[2], metric_p, jac_p, flag.p);
+    HIPCHK(hipGetLastError());
+    int b = 0;
+    HIPCHK(hipStreamSynchronize(stream));
+    HIPCHK(hipMemcpy(&b, flag.p, sizeof(int), hipMemcpyDeviceToHost));
+    return b == 0;
+  }
+};
+
 class SolverBase {
  public:
   virtual ~SolverBase() {}
@@ -141,6 +259,13 @@ class Solver : public SolverBase {
   int tile_ty = 4;
   void launch_plastic(int b);
   void write_outputs(int tstep);
+  // device-side set-up: the builder, and the FP64 grid arrays on the device
+  // (the solver's own for FP64, the builder's for FP32)
+  std::unique_ptr<DeviceGrid> dgrid;
+  const double* metric64 = nullptr;
+  const double* jac64 = nullptr;
+  // component c of the coordinates over an index box (C order)
+  void coords_box(int c, const int lo[3], const int hi[3], const int st[3], std::vector<double>& out);
   void sync_check();
 
   hipStream_t stream = nullptr;
@@ -361,8 +486,13 @@ static bool compact_face_sat2(const double* M, int d, double* fc) {
 template <typename R>
 Solver<R>::Solver(const std::string& path, const fdf_opts& o) {
   opts = o;
-  compile_problem(path, S, o.write_outputs != 0, o.quiet != 0);
   HIPCHK(hipSetDevice(o.device));
+  HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  // grid, metric and Jacobian built on the device (FDF_HOST_SETUP=1: on the
+  // host, then uploaded)
+  const char* eh = getenv("FDF_HOST_SETUP");
+  if (!(eh && atoi(eh) != 0)) dgrid.reset(new DeviceGrid(stream));
+  compile_problem(path, S, o.write_outputs != 0, o.quiet != 0, dgrid.get());
   // measured best tile heights on a 401x201x201 block: 4 rows (FP64), 8 (FP32)
   const int ty_default = sizeof(R) == 8 ? 4 : 8;
   tile_ty = ty_default;
@@ -371,7 +501,6 @@ Solver<R>::Solver(const std::string& path, const fdf_opts& o) {
   if (const char* e = getenv("FDF_TILE2_TW")) tile2_tw = atoi(e), tile2_env = true;
   if (tile2_tw != 64 && tile2_tw != 128 && tile2_tw != 256) tile2_tw = 64;
   if (const char* e = getenv("FDF_TILE2_CX")) tile2_cx = atoi(e) > 0 ? atoi(e) : 2, tile2_env = true;
-  HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
   for (int e = 0; e < 2 * NEV; e++) HIPCHK(hipEventCreate(&evs[e]));
   const size_t N = S.N;
   G.ndim = S.ndim;
@@ -411,9 +540,33 @@ Solver<R>::Solver(const std::string& path, const fdf_opts& o) {
       }
     }
   };
-  upload_bulk(metric, S.metric.data(), S.metric.size());
-  upload_bulk(jac, S.jac.data(), S.jac.size());
-  x.upload(S.x.data(), S.x.size());
+  if (dgrid) {
+    if constexpr (std::is_same<R, double>::value) {
+      metric = std::move(dgrid->metric);
+      jac = std::move(dgrid->jac);
+      metric64 = metric.p;
+      jac64 = jac.p;
+    } else {
+      auto conv = [&](DBuf<R>& dst, const DBuf<double>& src) {
+        dst.alloc(src.n);
+        if (src.n) {
+          hipLaunchKernelGGL(convert_from_double<R>, dim3((unsigned)((src.n + 255) / 256)), dim3(256), 0, stream, src.p,
+                             dst.p, (long long)src.n);
+          HIPCHK(hipGetLastError());
+        }
+      };
+      conv(metric, dgrid->metric);
+      conv(jac, dgrid->jac);
+      HIPCHK(hipStreamSynchronize(stream));
+      metric64 = dgrid->metric.p;
+      jac64 = dgrid->jac.p;
+    }
+    x = std::move(dgrid->x);
+  } else {
+    upload_bulk(metric, S.metric.data(), S.metric.size());
+    upload_bulk(jac, S.jac.data(), S.jac.size());
+    x.upload(S.x.data(), S.x.size());
+  }
   if (S.hetmat) upload_bulk(mat, S.mat.data(), S.mat.size());
   if (S.hetstress) hs.upload(S.s);
   for (int b = 0; b < 2; b++) {
@@ -488,7 +641,7 @@ Solver<R>::Solver(const std::string& path, const fdf_opts& o) {
                !(getenv("FDF_CURV") && atoi(getenv("FDF_CURV")) == 0);
     for (int d = 0; d < nd && okc; d++) okc = B.mrb[d] > B.mc[d] && B.nx[d] >= 3 * (S.sbporder - 1);
     double mdiag[3] = {0., 0., 0.};
-    if (ok) ok = block_metric_uniform(S, B, mdiag);
+    if (ok) ok = dgrid ? dgrid->uniform(S, B, mdiag, metric64, jac64) : block_metric_uniform(S, B, mdiag);
     Tiled3D ta{};
     Tiled2D t2{};
     if (ok && ok3) {
@@ -923,9 +1076,7 @@ Solver<R>::Solver(const std::string& path, const fdf_opts& o) {
         FILE* xf = fopen((base + "_" + xyz[c] + ".dat").c_str(), "wb");
         if (!xf) throw Error{"Error opening file in outputunit.cpp"};
         std::vector<double> buf;
-        for (int a = U.xm[0]; a <= U.xp[0]; a += U.xs[0])
-          for (int b = U.xm[1]; b <= U.xp[1]; b += U.xs[1])
-            for (int cc = U.xm[2]; cc <= U.xp[2]; cc += U.xs[2]) buf.push_back(S.x[(size_t)c * N + S.idx(a, b, cc)]);
+        coords_box(c, U.xm, U.xp, U.xs, buf);
         fwrite(buf.data(), sizeof(double), buf.size(), xf);
         fclose(xf);
       }
@@ -1432,6 +1583,31 @@ void Solver<R>::write_outputs(int tstep) {
 }
 
 template <typename R>
+void Solver<R>::coords_box(int c, const int lo[3], const int hi[3], const int st[3], std::vector<double>& out) {
+  int n[3];
+  for (int d = 0; d < 3; d++) n[d] = hi[d] < lo[d] ? 0 : (hi[d] - lo[d]) / st[d] + 1;
+  const size_t cnt = (size_t)n[0] * n[1] * n[2];
+  out.clear();
+  if (!cnt) return;
+  if (!dgrid) {
+    out.reserve(cnt);
+    for (int a = lo[0]; a <= hi[0]; a += st[0])
+      for (int b = lo[1]; b <= hi[1]; b += st[1])
+        for (int cc = lo[2]; cc <= hi[2]; cc += st[2]) out.push_back(S.x[(size_t)c * S.N + S.idx(a, b, cc)]);
+    return;
+  }
+  out.resize(cnt);
+  DBuf<double> tmp;
+  tmp.alloc(cnt);
+  hipLaunchKernelGGL((gather_box<double>), dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, stream,
+                     x.p + (size_t)c * S.N, hs.p, 0., -1, G.N, G.Ny, G.Nz, lo[0], lo[1], lo[2], st[0], st[1], st[2],
+                     n[0], n[1], n[2], tmp.p);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(stream));
+  HIPCHK(hipMemcpy(out.data(), tmp.p, cnt * sizeof(double), hipMemcpyDeviceToHost));
+}
+
+template <typename R>
 void Solver<R>::read_field(int field, const int lo[3], const int hi[3], const int st[3], double* h) {
   int n[3];
   for (int d = 0; d < 3; d++) {
@@ -1482,17 +1658,19 @@ void Solver<R>::read_array(int which, double* h, long n) {
       fetch(df.p, (size_t)n, h);
       break;
     case 2:
-      if ((size_t)n != S.x.size()) throw Error{"read_array: size mismatch"};
-      memcpy(h, S.x.data(), n * sizeof(double));
-      break;
     case 3:
-      if ((size_t)n != S.metric.size()) throw Error{"read_array: size mismatch"};
-      memcpy(h, S.metric.data(), n * sizeof(double));
+    case 4: {
+      const size_t want = which == 2 ? (size_t)S.ndim * N : (which == 3 ? (size_t)S.ndim * S.ndim * N : N);
+      if ((size_t)n != want) throw Error{"read_array: size mismatch"};
+      if (dgrid) {
+        const double* src = which == 2 ? x.p : (which == 3 ? metric64 : jac64);
+        HIPCHK(hipMemcpy(h, src, want * sizeof(double), hipMemcpyDeviceToHost));
+      } else {
+        const double* src = which == 2 ? S.x.data() : (which == 3 ? S.metric.data() : S.jac.data());
+        memcpy(h, src, want * sizeof(double));
+      }
       break;
-    case 4:
-      if ((size_t)n != S.jac.size()) throw Error{"read_array: size mismatch"};
-      memcpy(h, S.jac.data(), n * sizeof(double));
-      break;
+    }
     default:
       throw Error{"read_array: bad array id"};
   }
@@ -1589,9 +1767,8 @@ void Solver<R>::finish() {
       FILE* xf = fopen((pre + "_" + xyz[c] + ".dat").c_str(), "wb");
       if (!xf) throw Error{"Error opening file in outputunit.cpp"};
       std::vector<double> buf;
-      for (int a = I.xm[0]; a <= I.xp[0]; a++)
-        for (int b = I.xm[1]; b <= I.xp[1]; b++)
-          for (int cc = I.xm[2]; cc <= I.xp[2]; cc++) buf.push_back(S.x[(size_t)c * S.N + S.idx(a, b, cc)]);
+      const int one[3] = {1, 1, 1};
+      coords_box(c, I.xm, I.xp, one, buf);
       fwrite(buf.data(), sizeof(double), buf.size(), xf);
       fclose(xf);
     }

@@ -373,7 +373,7 @@ void build_block_grid(SetupH& S, BlockH& B, const SurfView* F) {
 double unit_normal(const SetupH& S, int d, size_t g, double* nrm) {
   double len = 0.;
   for (int k = 0; k < S.ndim; k++) {
-    const double m = S.metric[(size_t)(d * S.ndim + k) * S.N + g];
+    const double m = S.metric_at(d * S.ndim + k, g);
     len += m * m;
     nrm[k] = m;
   }
@@ -409,6 +409,12 @@ double min_dx_over_cs(const SetupH& S) {  // block.cpp:368-403, domain.cpp:199-2
   double best = 0.;
   for (const BlockH& B : S.blocks) {
     double worst = 0.;
+    if (S.gb) {
+      worst = S.gb->block_worst(S, B);
+      const double v = 1. / worst;
+      if (v > 1.e-14 && (best <= 1.e-14 || v < best)) best = v;
+      continue;
+    }
 #pragma omp parallel for schedule(static) reduction(max : worst)
     for (int i = B.mlb[0]; i < B.prb[0]; i++)
       for (int j = B.mlb[1]; j < B.prb[1]; j++)
@@ -434,7 +440,21 @@ double min_dx_over_cs(const SetupH& S) {  // block.cpp:368-403, domain.cpp:199-2
 
 }  // namespace
 
+void SetupH::gather(const std::vector<size_t>& idx) {
+  if (!gb) return;
+  std::vector<size_t> todo;
+  for (size_t g : idx)
+    if (gslot.emplace(g, gslot.size()).second) todo.push_back(g);
+  if (todo.empty()) return;
+  const size_t base = gj.size();
+  gx.resize((base + todo.size()) * ndim);
+  gm.resize((base + todo.size()) * ndim * ndim);
+  gj.resize(base + todo.size());
+  gb->gather(*this, todo, gx.data() + base * ndim, gm.data() + base * ndim * ndim, gj.data() + base);
+}
+
 bool block_metric_uniform(const SetupH& S, const BlockH& B, double mdiag[3]) {
+  if (S.gb) fail("block_metric_uniform: the grid arrays are on the device");
   const int nd = S.ndim;
   const size_t N = S.N;
   const size_t gc = S.idx((B.mlb[0] + B.prb[0]) / 2, (B.mlb[1] + B.prb[1]) / 2, (B.mlb[2] + B.prb[2]) / 2);
@@ -456,7 +476,8 @@ bool block_metric_uniform(const SetupH& S, const BlockH& B, double mdiag[3]) {
   return !bad;
 }
 
-void compile_problem(const std::string& path, SetupH& S, bool outputs, bool quiet) {
+void compile_problem(const std::string& path, SetupH& S, bool outputs, bool quiet, GridBuilder* gb) {
+  S.gb = gb;
   Scanner sc(path);
   // [fdfault.problem], problem.cpp:27-44
   if (!sc.seek("[fdfault.problem]")) fail("Error reading problem from input file");
@@ -510,9 +531,11 @@ void compile_problem(const std::string& path, SetupH& S, bool outputs, bool quie
   }
   S.N = (size_t)S.nx[0] * S.nx[1] * S.nx[2];
   // (blocks tile the grid, so build_block_grid writes every entry)
-  S.x.resize((size_t)S.ndim * S.N);
-  S.metric.resize((size_t)S.ndim * S.ndim * S.N);
-  S.jac.resize(S.N);
+  if (!S.gb) {
+    S.x.resize((size_t)S.ndim * S.N);
+    S.metric.resize((size_t)S.ndim * S.ndim * S.N);
+    S.jac.resize(S.N);
+  }
   if (loadfile != "none") { S.hetstress = true; S.s = load_f8(loadfile, (size_t)S.ns * S.N, "heterogeneous stress"); }
   if (matfile != "none") { S.hetmat = true; S.mat = load_f8(matfile, (size_t)S.nmat * S.N, "heterogeneous material"); }
   if (S.hetmat && S.ndim == 2 && S.mode == 3)
@@ -577,8 +600,39 @@ void compile_problem(const std::string& path, SetupH& S, bool outputs, bool quie
         static const int e1[12] = {1, 3, 1, 3, 0, 2, 0, 2, 0, 2, 0, 2}, e2[12] = {1, 1, 3, 3, 1, 1, 3, 3, 0, 0, 2, 2};
         for (int e = 0; e < (S.ndim == 3 ? 12 : 4); e++)
           if (!edges_match(S, view[s1[e]], e1[e], view[s2[e]], e2[e])) fail("Surface edges do not match in block.cpp");
-        build_block_grid(S, B, view);
+        if (S.gb) {
+          for (int i = 0; i < 3; i++) B.dx[i] = 1. / (double)(B.nx[i] - 1);
+          int n0s[6], n1s[6];
+          for (int f = 0; f < 2 * S.ndim; f++) {
+            n0s[f] = view[f].n0;
+            n1s[f] = view[f].n1;
+          }
+          S.gb->build_block(S, B, surf, n0s, n1s);
+        } else {
+          build_block_grid(S, B, view);
+        }
       }
+  if (S.gb) {
+    if (!S.gb->jacobians_positive()) fail("non-positive Jacobian in block grid (grid is not smooth enough)");
+    // the points whose grid values the host needs: every face point of
+    // every block (normals and SAT scales) and each block's centre point
+    std::vector<size_t> need;
+    for (const BlockH& B : S.blocks) {
+      need.push_back(S.idx((B.mlb[0] + B.prb[0]) / 2, (B.mlb[1] + B.prb[1]) / 2, (B.mlb[2] + B.prb[2]) / 2));
+      for (int f = 0; f < 2 * S.ndim; f++) {
+        const int d = f / 2, a = (d == 0) ? 1 : 0, b = (d == 2) ? 1 : 2;
+        int c[3];
+        c[d] = B.xm[d] + ((f & 1) ? B.nx[d] - 1 : 0);
+        for (int u = 0; u < B.nx[a]; u++)
+          for (int v = 0; v < B.nx[b]; v++) {
+            c[a] = B.xm[a] + u;
+            c[b] = B.xm[b] + v;
+            need.push_back(S.idx(c[0], c[1], c[2]));
+          }
+      }
+    }
+    S.gather(need);
+  }
   // external boundary faces, boundary.cpp:15-240
   for (BlockH& B : S.blocks)
     for (int f = 0; f < 2 * S.ndim; f++) {
@@ -663,7 +717,7 @@ void compile_problem(const std::string& path, SetupH& S, bool outputs, bool quie
         c[d] += 1;
         const size_t g2 = S.idx(c[0], c[1], c[2]);
         for (int k = 0; k < S.ndim; k++) {  // interface.cpp:333-347 edge agreement
-          const double xa = S.x[(size_t)k * S.N + g1], xb = S.x[(size_t)k * S.N + g2];
+          const double xa = S.x_at(k, g1), xb = S.x_at(k, g2);
           if (!(fabs(xa - xb) <= 1.e-8 + 1.e-5 * fabs(xb))) fail("interface block edges do not match");
         }
         double nn[3], nn2[3];

@@ -15,6 +15,7 @@
 #include <new>
 #include <string>
 #include <utility>
+#include <unordered_map>
 #include <vector>
 
 namespace fdf {
@@ -120,6 +121,26 @@ struct OutputH {
   int xm[3], xp[3], xs[3], nx[3];
 };
 
+struct SetupH;
+
+// Device-side grid set-up, implemented by the runtime: the grid arrays (x,
+// metric, Jacobian) of every block are built on the GPU and stay there; the
+// host keeps the values of the points it needs (faces, interfaces, outputs),
+// gathered on request.
+struct GridBuilder {
+  virtual ~GridBuilder() {}
+  // transfinite interpolation of block B from its 2 ndim face surfaces
+  // ([ndim][n0][n1] each) and its metric and Jacobian by the SBP operator
+  virtual void build_block(const SetupH& S, const BlockH& B, const std::vector<double>* surf, const int* n0,
+                           const int* n1) = 0;
+  // false if any Jacobian of the blocks built so far is not positive
+  virtual bool jacobians_positive() = 0;
+  // x (ndim per point), metric (ndim^2) and Jacobian at global points idx
+  virtual void gather(const SetupH& S, const std::vector<size_t>& idx, double* x, double* metric, double* jac) = 0;
+  // max over block B's points and directions l of cs |grad xi_l| / dxi_l
+  virtual double block_worst(const SetupH& S, const BlockH& B) = 0;
+};
+
 struct SetupH {
   std::string name, datadir;
   int nt = 0, ninfo = 1, rkorder = 0;
@@ -134,7 +155,20 @@ struct SetupH {
   bool hetstress = false, hetmat = false;
   double cdiss = 0.;
   size_t N = 0;
-  gvec x, metric, jac;  // every entry is written by the block set-up
+  gvec x, metric, jac;  // host set-up: every entry is written by the block set-up
+  // device set-up: the grid arrays live on the GPU; gx/gm/gj hold the values
+  // of the gathered points, slot gslot[g] of global point g
+  GridBuilder* gb = nullptr;
+  std::unordered_map<size_t, size_t> gslot;
+  std::vector<double> gx, gm, gj;
+  void gather(const std::vector<size_t>& idx);  // adds the points not gathered yet
+  double x_at(int c, size_t g) const {
+    return gb ? gx[gslot.at(g) * ndim + c] : x[(size_t)c * N + g];
+  }
+  double metric_at(int lm, size_t g) const {
+    return gb ? gm[gslot.at(g) * ndim * ndim + lm] : metric[(size_t)lm * N + g];
+  }
+  double jac_at(size_t g) const { return gb ? gj[gslot.at(g)] : jac[g]; }
   std::vector<double> s, mat;
   std::vector<BlockH> blocks;
   std::vector<IfaceH> ifaces;
@@ -146,8 +180,9 @@ struct SetupH {
   size_t idx(int i, int j, int k) const { return ((size_t)i * nx[1] + j) * nx[2] + k; }
 };
 
-// Parse + build; throws Error.  `outputs` parses the output list (else skipped).
-void compile_problem(const std::string& path, SetupH& S, bool outputs, bool quiet);
+// Parse + build; throws Error.  `outputs` parses the output list (else
+// skipped).  With gb the grid arrays are built on the device (GridBuilder).
+void compile_problem(const std::string& path, SetupH& S, bool outputs, bool quiet, GridBuilder* gb = nullptr);
 
 // Whether block B's metric is diagonal and uniform and its Jacobian uniform,
 // to 1e-9 relative (the SBP derivative of TFI coordinates leaves ~1e-12
