@@ -52,9 +52,9 @@ MUTANTS = [
      "o.s1 = (z1 * (c.s1 - z1 * c.v1) + z2 * (c.s2 + z2 * c.v2)) / (z1 + z2);",
      [LAW + "::test_interface_reflection_transmission"]),
     ("iface_normal_penalty_cs", "df[N + id1] -= I.cp1 * h1 * o1.v2;", "df[N + id1] -= I.cs1 * h1 * o1.v2;",
-     [LAW + "::test_energy_rate_nonpositive"]),
+     [LAW + "::test_interface_penalty_one_stage"]),
     ("iface_shear_penalty_cp", "df[3 * N + id1] -= I.cs1 * h1 * o1.s12;", "df[3 * N + id1] -= I.cp1 * h1 * o1.s12;",
-     [LAW + "::test_energy_rate_nonpositive"]),
+     [LAW + "::test_interface_penalty_one_stage"]),
     ("load_ramp_step", "double tfunc(double t) const { return (t >= t0) ? 1. : t / t0; }",
      "double tfunc(double t) const { return (t >= t0) ? 1. : 0.; }", [LAW + "::test_load_time_ramp_and_stage_time"]),
     ("stage_time_no_C", "const double ts = t + P.rk.C[stage] * dt;", "const double ts = t;",

@@ -335,6 +335,60 @@ def test_energy_rate_nonpositive(tmp_path, sbp, bc):
         assert rate <= 1e-12 * scale
 
 
+@pytest.mark.parametrize("sbp", [2, 3, 4])
+def test_interface_penalty_one_stage(tmp_path, sbp):
+    """Penalty scale of the interface (Kozdon, Dunham & Nordstrom 2012, the
+    method of interface.cpp:573-1028): each block holds a uniform state, the
+    outer faces carry no penalty, so after one Euler stage only the
+    interface rows change, by -dt c / (h0 dy) (q - q_hat): c = cp on the
+    normal characteristic (vy, syy; sxx through gamma = lam / (lam + 2G)),
+    c = cs on the tangential one (vx, sxy), q_hat the locked characteristic
+    solution (continuous velocity and traction)."""
+    from test_oracle_pins import fd_tables
+    mats = ((2.67, 32.04, 32.04), (3.0, 10.0, 20.0))
+    nx, ny, dx = 21, 16, 0.1
+    d = os.path.join(str(tmp_path), "pen")
+    os.makedirs(os.path.join(d, "data"), exist_ok=True)
+    p = two_block_fault(ndim=2, mode=2, nx=nx, ny=ny, dx=dx, nt=1, sbporder=sbp, rkorder=1, law="locked", bc="none",
+                        loads=[], front=None, datadir=os.path.join(d, "data") + "/", name="pen")
+    for b in (0, 1):
+        p.blocks[b].update(rho=mats[b][0], lam=mats[b][1], g=mats[b][2])
+    o = O.Oracle(p.write(os.path.join(d, "pen.in")))
+    q1 = np.array([0.3, -0.2, -90., 25., -110.])  # vx vy sxx sxy syy below the interface
+    q2 = np.array([-0.1, 0.4, -95., 18., -100.])
+    f = np.zeros(o.array("f").shape)
+    f[:, :, :ny, 0] = q1[:, None, None]
+    f[:, :, ny:, 0] = q2[:, None, None]
+    o.set_array("f", f)
+    o.set_array("df", np.zeros(o.array("df").shape))
+    o.stage(0, 0)
+    df = o.array("df")[:, :, :, 0]
+    h0 = fd_tables(sbp)[2]
+    h = o.dt / (h0 * dx)
+    (r1, l1, g1), (r2, l2, g2) = mats
+    cs1, cp1 = math.sqrt(g1 / r1), math.sqrt((l1 + 2 * g1) / r1)
+    cs2, cp2 = math.sqrt(g2 / r2), math.sqrt((l2 + 2 * g2) / r2)
+    zs1, zp1, zs2, zp2 = r1 * cs1, r1 * cp1, r2 * cs2, r2 * cp2
+    # locked characteristic solutions: normal (vy, syy) and tangential (vx, sxy)
+    syy = (zp2 * (q1[4] - zp1 * q1[1]) + zp1 * (q2[4] + zp2 * q2[1])) / (zp1 + zp2)
+    vy = (syy - q1[4]) / zp1 + q1[1]
+    sxy = (zs2 * (q1[3] - zs1 * q1[0]) + zs1 * (q2[3] + zs2 * q2[0])) / (zs1 + zs2)
+    vx = (sxy - q1[3]) / zs1 + q1[0]
+    gam1, gam2 = l1 / (l1 + 2 * g1), l2 / (l2 + 2 * g2)
+    exp1 = -h * np.array([cs1 * (q1[0] - vx), cp1 * (q1[1] - vy), cp1 * gam1 * (q1[4] - syy),
+                          cs1 * (q1[3] - sxy), cp1 * (q1[4] - syy)])
+    exp2 = -h * np.array([cs2 * (q2[0] - vx), cp2 * (q2[1] - vy), cp2 * gam2 * (q2[4] - syy),
+                          cs2 * (q2[3] - sxy), cp2 * (q2[4] - syy)])
+    scale = np.abs(exp1).max()
+    inner = slice(2 * (sbp - 1), nx - 2 * (sbp - 1))  # x rows away from the block corners
+    for k in range(5):
+        np.testing.assert_allclose(df[k, inner, ny - 1], exp1[k], rtol=0, atol=1e-12 * scale, err_msg=str(k))
+        np.testing.assert_allclose(df[k, inner, ny], exp2[k], rtol=0, atol=1e-12 * scale, err_msg=str(k))
+    rest = np.ones(2 * ny, bool)
+    rest[[ny - 1, ny]] = False
+    assert np.abs(df[:, :, rest]).max() <= 1e-12 * scale
+
+
 def test_load_time_ramp_and_stage_time(tmp_path):
     """Frictionless fault, zero bulk fields: the hat normal traction is the
     ramped normal load, sn min(t/t0, 1), and the slip rate is |s2| / eta,
