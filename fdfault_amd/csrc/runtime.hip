@@ -2335,10 +2335,22 @@ static void mlp_batch_grad(MlpBufs& b, const int* idx, int B, hipStream_t st) {
 // initial weights, Adam over shuffled minibatches with the learning rate
 // decaying geometrically to lr/10 over the run.  Returns the parameter
 // vector (with normalisation) and the mean loss of each epoch.
-static void mlp_train(const double* rows, long n, int epochs, int B, float lr, unsigned seed, float* params,
+static void mlp_train(const double* rows_in, long n, int epochs, int B, float lr, unsigned seed, float* params,
                       float* epoch_loss) {
   if (n < 1 || epochs < 1 || B < 1) throw Error{"mlp_train: empty data or bad settings"};
   if (B > n) B = (int)n;
+  // the fault kernel logs its rows in atomic order; training sees them in
+  // lexicographic order, so the statistics, the shuffled minibatches and the
+  // trained network depend on the set of rows only
+  std::vector<long> ord((size_t)n);
+  for (long r = 0; r < n; r++) ord[r] = r;
+  std::sort(ord.begin(), ord.end(), [&](long a, long b) {
+    return std::lexicographical_compare(rows_in + a * 5, rows_in + a * 5 + 5, rows_in + b * 5, rows_in + b * 5 + 5);
+  });
+  std::vector<double> sorted_rows((size_t)n * 5);
+  for (long r = 0; r < n; r++)
+    for (int c = 0; c < 5; c++) sorted_rows[(size_t)r * 5 + c] = rows_in[ord[r] * 5 + c];
+  const double* rows = sorted_rows.data();
   std::vector<float> P(MLP_NP, 0.f);
   double mean[5] = {0, 0, 0, 0, 0}, sd[5] = {0, 0, 0, 0, 0};
   for (long r = 0; r < n; r++)

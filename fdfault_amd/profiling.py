@@ -14,6 +14,7 @@ known byte counts (the copy kernel of fdfault_amd/csrc/copy_bw.hip moving
 
 Usage (after the rocprofv3 runs):
   python -m fdfault_amd.profiling stats  <kernel_stats.csv> <out.md> <title> <command>
+  python -m fdfault_amd.profiling line   <bench.log> <out.md> <title> <command>   (append a bench line)
   python -m fdfault_amd.profiling stages <kernel_trace.csv> <kernel-substring> <launches-per-stage> <out.md>
   python -m fdfault_amd.profiling traffic <key> <kernel-substring> <fetch.csv> <write.csv>
          <copy_fetch.csv> <copy_write.csv> <copy_mib> <points_per_launch> <out.md> <command...>
@@ -121,6 +122,36 @@ def append_stage_times(trace_csv, kernel_sub, per_stage, out_md):
             fh.write("| %d | %s | %.1f |\n" % (st, what, sum(v) / len(v)))
 
 
+def _flat(d, pre=""):
+    for k, v in d.items():
+        if isinstance(v, dict):
+            yield from _flat(v, pre + k + ".")
+        else:
+            yield pre + k, v
+
+
+def append_bench_line(log, out_md, title, command):
+    """Append one bench.py JSON line (the last line of `log` that parses) to a
+    summary as a section: the command and every field, nested ones flattened
+    as a.b."""
+    line = None
+    with open(log) as fh:
+        for ln in fh:
+            ln = ln.strip()
+            if ln.startswith("{"):
+                try:
+                    line = json.loads(ln)
+                except ValueError:
+                    pass
+    if line is None:
+        raise ValueError("no JSON line in %s" % log)
+    with open(out_md, "a") as fh:
+        fh.write("\n## %s\n\n```\n%s\n```\n\n| field | value |\n|---|---|\n" % (title, command))
+        for k, v in _flat(line):
+            txt = ("%.6g" % v) if isinstance(v, float) else (json.dumps(v) if isinstance(v, (list, str)) else str(v))
+            fh.write("| %s | %s |\n" % (k, txt))
+
+
 def lookup_traffic(key):
     """Recorded traffic for a workload key, or None."""
     if not os.path.exists(TRAFFIC_JSON):
@@ -133,6 +164,8 @@ if __name__ == "__main__":
     a = sys.argv[1:]
     if a and a[0] == "stats" and len(a) == 5:
         write_stats(a[1], a[2], a[3], a[4])
+    elif a and a[0] == "line" and len(a) == 5:
+        append_bench_line(a[1], a[2], a[3], a[4])
     elif a and a[0] == "stages" and len(a) == 5:
         append_stage_times(a[1], a[2], int(a[3]), a[4])
     elif a and a[0] == "traffic" and len(a) >= 11:

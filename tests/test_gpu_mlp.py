@@ -88,6 +88,22 @@ def test_gpu_training_fits_smooth_target():
     assert P.size == lib.mlp_param_count()
 
 
+def test_training_independent_of_row_order():
+    """The fault kernel logs solves in atomic order; training depends on the
+    set of rows only (rows sorted before the statistics and the shuffle,
+    fixed-order gradient reductions): bit-identical parameters."""
+    from fdfault_amd import lib
+    rng = np.random.default_rng(8)
+    n = 5000
+    x = rng.uniform(-1, 1, size=(n, 4))
+    rows = np.empty((n, 5))
+    rows[:, :4] = x * np.array([10., 20., 0.01, 1.]) + np.array([70., -120., 0.07, 4.6])
+    rows[:, 4] = np.sin(2 * x[:, 0]) + x[:, 3]
+    P1, l1 = lib.mlp_train(rows, epochs=3, batch=256, lr=3e-3, seed=7)
+    P2, l2 = lib.mlp_train(rows[rng.permutation(n)], epochs=3, batch=256, lr=3e-3, seed=7)
+    assert np.array_equal(P1, P2)
+
+
 def _stz_case(tmp, name, seed, nx=161, ny=41, nt=40):
     d = os.path.join(str(tmp), name)
     os.makedirs(d, exist_ok=True)
