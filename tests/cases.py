@@ -61,12 +61,14 @@ def curved_surfaces(tmpdir, nx, ny, dx, amp=0.05, ndim=2, nz=1):
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
-def reference_example(tmpdir: str, name: str, nt: int, datadir: str) -> str:
+def reference_example(tmpdir: str, name: str, nt: int, datadir: str, text: str = None) -> str:
     """The reference's example problem `name` (tests/golden/<name>.in, a
-    verbatim copy of problems/<name>.in) with its data directory set to
-    `datadir`, its step count cut to `nt` and each output unit's last step
-    clipped to nt; every other token is unchanged."""
-    lines = open(os.path.join(GOLDEN, name + ".in")).read().split("\n")
+    verbatim copy of problems/<name>.in, or the given text) with its data
+    directory set to `datadir`, its step count cut to `nt` and each output
+    unit's last step clipped to nt; every other token is unchanged."""
+    if text is None:
+        text = open(os.path.join(GOLDEN, name + ".in")).read()
+    lines = text.split("\n")
     out, sec, k = [], None, 0
     for ln in lines:
         t = ln.strip()
@@ -91,3 +93,17 @@ def reference_example(tmpdir: str, name: str, nt: int, datadir: str) -> str:
     open(path, "w").write("\n".join(out))
     return path
 
+
+
+
+GOLDEN_PY = os.path.join(GOLDEN, "python")
+
+
+def python_example(tmpdir: str, name: str, nt: int, datadir: str) -> str:
+    """The reference's Python-written example `name` (tests/golden/python/
+    <name>.in, written by the unchanged fdfault module from
+    problems/<name>.py) shortened like reference_example, with its side-file
+    paths (written relative to the reference checkout as problems/...)
+    pointed at the fixture directory."""
+    text = open(os.path.join(GOLDEN_PY, name + ".in")).read().replace("problems/", GOLDEN_PY + "/")
+    return reference_example(tmpdir, name, nt, datadir, text=text)
