@@ -46,7 +46,7 @@ def read_outputs(data, name):
     return res, front
 
 
-def compare(got, front, want, rtol=1e-12):
+def compare(got, front, want, rtol=1e-12, atol=0.):
     assert sorted(got) == sorted(want["outputs"]), "output units differ"
     for unit, rec in want["outputs"].items():
         g = got[unit]
@@ -57,7 +57,7 @@ def compare(got, front, want, rtol=1e-12):
             assert g[key]["shape"] == rec[key]["shape"], (unit, key)
             for s in ("sum", "wsum"):
                 a, b = g[key][s], rec[key][s]
-                assert abs(a - b) <= rtol * max(abs(b), 1e-300) + 1e-300, (unit, key, s, a, b)
+                assert abs(a - b) <= rtol * abs(b) + atol, (unit, key, s, a, b)
     if want["front"] is None:
         assert front is None
     else:
@@ -65,4 +65,4 @@ def compare(got, front, want, rtol=1e-12):
             assert front[c]["shape"] == want["front"][c]["shape"]
             for s in ("sum", "wsum"):
                 a, b = front[c][s], want["front"][c][s]
-                assert abs(a - b) <= rtol * max(abs(b), 1e-300), (c, s, a, b)
+                assert abs(a - b) <= rtol * abs(b) + atol, (c, s, a, b)

@@ -41,6 +41,14 @@ def test_python_example_dropin_matches_oracle(tmp_path, name, exact):
     assert r.returncode == 0 and "Problem finished" in r.stdout
     gfiles, ofiles = sorted(os.listdir(gdir)), sorted(os.listdir(odir))
     assert gfiles == ofiles
+    # scale of each quantity (by its field name's letter: S traction, U
+    # slip, V slip rate, v velocity, s stress) over every unit of the run:
+    # station outputs that vanish by symmetry hold rounding noise only
+    scale = {}
+    for fn in ofiles:
+        if fn.endswith(".dat") and not fn.endswith(("_x.dat", "_y.dat", "_z.dat", "_t.dat")) and "_front_" not in fn:
+            q = fn[:-4].rsplit("_", 1)[1][:1]
+            scale[q] = max(scale.get(q, 0.), float(np.abs(np.fromfile(os.path.join(odir, fn))).max()))
     for fn in gfiles:
         a, b = os.path.join(gdir, fn), os.path.join(odir, fn)
         if not fn.endswith(".dat"):
@@ -50,6 +58,7 @@ def test_python_example_dropin_matches_oracle(tmp_path, name, exact):
         else:
             x, y = np.fromfile(a), np.fromfile(b)
             assert x.shape == y.shape, fn
-            assert np.abs(x - y).max() <= 1e-10 * max(np.abs(y).max(), 1e-300), fn
+            q = fn[:-4].rsplit("_", 1)[1][:1]
+            assert np.abs(x - y).max() <= 1e-10 * max(scale.get(q, 0.), np.abs(y).max(), 1e-300), fn
     got, front = read_outputs(gdir, name)
-    compare(got, front, WANT[name], rtol=1e-12 if exact else 1e-9)
+    compare(got, front, WANT[name], rtol=1e-12 if exact else 1e-9, atol=0. if exact else 1e-12)
