@@ -1269,6 +1269,7 @@ struct Tiled3D {
   int rows_var;                  // row-tile variant: 1 double-buffered LDS tile, 0 single
   int rows_shape;                // row-tile workgroup (FP32): 0 TY = 4 rows, 1 TY = 6 (LDS stride 160), 2 TY = 5
   int grp_y, grp_x;              // row-tile order: patches of grp_y y tiles x grp_x x chunks
+  int wide;                      // 1: one buffer descriptor per field (9 N elements above 4 GiB)
   int face_active[6];            // external SAT faces
   // external SAT of a Cartesian face normal to axis d (other axes a < b) as
   // its 16 non-zero linear-map entries, [6 faces][16], in the bulk precision,
@@ -1280,9 +1281,6 @@ struct Tiled3D {
   const double* if_sat[6];
 };
 
-// derivative along one direction at a one-sided boundary row, taps read from
-// global memory: left rows use c_fd[pos-lo+1][n] at lo+n, right rows
-// -c_fd[hi-pos][n] at hi-1-n (block.cpp:1057-1162 pattern)
 // Raw buffer access (gfx9 buffer descriptor in scalar registers, 32-bit
 // per-lane byte offset plus a wave-uniform scalar byte offset): no per-lane
 // 64-bit address arithmetic, and no address registers for the compiler to
@@ -1308,12 +1306,44 @@ __device__ __forceinline__ void bst(rsrc_t r, T v, unsigned voff, unsigned soff)
   }
 }
 
+// A field array [nf][N] through buffer descriptors; field q at (per-lane
+// byte offset voff, wave-uniform byte offset soff) within the field.
+// Fields1: one descriptor over the whole array (nf N elements within 4 GiB),
+// field q at scalar offset q fb.  FieldsQ: one descriptor per field, formed
+// from that field's 64-bit base (each field within 4 GiB), for arrays above
+// 4 GiB.  on = false gives descriptors with no records: loads return 0 and
+// stores are dropped, without a memory request, and the instruction stream
+// stays the same.
+template <typename R>
+struct Fields1 {
+  rsrc_t r;
+  unsigned fb;
+  __device__ __forceinline__ Fields1(const R* p, long long N, int nf, bool on)
+      : r(buf_rsrc(p, on ? nf * N : 0)), fb((unsigned)(N * (long long)sizeof(R))) {}
+  __device__ __forceinline__ R ld(int q, unsigned voff, unsigned soff) const { return bld<R>(r, voff, q * fb + soff); }
+  __device__ __forceinline__ void st(int q, R v, unsigned voff, unsigned soff) const { bst<R>(r, v, voff, q * fb + soff); }
+};
+template <typename R>
+struct FieldsQ {
+  const R* p;
+  long long N, rec;
+  __device__ __forceinline__ FieldsQ(const R* p_, long long N_, int, bool on) : p(p_), N(N_), rec(on ? N_ : 0) {}
+  __device__ __forceinline__ R ld(int q, unsigned voff, unsigned soff) const {
+    return bld<R>(buf_rsrc(p + q * N, rec), voff, soff);
+  }
+  __device__ __forceinline__ void st(int q, R v, unsigned voff, unsigned soff) const {
+    bst<R>(buf_rsrc(p + q * N, rec), v, voff, soff);
+  }
+};
+template <typename R, bool WIDE> struct FieldsOf { typedef Fields1<R> type; };
+template <typename R> struct FieldsOf<R, true> { typedef FieldsQ<R> type; };
+
 // one-sided SBP row whose taps lie in one x plane: taps at element offsets
 // lane0 + tap*stride from the wave-uniform plane pointer fpl; left rows use
 // c_fd[pos-lo+1][n] at lo+n, right rows -c_fd[hi-pos][n] at hi-1-n
-template <typename R, typename C, int S>
-__device__ __forceinline__ void edge_plane(C* out, rsrc_t rf, unsigned fb, unsigned plb, const int* fq, int lane0,
-                                           int stride, int pos, int lo, int hi, const C (*sfd)[3 * (S - 1)]) {
+template <typename R, typename C, int S, typename FA>
+__device__ __forceinline__ void edge_plane(C* out, const FA& F, unsigned plb, const int* fq, int lane0, int stride,
+                                           int pos, int lo, int hi, const C (*sfd)[3 * (S - 1)]) {
   constexpr int NT = 3 * (S - 1);
   const bool left = pos - lo < 2 * (S - 1);
   const int row = left ? pos - lo + 1 : hi - pos;
@@ -1324,7 +1354,7 @@ __device__ __forceinline__ void edge_plane(C* out, rsrc_t rf, unsigned fb, unsig
     const unsigned off = (unsigned)((lane0 + tap * stride) * (int)sizeof(R));
 #pragma unroll
     for (int q = 0; q < 6; q++) {
-      const C v = (C)bld<R>(rf, off, fq[q] * fb + plb);
+      const C v = (C)F.ld(fq[q], off, plb);
       out[q] = n == 0 ? c * v : out[q] + c * v;
     }
   }
@@ -1334,9 +1364,8 @@ __device__ __forceinline__ void edge_plane(C* out, rsrc_t rf, unsigned fb, unsig
 }
 
 // one-sided SBP row along x at the wave-uniform plane i: taps are whole planes
-template <typename R, typename C, int S>
-__device__ __forceinline__ void edge_x(C* out, rsrc_t rf, unsigned fb, unsigned psb, unsigned colb, int i, int lo,
-                                       int hi) {
+template <typename R, typename C, int S, typename FA>
+__device__ __forceinline__ void edge_x(C* out, const FA& F, unsigned psb, unsigned colb, int i, int lo, int hi) {
   constexpr int NT = 3 * (S - 1);
   const bool left = i - lo < 2 * (S - 1);
   const int row = left ? i - lo + 1 : hi - i;
@@ -1347,7 +1376,7 @@ __device__ __forceinline__ void edge_x(C* out, rsrc_t rf, unsigned fb, unsigned 
     const unsigned plb = (unsigned)tap * psb;
 #pragma unroll
     for (int q = 0; q < 6; q++) {
-      const C v = (C)bld<R>(rf, colb, q * fb + plb);
+      const C v = (C)F.ld(q, colb, plb);
       out[q] = n == 0 ? c * v : out[q] + c * v;
     }
   }
@@ -1356,6 +1385,9 @@ __device__ __forceinline__ void edge_x(C* out, rsrc_t rf, unsigned fb, unsigned 
     for (int q = 0; q < 6; q++) out[q] = -out[q];
 }
 
+// derivative along one direction at a one-sided boundary row, taps read from
+// global memory: left rows use c_fd[pos-lo+1][n] at lo+n, right rows
+// -c_fd[hi-pos][n] at hi-1-n (block.cpp:1057-1162 pattern)
 template <typename R, typename C, int S, int NQ>
 __device__ __forceinline__ void edge_deriv(C* out, const R* __restrict__ f, const int* fq, long long N, long long base,
                                            long long stride, int pos, int lo, int hi) {
@@ -1388,7 +1420,7 @@ __device__ __forceinline__ C central(const C* cd, F u) {
 }
 
 // WPE: minimum waves per SIMD the register allocation must allow
-template <typename R, typename C, int S, int TY, int WPE>
+template <typename R, typename C, int S, int TY, int WPE, bool WIDE = false>
 __global__ void __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(WPE))) bulk_tiled3d(Tiled3D ta, const R* __restrict__ f, R* __restrict__ fn,
                                                         R* __restrict__ df, const double* __restrict__ hs) {
   // This kernel serves the FP32 path and the fast FP64 path, neither of which
@@ -1427,15 +1459,13 @@ __global__ void __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(WP
   const long long col = (long long)jc * sy + kc;
   const unsigned colb = (unsigned)(col * (long long)sizeof(R));  // < one plane in bytes
   const unsigned psb = (unsigned)(sx * (long long)sizeof(R));    // plane size in bytes
-  // one descriptor per array (the host keeps 9 N elements under 4 GiB);
-  // field q of plane i lies at scalar byte offset q*fb + i*psb
+  // field q of plane i: scalar byte offset i*psb within the field
   // df/f traffic a stage does not need goes through a descriptor with no
   // records: out-of-range buffer loads return 0 and stores are dropped,
   // without a memory request, and the instruction stream stays the same
-  const rsrc_t rf = buf_rsrc(f, 9 * N), rd = buf_rsrc(df, (ta.dfmode & DF_LOAD) ? 9 * N : 0),
-               rds = buf_rsrc(df, (ta.dfmode & DF_STORE) ? 9 * N : 0),
-               rn = buf_rsrc(fn, (ta.dfmode & F_STORE) ? 9 * N : 0);
-  const unsigned fb = (unsigned)(N * (long long)sizeof(R));
+  typedef typename FieldsOf<R, WIDE>::type FA;
+  const FA rf(f, N, 9, true), rd(df, N, 9, ta.dfmode & DF_LOAD), rds(df, N, 9, ta.dfmode & DF_STORE),
+      rn(fn, N, 9, ta.dfmode & F_STORE);
   const bool yin = j >= ta.mc[1] && j < ta.mrb[1];
   const bool zin = k >= ta.mc[2] && k < ta.mrb[2];
   // block constants in the compute precision (uniform, scalar registers)
@@ -1460,7 +1490,7 @@ __global__ void __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(WP
   for (int n = 0; n < W - 1; n++) {
     const unsigned pw = xplane(ib - r + n);
 #pragma unroll
-    for (int q = 0; q < 6; q++) win[q][n] = bld<R>(rf, colb, q * fb + pw);
+    for (int q = 0; q < 6; q++) win[q][n] = rf.ld(q, colb, pw);
   }
   constexpr int tf[8] = {0, 1, 2, 4, 5, 6, 7, 8};  // f index of each tile field
   constexpr int yq[6] = {0, 1, 2, 4, 6, 7};  // vx vy vz sxy syy syz
@@ -1494,14 +1524,14 @@ __global__ void __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(WP
   auto issue = [&](int ip) {
     const unsigned pw = xplane(ip + r), pp = (unsigned)ip * psb;
 #pragma unroll
-    for (int q = 0; q < 6; q++) nwin[q] = bld<R>(rf, colb, q * fb + pw);
+    for (int q = 0; q < 6; q++) nwin[q] = rf.ld(q, colb, pw);
 #pragma unroll
-    for (int q = 0; q < 3; q++) nown[q] = bld<R>(rf, colb, (6 + q) * fb + pp);
+    for (int q = 0; q < 3; q++) nown[q] = rf.ld(6 + q, colb, pp);
 #pragma unroll
     for (int u = 0; u < NHT; u++)
       if (hly[u] >= 0) {
 #pragma unroll
-        for (int q = 0; q < 8; q++) nhal[u][q] = bld<R>(rf, hoff[u], tf[q] * fb + pp);
+        for (int q = 0; q < 8; q++) nhal[u][q] = rf.ld(tf[q], hoff[u], pp);
       }
   };
   issue(ib);
@@ -1533,7 +1563,7 @@ __global__ void __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(WP
     // here rather than a plane ahead keeps it out of the loop-carried state
     R odf[9];
 #pragma unroll
-    for (int q = 0; q < 9; q++) odf[q] = bld<R>(rd, colb, q * fb + plb);
+    for (int q = 0; q < 9; q++) odf[q] = rd.ld(q, colb, plb);
     if (active) {
       C dxv[6], dyv[6], dzv[6];
       const int jw = __builtin_amdgcn_readfirstlane(j);  // a wave is one y row
@@ -1542,7 +1572,7 @@ __global__ void __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(WP
         for (int q = 0; q < 6; q++)  // d/dx of vx vy vz sxx sxy sxz
           dxv[q] = central<C, r>(cd, [&](int n) { return (C)win[q][n]; });
       } else {
-        edge_x<R, C, S>(dxv, rf, fb, psb, colb, i, ta.lo[0], ta.hi[0]);
+        edge_x<R, C, S>(dxv, rf, psb, colb, i, ta.lo[0], ta.hi[0]);
       }
       constexpr int ymap[6] = {0, 1, 2, 3, 5, 6}, zmap[6] = {0, 1, 2, 4, 6, 7};
       if (yin) {
@@ -1566,7 +1596,7 @@ __global__ void __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(WP
             dyv[q] = left ? sy_ : -sy_;
           }
         } else {
-          edge_plane<R, C, S>(dyv, rf, fb, plb, yq, k, (int)sy, jw, ta.lo[1], ta.hi[1], sfd);
+          edge_plane<R, C, S>(dyv, rf, plb, yq, k, (int)sy, jw, ta.lo[1], ta.hi[1], sfd);
         }
       }
       if (zin) {
@@ -1589,7 +1619,7 @@ __global__ void __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(WP
             dzv[q] = left ? sz_ : -sz_;
           }
         } else {
-          edge_plane<R, C, S>(dzv, rf, fb, plb, zq, jw * (int)sy, 1, k, ta.lo[2], ta.hi[2], sfd);
+          edge_plane<R, C, S>(dzv, rf, plb, zq, jw * (int)sy, 1, k, ta.lo[2], ta.hi[2], sfd);
         }
       }
       const long long id = pl + (long long)j * sy + k;
@@ -1646,7 +1676,7 @@ __global__ void __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(WP
       }
 #pragma unroll
       for (int q = 0; q < 9; q++) {
-        bst<R>(rds, (R)acc[q], colb, q * fb + plb);
+        rds.st(q, (R)acc[q], colb, plb);
         C v = fo[q] + kB * acc[q];
         if (q >= 3 && ta.roundtrip) {  // set_stress / remove_stress rounding (FP64 only)
           double w = (double)v;
@@ -1656,7 +1686,7 @@ __global__ void __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(WP
           if (ta.hetstress) w -= hs[(q - 3) * N + id];
           v = (C)w;
         }
-        bst<R>(rn, (R)v, colb, q * fb + plb);
+        rn.st(q, (R)v, colb, plb);
       }
     }
     lds_barrier();  // every wave is done reading this plane's tile
@@ -1686,7 +1716,8 @@ __global__ void __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(WP
 // plane i-1's buffer may still be read, so one barrier per plane suffices
 // (a wave writing buffer b at plane i+1 has passed plane i's barrier, which
 // every wave reaches only after its plane i-1 reads of b).
-template <typename R, typename C, int S, int TY, int NTMAX, int NHT, int WPE, int NBUF = 1, int ZS = 256>
+template <typename R, typename C, int S, int TY, int NTMAX, int NHT, int WPE, int NBUF = 1, int ZS = 256,
+          bool WIDE = false>
 __global__ void __launch_bounds__(NTMAX) __attribute__((amdgpu_waves_per_eu(WPE)))
     bulk_rows3d(Tiled3D ta, const R* __restrict__ f, R* __restrict__ fn, R* __restrict__ df,
                 const double* __restrict__ hs) {
@@ -1748,10 +1779,9 @@ __global__ void __launch_bounds__(NTMAX) __attribute__((amdgpu_waves_per_eu(WPE)
   // df/f traffic a stage does not need goes through a descriptor with no
   // records: out-of-range buffer loads return 0 and stores are dropped,
   // without a memory request, and the instruction stream stays the same
-  const rsrc_t rf = buf_rsrc(f, 9 * N), rd = buf_rsrc(df, (ta.dfmode & DF_LOAD) ? 9 * N : 0),
-               rds = buf_rsrc(df, (ta.dfmode & DF_STORE) ? 9 * N : 0),
-               rn = buf_rsrc(fn, (ta.dfmode & F_STORE) ? 9 * N : 0);
-  const unsigned fb = (unsigned)(N * (long long)sizeof(R));
+  typedef typename FieldsOf<R, WIDE>::type FA;
+  const FA rf(f, N, 9, true), rd(df, N, 9, ta.dfmode & DF_LOAD), rds(df, N, 9, ta.dfmode & DF_STORE),
+      rn(fn, N, 9, ta.dfmode & F_STORE);
   const bool yin = j >= ta.mc[1] && j < ta.mrb[1];
   // z row of this lane: coefficient row and first tap column
   int zrow = 0, zoff = -r;
@@ -1776,7 +1806,7 @@ __global__ void __launch_bounds__(NTMAX) __attribute__((amdgpu_waves_per_eu(WPE)
   for (int n = 0; n < W - 1; n++) {
     const unsigned pw = xplane(ib - r + n);
 #pragma unroll
-    for (int q = 0; q < 6; q++) win[q][n] = bld<R>(rf, colb, q * fb + pw);
+    for (int q = 0; q < 6; q++) win[q][n] = rf.ld(q, colb, pw);
   }
   constexpr int tf[8] = {0, 1, 2, 4, 5, 6, 7, 8};  // f index of each tile field
   constexpr int yq[6] = {0, 1, 2, 4, 6, 7};        // vx vy vz sxy syy syz
@@ -1798,14 +1828,14 @@ __global__ void __launch_bounds__(NTMAX) __attribute__((amdgpu_waves_per_eu(WPE)
   auto issue = [&](int ip) {
     const unsigned pw = xplane(ip + r), pp = (unsigned)ip * psb;
 #pragma unroll
-    for (int q = 0; q < 6; q++) nwin[q] = bld<R>(rf, colb, q * fb + pw);
+    for (int q = 0; q < 6; q++) nwin[q] = rf.ld(q, colb, pw);
 #pragma unroll
-    for (int q = 0; q < 3; q++) nown[q] = bld<R>(rf, colb, (6 + q) * fb + pp);
+    for (int q = 0; q < 3; q++) nown[q] = rf.ld(6 + q, colb, pp);
 #pragma unroll
     for (int u = 0; u < NHT; u++)
       if (hrow[u] >= 0) {
 #pragma unroll
-        for (int q = 0; q < 6; q++) nhal[u][q] = bld<R>(rf, hoff[u], tf[hq[q]] * fb + pp);
+        for (int q = 0; q < 6; q++) nhal[u][q] = rf.ld(tf[hq[q]], hoff[u], pp);
       }
   };
   constexpr int LPL = LY * ZS;  // tile entries per field
@@ -1844,13 +1874,13 @@ __global__ void __launch_bounds__(NTMAX) __attribute__((amdgpu_waves_per_eu(WPE)
     if constexpr (sizeof(R) == 4) {
       if (i + 1 < ie) issue(i + 1);
 #pragma unroll
-      for (int q = 0; q < 9; q++) odf[q] = bld<R>(rd, colb, q * fb + plb);
+      for (int q = 0; q < 9; q++) odf[q] = rd.ld(q, colb, plb);
     } else {
       // FP64: this plane's df first, so that waiting for it does not wait
       // for the next plane's loads too (vmcnt counts in issue order;
       // measured 1.243 -> 1.232 ms per block launch, FP32 unchanged)
 #pragma unroll
-      for (int q = 0; q < 9; q++) odf[q] = bld<R>(rd, colb, q * fb + plb);
+      for (int q = 0; q < 9; q++) odf[q] = rd.ld(q, colb, plb);
       asm volatile("" ::: "memory");
       if (i + 1 < ie) issue(i + 1);
     }
@@ -1862,7 +1892,7 @@ __global__ void __launch_bounds__(NTMAX) __attribute__((amdgpu_waves_per_eu(WPE)
         for (int q = 0; q < 6; q++)
           dxv[q] = central<C, r>(cd, [&](int n) { return (C)win[q][n]; });
       } else {
-        edge_x<R, C, S>(dxv, rf, fb, psb, colb, i, ta.lo[0], ta.hi[0]);
+        edge_x<R, C, S>(dxv, rf, psb, colb, i, ta.lo[0], ta.hi[0]);
       }
       constexpr int ymap[6] = {0, 1, 2, 3, 5, 6}, zmap[6] = {0, 1, 2, 4, 6, 7};
       if (yin) {
@@ -1885,7 +1915,7 @@ __global__ void __launch_bounds__(NTMAX) __attribute__((amdgpu_waves_per_eu(WPE)
             dyv[q] = left ? sy_ : -sy_;
           }
         } else {
-          edge_plane<R, C, S>(dyv, rf, fb, plb, yq, k, (int)sy, j, ta.lo[1], ta.hi[1], sfd);
+          edge_plane<R, C, S>(dyv, rf, plb, yq, k, (int)sy, j, ta.lo[1], ta.hi[1], sfd);
         }
       }
       {  // z: NTB taps from column kk + zoff with the lane's coefficient row
@@ -1947,7 +1977,7 @@ __global__ void __launch_bounds__(NTMAX) __attribute__((amdgpu_waves_per_eu(WPE)
       }
 #pragma unroll
       for (int q = 0; q < 9; q++) {
-        bst<R>(rds, (R)acc[q], colb, q * fb + plb);
+        rds.st(q, (R)acc[q], colb, plb);
         C v = fo[q] + kB * acc[q];
         if (q >= 3 && ta.roundtrip) {
           double w = (double)v;
@@ -1957,7 +1987,7 @@ __global__ void __launch_bounds__(NTMAX) __attribute__((amdgpu_waves_per_eu(WPE)
           if (ta.hetstress) w -= hs[(q - 3) * N + id];
           v = (C)w;
         }
-        bst<R>(rn, (R)v, colb, q * fb + plb);
+        rn.st(q, (R)v, colb, plb);
       }
     }
     if constexpr (NBUF == 1) lds_barrier();  // every wave is done reading this plane's tile
@@ -1995,6 +2025,7 @@ struct Tiled2D {
   float fcv[2], fcg2[2], fcl[2], fcg[2], fA, fB;
   int roundtrip, hetstress;
   int dfmode = DF_ALL;  // DF_LOAD / DF_STORE / F_STORE bits (single-block launch)
+  int wide;           // 1: one buffer descriptor per field (nf N elements above 4 GiB)
   double s0[3];       // sxx sxy syy
   int face_active[4];
   const void* fcoef;  // [4 faces][10] external SAT entries in the bulk precision
@@ -2008,7 +2039,7 @@ struct Tiled2D {
   const double* hs;
 };
 
-template <typename R, typename C, int S, int TW>
+template <typename R, typename C, int S, int TW, bool WIDE>
 __device__ __forceinline__ void tile2d(const Tiled2D& ta, int tile_id, int par, int dfm) {
 #pragma clang fp contract(fast)
   constexpr int r = S - 1, W = 2 * r + 1;
@@ -2032,11 +2063,10 @@ __device__ __forceinline__ void tile2d(const Tiled2D& ta, int tile_id, int par, 
   const int jc = min(j, ta.hi[1] - 1);
   const unsigned colb = (unsigned)jc * (unsigned)sizeof(R);
   const unsigned psb = (unsigned)ta.Ny * (unsigned)sizeof(R);
-  const unsigned fb = (unsigned)(N * (long long)sizeof(R));
   // df/f traffic a stage does not need: descriptors with no records (loads
   // return 0, stores are dropped, no memory request)
-  const rsrc_t rf = buf_rsrc(f, 5 * N), rd = buf_rsrc(df, (dfm & DF_LOAD) ? 5 * N : 0),
-               rds = buf_rsrc(df, (dfm & DF_STORE) ? 5 * N : 0), rn = buf_rsrc(fn, (dfm & F_STORE) ? 5 * N : 0);
+  typedef typename FieldsOf<R, WIDE>::type FA;
+  const FA rf(f, N, 5, true), rd(df, N, 5, dfm & DF_LOAD), rds(df, N, 5, dfm & DF_STORE), rn(fn, N, 5, dfm & F_STORE);
   const bool yin = j >= ta.mc[1] && j < ta.mrb[1];
   C kcv[2], kcg2[2], kcl[2], kcg[2], kA, kB;
 #pragma unroll
@@ -2063,18 +2093,18 @@ __device__ __forceinline__ void tile2d(const Tiled2D& ta, int tile_id, int par, 
   for (int n = 0; n < W - 1; n++) {
     const unsigned pw = xplane(ib - r + n);
 #pragma unroll
-    for (int q = 0; q < 4; q++) win[q][n] = bld<R>(rf, colb, q * fb + pw);
+    for (int q = 0; q < 4; q++) win[q][n] = rf.ld(q, colb, pw);
   }
   constexpr int tf[4] = {0, 1, 3, 4};  // f index of each LDS row field
   R nwin[4], nown, nhal[4];
   auto issue = [&](int ip) {
     const unsigned pw = xplane(ip + r), pp = (unsigned)ip * psb;
 #pragma unroll
-    for (int q = 0; q < 4; q++) nwin[q] = bld<R>(rf, colb, q * fb + pw);
-    nown = bld<R>(rf, colb, 4 * fb + pp);
+    for (int q = 0; q < 4; q++) nwin[q] = rf.ld(q, colb, pw);
+    nown = rf.ld(4, colb, pp);
     if (hal) {
 #pragma unroll
-      for (int q = 0; q < 4; q++) nhal[q] = bld<R>(rf, hoff, tf[q] * fb + pp);
+      for (int q = 0; q < 4; q++) nhal[q] = rf.ld(tf[q], hoff, pp);
     }
   };
   issue(ib);
@@ -2096,7 +2126,7 @@ __device__ __forceinline__ void tile2d(const Tiled2D& ta, int tile_id, int par, 
     if (i + 1 < ie) issue(i + 1);
     R odf[5];
 #pragma unroll
-    for (int q = 0; q < 5; q++) odf[q] = bld<R>(rd, colb, q * fb + plb);
+    for (int q = 0; q < 5; q++) odf[q] = rd.ld(q, colb, plb);
     if (active) {
       C dxv[4], dyv[4];  // d/dx of vx vy sxx sxy; d/dy of vx vy sxy syy
       if (i >= ta.mc[0] && i < ta.mrb[0]) {
@@ -2111,7 +2141,7 @@ __device__ __forceinline__ void tile2d(const Tiled2D& ta, int tile_id, int par, 
           const C c = (C)c_fd[rw][n];
 #pragma unroll
           for (int q = 0; q < 4; q++) {
-            const C v = (C)bld<R>(rf, colb, q * fb + (unsigned)tap * psb);
+            const C v = (C)rf.ld(q, colb, (unsigned)tap * psb);
             dxv[q] = n == 0 ? c * v : dxv[q] + c * v;
           }
         }
@@ -2177,7 +2207,7 @@ __device__ __forceinline__ void tile2d(const Tiled2D& ta, int tile_id, int par, 
       constexpr int s0i[5] = {-1, -1, 0, 1, 2};
 #pragma unroll
       for (int q = 0; q < 5; q++) {
-        bst<R>(rds, (R)acc[q], colb, q * fb + plb);
+        rds.st(q, (R)acc[q], colb, plb);
         C v = fo[q] + kB * acc[q];
         if (q >= 2 && ta.roundtrip) {  // set_stress / remove_stress rounding (FP64 only)
           const long long id = (long long)i * ta.Ny + j;
@@ -2188,7 +2218,7 @@ __device__ __forceinline__ void tile2d(const Tiled2D& ta, int tile_id, int par, 
           if (ta.hetstress) w -= ta.hs[(q - 2) * N + id];
           v = (C)w;
         }
-        bst<R>(rn, (R)v, colb, q * fb + plb);
+        rn.st(q, (R)v, colb, plb);
       }
     }
 #pragma unroll
@@ -2200,13 +2230,13 @@ __device__ __forceinline__ void tile2d(const Tiled2D& ta, int tile_id, int par, 
 }
 
 // one block: workgroup b is tile b in XCD-aware order
-template <typename R, typename C, int S, int TW>
+template <typename R, typename C, int S, int TW, bool WIDE = false>
 __global__ void __launch_bounds__(TW) bulk_tiled2d(Tiled2D ta, int par) {
   const int nxcd = 8;
   const int per = (ta.ntiles + nxcd - 1) / nxcd;
   const int tile_id = (blockIdx.x % nxcd) * per + blockIdx.x / nxcd;
   if (tile_id >= ta.ntiles) return;
-  tile2d<R, C, S, TW>(ta, tile_id, par, ta.dfmode);
+  tile2d<R, C, S, TW, WIDE>(ta, tile_id, par, ta.dfmode);
 }
 
 // ensemble: a persistent grid walks the work list (block descriptor, tile)
@@ -2214,14 +2244,14 @@ __global__ void __launch_bounds__(TW) bulk_tiled2d(Tiled2D ta, int par) {
 // (the grid is a multiple of 8: workgroup b runs on XCD b % 8, and the items
 // a round of workgroups takes are mapped so that each XCD gets a contiguous
 // run of them, neighbouring y tiles sharing that XCD's L2)
-template <typename R, typename C, int S, int TW>
+template <typename R, typename C, int S, int TW, bool WIDE = false>
 __global__ void __launch_bounds__(TW) bulk_tiled2d_batch(const Tiled2D* __restrict__ descs,
                                                         const int2* __restrict__ work, int nwork, int par,
                                                         int dfm) {
   const int first = (blockIdx.x % 8) * (gridDim.x / 8) + blockIdx.x / 8;
   for (int w = first; w < nwork; w += gridDim.x) {
     const int2 it = work[w];
-    tile2d<R, C, S, TW>(descs[it.x], it.y, par, dfm);
+    tile2d<R, C, S, TW, WIDE>(descs[it.x], it.y, par, dfm);
   }
 }
 
@@ -2239,7 +2269,7 @@ __global__ void __launch_bounds__(TW) bulk_tiled2d_batch(const Tiled2D* __restri
 // syy), assigned before the SAT terms (block.cpp:1280-1295).  Points on
 // external faces add the SAT of boundary.cpp:265-492 through its linear map
 // at that point (ta.smap, built by face_map_kernel from ext_sat).
-template <typename R, typename C, int S, int TW, bool PL>
+template <typename R, typename C, int S, int TW, bool PL, bool WIDE>
 __device__ __forceinline__ void curv2d_tile(const Tiled2D& ta, const BlockDev& bd, const R* __restrict__ mt,
                                             const R* __restrict__ jac, int tile_id, int par, int dfm) {
 #pragma clang fp contract(fast)
@@ -2263,12 +2293,12 @@ __device__ __forceinline__ void curv2d_tile(const Tiled2D& ta, const BlockDev& b
   const int jc = min(j, ta.hi[1] - 1);
   const unsigned colb = (unsigned)jc * (unsigned)sizeof(R);
   const unsigned psb = (unsigned)ta.Ny * (unsigned)sizeof(R);
-  const unsigned fb = (unsigned)(N * (long long)sizeof(R));
   // df/f traffic a stage does not need: descriptors with no records (loads
   // return 0, stores are dropped, no memory request)
-  const rsrc_t rf = buf_rsrc(f, NF * N), rd = buf_rsrc(df, (dfm & DF_LOAD) ? NF * N : 0),
-               rds = buf_rsrc(df, (dfm & DF_STORE) ? NF * N : 0), rn = buf_rsrc(fn, (dfm & F_STORE) ? NF * N : 0);
-  const rsrc_t rm = buf_rsrc(mt, 4 * N), rj = buf_rsrc(jac, N);
+  typedef typename FieldsOf<R, WIDE>::type FA;
+  const FA rf(f, N, NF, true), rd(df, N, NF, dfm & DF_LOAD), rds(df, N, NF, dfm & DF_STORE),
+      rn(fn, N, NF, dfm & F_STORE);
+  const FA rm(mt, N, 4, true), rj(jac, N, 1, true);
   const bool yin = j >= ta.mc[1] && j < ta.mrb[1];
   C kv[2], kg2l[2], kg[2], kl[2];
 #pragma unroll
@@ -2285,10 +2315,10 @@ __device__ __forceinline__ void curv2d_tile(const Tiled2D& ta, const BlockDev& b
   auto xplane = [&](int i) { return (unsigned)min(max(i, ta.lo[0]), ta.hi[0] - 1) * psb; };
   // x quantities of one plane at this column: vx, vy, Fx, Fy
   auto xq = [&](unsigned pw, C* o) {
-    const C vx = (C)bld<R>(rf, colb, pw), vy = (C)bld<R>(rf, colb, fb + pw);
-    const C sxx = (C)bld<R>(rf, colb, 2 * fb + pw), sxy = (C)bld<R>(rf, colb, 3 * fb + pw);
-    const C syy = (C)bld<R>(rf, colb, 4 * fb + pw);
-    const C J = (C)bld<R>(rj, colb, pw), m0 = (C)bld<R>(rm, colb, pw), m1 = (C)bld<R>(rm, colb, fb + pw);
+    const C vx = (C)rf.ld(0, colb, pw), vy = (C)rf.ld(1, colb, pw);
+    const C sxx = (C)rf.ld(2, colb, pw), sxy = (C)rf.ld(3, colb, pw);
+    const C syy = (C)rf.ld(4, colb, pw);
+    const C J = (C)rj.ld(0, colb, pw), m0 = (C)rm.ld(0, colb, pw), m1 = (C)rm.ld(1, colb, pw);
     o[0] = vx;
     o[1] = vy;
     o[2] = J * (m0 * sxx + m1 * sxy);
@@ -2314,22 +2344,22 @@ __device__ __forceinline__ void curv2d_tile(const Tiled2D& ta, const BlockDev& b
   auto issue = [&](int ip) {
     const unsigned pw = xplane(ip + r), pp = (unsigned)ip * psb;
 #pragma unroll
-    for (int q = 0; q < 5; q++) nlead[q] = bld<R>(rf, colb, q * fb + pw);
-    nlead[5] = bld<R>(rj, colb, pw);
-    nlead[6] = bld<R>(rm, colb, pw);
-    nlead[7] = bld<R>(rm, colb, fb + pw);
+    for (int q = 0; q < 5; q++) nlead[q] = rf.ld(q, colb, pw);
+    nlead[5] = rj.ld(0, colb, pw);
+    nlead[6] = rm.ld(0, colb, pw);
+    nlead[7] = rm.ld(1, colb, pw);
 #pragma unroll
-    for (int q = 0; q < 3; q++) nown[q] = bld<R>(rf, colb, (2 + q) * fb + pp);
-    nown[3] = PL ? bld<R>(rf, colb, 5 * fb + pp) : (R)0;
-    nown[4] = bld<R>(rj, colb, pp);
+    for (int q = 0; q < 3; q++) nown[q] = rf.ld(2 + q, colb, pp);
+    nown[3] = PL ? rf.ld(5, colb, pp) : (R)0;
+    nown[4] = rj.ld(0, colb, pp);
 #pragma unroll
-    for (int q = 0; q < 4; q++) nown[5 + q] = bld<R>(rm, colb, q * fb + pp);
+    for (int q = 0; q < 4; q++) nown[5 + q] = rm.ld(q, colb, pp);
     if (hal) {
 #pragma unroll
-      for (int q = 0; q < 5; q++) nhal[q] = bld<R>(rf, hoff, q * fb + pp);
-      nhal[5] = bld<R>(rj, hoff, pp);
-      nhal[6] = bld<R>(rm, hoff, 2 * fb + pp);
-      nhal[7] = bld<R>(rm, hoff, 3 * fb + pp);
+      for (int q = 0; q < 5; q++) nhal[q] = rf.ld(q, hoff, pp);
+      nhal[5] = rj.ld(0, hoff, pp);
+      nhal[6] = rm.ld(2, hoff, pp);
+      nhal[7] = rm.ld(3, hoff, pp);
     }
   };
   issue(ib);
@@ -2361,7 +2391,7 @@ __device__ __forceinline__ void curv2d_tile(const Tiled2D& ta, const BlockDev& b
     if (i + 1 < ie) issue(i + 1);
     R odf[5];  // (the szz rate is assigned, so its old value is not read)
 #pragma unroll
-    for (int q = 0; q < 5; q++) odf[q] = bld<R>(rd, colb, q * fb + plb);
+    for (int q = 0; q < 5; q++) odf[q] = rd.ld(q, colb, plb);
     if (active) {
       C dxv[4], dyv[4];  // d/dx of vx vy Fx Fy; d/dy of vx vy Gx Gy
       if (i >= ta.mc[0] && i < ta.mrb[0]) {
@@ -2441,7 +2471,7 @@ __device__ __forceinline__ void curv2d_tile(const Tiled2D& ta, const BlockDev& b
       constexpr int s0i[5] = {-1, -1, 0, 1, 2};
 #pragma unroll
       for (int q = 0; q < NF; q++) {
-        bst<R>(rds, (R)acc[q], colb, q * fb + plb);
+        rds.st(q, (R)acc[q], colb, plb);
         C v = fo[q] + kB * acc[q];
         if (!PL && q >= 2 && ta.roundtrip) {  // set_stress / remove_stress rounding (FP64 elastic only)
           double w = (double)v;
@@ -2451,7 +2481,7 @@ __device__ __forceinline__ void curv2d_tile(const Tiled2D& ta, const BlockDev& b
           if (ta.hetstress) w -= ta.hs[(q - 2) * N + id];
           v = (C)w;
         }
-        bst<R>(rn, (R)v, colb, q * fb + plb);
+        rn.st(q, (R)v, colb, plb);
       }
     }
 #pragma unroll
@@ -2463,20 +2493,20 @@ __device__ __forceinline__ void curv2d_tile(const Tiled2D& ta, const BlockDev& b
 }
 
 // one block: workgroup b is tile b in XCD-aware order
-template <typename R, typename C, int S, int TW, bool PL>
+template <typename R, typename C, int S, int TW, bool PL, bool WIDE = false>
 __global__ void __launch_bounds__(TW) bulk_curv2d(Tiled2D ta, BlockDev bd, const R* __restrict__ mt,
                                                   const R* __restrict__ jac, int par) {
   const int nxcd = 8;
   const int per = (ta.ntiles + nxcd - 1) / nxcd;
   const int tile_id = (blockIdx.x % nxcd) * per + blockIdx.x / nxcd;
   if (tile_id >= ta.ntiles) return;
-  curv2d_tile<R, C, S, TW, PL>(ta, bd, mt, jac, tile_id, par, ta.dfmode);
+  curv2d_tile<R, C, S, TW, PL, WIDE>(ta, bd, mt, jac, tile_id, par, ta.dfmode);
 }
 
 // every block of a problem in one launch: work item (block, tile), one
 // workgroup per item (grid a multiple of 8, XCD-grouped as in
 // bulk_tiled2d_batch)
-template <typename R, typename C, int S, int TW, bool PL>
+template <typename R, typename C, int S, int TW, bool PL, bool WIDE = false>
 __global__ void __launch_bounds__(TW) bulk_curv2d_multi(const Tiled2D* __restrict__ descs,
                                                         const BlockDev* __restrict__ bds, const int2* __restrict__ work,
                                                         int nwork, const R* __restrict__ mt, const R* __restrict__ jac,
@@ -2484,7 +2514,7 @@ __global__ void __launch_bounds__(TW) bulk_curv2d_multi(const Tiled2D* __restric
   const int w = (blockIdx.x % 8) * (gridDim.x / 8) + blockIdx.x / 8;
   if (w >= nwork) return;
   const int2 it = work[w];
-  curv2d_tile<R, C, S, TW, PL>(descs[it.x], bds[it.x], mt, jac, it.y, par, dfm);
+  curv2d_tile<R, C, S, TW, PL, WIDE>(descs[it.x], bds[it.x], mt, jac, it.y, par, dfm);
 }
 
 // ensemble fault update: work item = (instance, first fault point of a

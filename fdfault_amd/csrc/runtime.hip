@@ -215,6 +215,7 @@ class SolverBase {
   int ntiled = 0;  // blocks on a rectilinear tiled kernel
   int nrows = 0;   // of which on the row-tile kernel
   int ncurv = 0;   // 2-D blocks on the curvilinear tiled kernel
+  int nwide = 0;   // of the tiled and curvilinear blocks, with one descriptor per field
 };
 
 template <typename R>
@@ -385,16 +386,20 @@ static int rows_zs(int shape) {
   return shape == 1 ? RowsCfg<R, 1>::ZS : (shape == 2 ? RowsCfg<R, 2>::ZS : RowsCfg<R, 0>::ZS);
 }
 
-template <typename R, int S, int NHT, int SH>
+template <typename R, int S, int NHT, int SH, bool WIDE = false>
 static const void* rows_kernel_sh(int var) {
   using Cf = RowsCfg<R, SH>;
   constexpr int NB2 = sizeof(R) == 4 ? 2 : 1;  // an FP64 double-buffered tile exceeds the LDS
-  if (var & 1) return (const void*)bulk_rows3d<R, R, S, Cf::TY, Cf::NTMAX, NHT, Cf::WPE, NB2, Cf::ZS>;
-  return (const void*)bulk_rows3d<R, R, S, Cf::TY, Cf::NTMAX, NHT, Cf::WPE, 1, Cf::ZS>;
+  if (var & 1) return (const void*)bulk_rows3d<R, R, S, Cf::TY, Cf::NTMAX, NHT, Cf::WPE, NB2, Cf::ZS, WIDE>;
+  return (const void*)bulk_rows3d<R, R, S, Cf::TY, Cf::NTMAX, NHT, Cf::WPE, 1, Cf::ZS, WIDE>;
 }
 
 template <typename R, int S, int NHT>
-static const void* rows_kernel(int var, int shape) {
+static const void* rows_kernel(int var, int shape, bool wide) {
+  if (wide) {  // one descriptor per field: 4-row workgroups (FP32 takes no row tiles at sbporder 4)
+    if constexpr (sizeof(R) == 4 && S == 4) throw Error{"row tiles: no FP32 kernel at sbporder 4"};
+    else return rows_kernel_sh<R, S, NHT, 0, true>(var);
+  }
   if constexpr (sizeof(R) == 4) {
     if (shape == 1) return rows_kernel_sh<R, S, NHT, 1>(var);
     if (shape == 2) return rows_kernel_sh<R, S, NHT, 2>(var);
@@ -403,11 +408,11 @@ static const void* rows_kernel(int var, int shape) {
 }
 
 template <typename R>
-static const void* rows_kernel_for(int s, int nht, int var, int shape) {
+static const void* rows_kernel_for(int s, int nht, int var, int shape, bool wide) {
   constexpr int N0 = RowsCfg<R, 0>::NHT0;
-  if (s == 2) return nht == N0 ? rows_kernel<R, 2, N0>(var, shape) : rows_kernel<R, 2, N0 + 1>(var, shape);
-  if (s == 3) return nht == N0 ? rows_kernel<R, 3, N0>(var, shape) : rows_kernel<R, 3, N0 + 1>(var, shape);
-  return nht == N0 ? rows_kernel<R, 4, N0>(var, shape) : rows_kernel<R, 4, N0 + 1>(var, shape);
+  if (s == 2) return nht == N0 ? rows_kernel<R, 2, N0>(var, shape, wide) : rows_kernel<R, 2, N0 + 1>(var, shape, wide);
+  if (s == 3) return nht == N0 ? rows_kernel<R, 3, N0>(var, shape, wide) : rows_kernel<R, 3, N0 + 1>(var, shape, wide);
+  return nht == N0 ? rows_kernel<R, 4, N0>(var, shape, wide) : rows_kernel<R, 4, N0 + 1>(var, shape, wide);
 }
 
 // dynamic LDS of the row-tile kernel: [NBUF][8][TY + 2r][ZS]
@@ -425,8 +430,8 @@ static void launch_rows(const Tiled3D& ta, int s, const R* f, R* fn, R* df, cons
   const size_t lds = rows_lds<R>(s, ta.rows_var, ta.rows_shape);
   const dim3 grid((unsigned)(((ta.ntiles + 7) / 8) * 8));
   void* args[] = {(void*)&ta, (void*)&f, (void*)&fn, (void*)&df, (void*)&hs};
-  HIPCHK(hipLaunchKernel(rows_kernel_for<R>(s, ta.rows_nht, ta.rows_var, ta.rows_shape), grid, dim3(nt), args, lds,
-                         st));
+  HIPCHK(hipLaunchKernel(rows_kernel_for<R>(s, ta.rows_nht, ta.rows_var, ta.rows_shape, ta.wide != 0), grid, dim3(nt),
+                         args, lds, st));
 }
 
 // The external SAT of a face normal to axis d, as the 9x9 map M[row][col] of
@@ -628,18 +633,26 @@ Solver<R>::Solver(const std::string& path, const fdf_opts& o) {
     }
     bdev.push_back(D);
     // rectilinear fast paths: 3-D, and 2-D mode II without plasticity;
-    // homogeneous material, no dissipation, not the bit-faithful FP64 mode
-    // (the tiled kernels address each field array with 32-bit buffer offsets)
+    // homogeneous material, no dissipation, not the bit-faithful FP64 mode.
+    // The tiled kernels address the field arrays through buffer descriptors
+    // with 32-bit offsets: one descriptor over all fields while nf N
+    // elements fit in 4 GiB, else one per field (each field within 4 GiB;
+    // FDF_WIDE=1 takes the per-field form at any size)
+    const double fbytes = (double)N * sizeof(R);
+    const bool fits = fbytes < 4294967296.;
+    const char* ew = getenv("FDF_WIDE");
+    const bool wide = (double)S.nf * fbytes >= 4294967296. || (ew && atoi(ew) != 0);
     const bool ok3 = nd == 3, ok2 = nd == 2 && S.mode == 2 && !S.plastic;
-    bool ok = (ok3 || ok2) && !S.hetmat && !(S.cdiss > 0.) && !(sizeof(R) == 8 && o.exact_fp64) &&
-              (double)S.nf * (double)N * sizeof(R) < 4294967296.;
+    const bool kind = !S.hetmat && !(S.cdiss > 0.) && !(sizeof(R) == 8 && o.exact_fp64);
+    bool ok = (ok3 || ok2) && kind && fits;
     for (int d = 0; d < nd && ok; d++) ok = B.mrb[d] > B.mc[d] && B.nx[d] >= 3 * (S.sbporder - 1);
     // any other 2-D mode II block (curvilinear, or plastic) may take the
     // curvilinear tiled kernel, which reads the metric per point
-    bool okc = nd == 2 && S.mode == 2 && !S.hetmat && !(S.cdiss > 0.) && !(sizeof(R) == 8 && o.exact_fp64) &&
-               (double)S.nf * (double)N * sizeof(R) < 4294967296. && 4. * (double)N * sizeof(R) < 4294967296. &&
-               !(getenv("FDF_CURV") && atoi(getenv("FDF_CURV")) == 0);
+    bool okc = nd == 2 && S.mode == 2 && kind && fits && !(getenv("FDF_CURV") && atoi(getenv("FDF_CURV")) == 0);
     for (int d = 0; d < nd && okc; d++) okc = B.mrb[d] > B.mc[d] && B.nx[d] >= 3 * (S.sbporder - 1);
+    if (!fits && (ok3 || (nd == 2 && S.mode == 2)) && kind && !o.quiet)
+      fprintf(stderr, "fdfault_amd: block %zu runs on the generic bulk kernel: one field of %lld points is "
+              "above 4 GiB\n", bi, (long long)N);
     double mdiag[3] = {0., 0., 0.};
     if (ok) ok = dgrid ? dgrid->uniform(S, B, mdiag, metric64, jac64) : block_metric_uniform(S, B, mdiag);
     Tiled3D ta{};
@@ -672,6 +685,7 @@ Solver<R>::Solver(const std::string& path, const fdf_opts& o) {
         ta.fcl[d] = (float)ta.cl[d];
       }
       ta.hetstress = S.hetstress;
+      ta.wide = wide;
       for (int q = 0; q < 6; q++) ta.s0[q] = S.s0[q];
       for (int e = 0; e < 6; e++) ta.face_active[e] = D.face[e].active;
       // row-shaped tiles when TY whole rows fit one workgroup and its halo
@@ -681,8 +695,8 @@ Solver<R>::Solver(const std::string& path, const fdf_opts& o) {
       {
         const int nz = ta.hi[2] - ta.lo[2], r = S.sbporder - 1;
         const char* es = getenv("FDF_ROWS_SHAPE");
-        int shape = 0;
-        if (sizeof(R) == 4 && !(es && atoi(es) == 0))
+        int shape = 0;  // (per-field descriptors: 4-row workgroups only)
+        if (sizeof(R) == 4 && !(es && atoi(es) == 0) && !wide)
           shape = nz <= RowsCfg<R, 1>::ZS ? 1 : (RowsCfg<R, 2>::TY * nz <= 1024 ? 2 : 0);
         const int TYs = rows_ty<R>(shape), NTM = RowsCfg<R, 0>::NTMAX;
         const int N0 = RowsCfg<R, 0>::NHT0;
@@ -704,7 +718,7 @@ Solver<R>::Solver(const std::string& path, const fdf_opts& o) {
           ta.rows_var = var;
           ta.rows_shape = shape;
           ta.nyt = (ta.hi[1] - ta.lo[1] + TYs - 1) / TYs;
-          HIPCHK(hipFuncSetAttribute(rows_kernel_for<R>(S.sbporder, ta.rows_nht, var, shape),
+          HIPCHK(hipFuncSetAttribute(rows_kernel_for<R>(S.sbporder, ta.rows_nht, var, shape, wide),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
           ta.ntiles = ta.nyt * ta.nxc;
           // tile patches of grp_y x grp_x: one XCD's worth of workgroups
@@ -731,7 +745,8 @@ Solver<R>::Solver(const std::string& path, const fdf_opts& o) {
       t2.Ny = S.nx[1];
       t2.N = (long long)N;
       t2.cx = tile2_cx;
-      t2.tw = tile2_tw;
+      t2.tw = wide ? 64 : tile2_tw;  // (per-field descriptors: 64-wide tiles only)
+      t2.wide = wide;
       if (!tile2_env) {
         // 64-wide tiles, 2-plane chunks (short marches keep the DRAM access
         // stream dense: measured best on 2x1601x801 and 2x801x401); blocks
@@ -1030,6 +1045,7 @@ Solver<R>::Solver(const std::string& path, const fdf_opts& o) {
   for (size_t b = 0; b < tiled_ok.size(); b++) {
     ncurv += curv_ok[b] ? 1 : 0;
     ntiled += tiled_ok[b] ? 1 : 0;
+    nwide += ((tiled_ok[b] && (nd == 3 ? tiles[b].wide : tiles2[b].wide)) || (curv_ok[b] && tiles2[b].wide)) ? 1 : 0;
     nrows += (tiled_ok[b] && nd == 3 && tiles[b].rows_nht > 0) ? 1 : 0;
   }
   // t = 0: interface solve without SAT (domain.cpp:141-143) on f = 0
@@ -1206,66 +1222,71 @@ static Boxes make_boxes(const BlockDev& D, bool shell) {
 
 // interior kernel launch; the compute type is FP64 for FP64 storage and
 // FP32 for FP32 storage
-template <typename R, int TY>
+template <typename R, int TY, bool WIDE = false>
 static void launch_tiled_ty(const Tiled3D& ta, int s, const R* f, R* fn, R* df, const double* hs, hipStream_t st) {
   const dim3 grid((unsigned)(((ta.ntiles + 7) / 8) * 8)), blk(64 * TY);
   // waves per SIMD the register allocation must allow: 4 (<= 128 VGPRs)
   // for FP32; FP64 holds twice the registers per value and runs at 2
   constexpr int WPE = sizeof(R) == 4 ? 4 : 2;
-  if (s == 2) hipLaunchKernelGGL((bulk_tiled3d<R, R, 2, TY, WPE>), grid, blk, 0, st, ta, f, fn, df, hs);
-  else if (s == 3) hipLaunchKernelGGL((bulk_tiled3d<R, R, 3, TY, WPE>), grid, blk, 0, st, ta, f, fn, df, hs);
-  else hipLaunchKernelGGL((bulk_tiled3d<R, R, 4, TY, WPE>), grid, blk, 0, st, ta, f, fn, df, hs);
+  if (s == 2) hipLaunchKernelGGL((bulk_tiled3d<R, R, 2, TY, WPE, WIDE>), grid, blk, 0, st, ta, f, fn, df, hs);
+  else if (s == 3) hipLaunchKernelGGL((bulk_tiled3d<R, R, 3, TY, WPE, WIDE>), grid, blk, 0, st, ta, f, fn, df, hs);
+  else hipLaunchKernelGGL((bulk_tiled3d<R, R, 4, TY, WPE, WIDE>), grid, blk, 0, st, ta, f, fn, df, hs);
 }
 
-template <typename R, int TW>
+// (WIDE: one descriptor per field; instantiated for 64-wide tiles only)
+template <typename R, int TW, bool WIDE = false>
 static void launch_tiled2_tw(const Tiled2D& t2, int s, int par, hipStream_t st) {
   const dim3 grid((unsigned)(((t2.ntiles + 7) / 8) * 8)), blk(TW);
-  if (s == 2) hipLaunchKernelGGL((bulk_tiled2d<R, R, 2, TW>), grid, blk, 0, st, t2, par);
-  else if (s == 3) hipLaunchKernelGGL((bulk_tiled2d<R, R, 3, TW>), grid, blk, 0, st, t2, par);
-  else hipLaunchKernelGGL((bulk_tiled2d<R, R, 4, TW>), grid, blk, 0, st, t2, par);
+  if (s == 2) hipLaunchKernelGGL((bulk_tiled2d<R, R, 2, TW, WIDE>), grid, blk, 0, st, t2, par);
+  else if (s == 3) hipLaunchKernelGGL((bulk_tiled2d<R, R, 3, TW, WIDE>), grid, blk, 0, st, t2, par);
+  else hipLaunchKernelGGL((bulk_tiled2d<R, R, 4, TW, WIDE>), grid, blk, 0, st, t2, par);
 }
 
-template <typename R, int TW, bool PL>
+template <typename R, int TW, bool PL, bool WIDE>
 static void launch_curv2_pl(const Tiled2D& t2, const BlockDev& D, const R* mt, const R* jac, int s, int par,
                             hipStream_t st) {
   const dim3 grid((unsigned)(((t2.ntiles + 7) / 8) * 8)), blk(TW);
-  if (s == 2) hipLaunchKernelGGL((bulk_curv2d<R, R, 2, TW, PL>), grid, blk, 0, st, t2, D, mt, jac, par);
-  else if (s == 3) hipLaunchKernelGGL((bulk_curv2d<R, R, 3, TW, PL>), grid, blk, 0, st, t2, D, mt, jac, par);
-  else hipLaunchKernelGGL((bulk_curv2d<R, R, 4, TW, PL>), grid, blk, 0, st, t2, D, mt, jac, par);
+  if (s == 2) hipLaunchKernelGGL((bulk_curv2d<R, R, 2, TW, PL, WIDE>), grid, blk, 0, st, t2, D, mt, jac, par);
+  else if (s == 3) hipLaunchKernelGGL((bulk_curv2d<R, R, 3, TW, PL, WIDE>), grid, blk, 0, st, t2, D, mt, jac, par);
+  else hipLaunchKernelGGL((bulk_curv2d<R, R, 4, TW, PL, WIDE>), grid, blk, 0, st, t2, D, mt, jac, par);
 }
 
-template <typename R, int TW>
+template <typename R, int TW, bool WIDE = false>
 static void launch_curv2_tw(const Tiled2D& t2, const BlockDev& D, const R* mt, const R* jac, int s, bool plastic,
                             int par, hipStream_t st) {
-  if (plastic) launch_curv2_pl<R, TW, true>(t2, D, mt, jac, s, par, st);
-  else launch_curv2_pl<R, TW, false>(t2, D, mt, jac, s, par, st);
+  if (plastic) launch_curv2_pl<R, TW, true, WIDE>(t2, D, mt, jac, s, par, st);
+  else launch_curv2_pl<R, TW, false, WIDE>(t2, D, mt, jac, s, par, st);
 }
 
 // merged launches over every block of a 2-D problem, one workgroup per tile
-template <typename R, int TW>
+template <typename R, int TW, bool WIDE = false>
 static void launch_merged_tw(const Tiled2D* d, const int2* wk, int nw, int s, int par, int dfm, dim3 grid,
                              hipStream_t st) {
-  if (s == 2) hipLaunchKernelGGL((bulk_tiled2d_batch<R, R, 2, TW>), grid, dim3(TW), 0, st, d, wk, nw, par, dfm);
-  else if (s == 3) hipLaunchKernelGGL((bulk_tiled2d_batch<R, R, 3, TW>), grid, dim3(TW), 0, st, d, wk, nw, par, dfm);
-  else hipLaunchKernelGGL((bulk_tiled2d_batch<R, R, 4, TW>), grid, dim3(TW), 0, st, d, wk, nw, par, dfm);
+  if (s == 2) hipLaunchKernelGGL((bulk_tiled2d_batch<R, R, 2, TW, WIDE>), grid, dim3(TW), 0, st, d, wk, nw, par, dfm);
+  else if (s == 3)
+    hipLaunchKernelGGL((bulk_tiled2d_batch<R, R, 3, TW, WIDE>), grid, dim3(TW), 0, st, d, wk, nw, par, dfm);
+  else hipLaunchKernelGGL((bulk_tiled2d_batch<R, R, 4, TW, WIDE>), grid, dim3(TW), 0, st, d, wk, nw, par, dfm);
 }
 
-template <typename R, int TW, bool PL>
+template <typename R, int TW, bool PL, bool WIDE>
 static void launch_mcurv_pl(const Tiled2D* d, const BlockDev* bds, const int2* wk, int nw, const R* mt, const R* jac,
                             int s, int par, int dfm, dim3 grid, hipStream_t st) {
   if (s == 2)
-    hipLaunchKernelGGL((bulk_curv2d_multi<R, R, 2, TW, PL>), grid, dim3(TW), 0, st, d, bds, wk, nw, mt, jac, par, dfm);
+    hipLaunchKernelGGL((bulk_curv2d_multi<R, R, 2, TW, PL, WIDE>), grid, dim3(TW), 0, st, d, bds, wk, nw, mt, jac,
+                       par, dfm);
   else if (s == 3)
-    hipLaunchKernelGGL((bulk_curv2d_multi<R, R, 3, TW, PL>), grid, dim3(TW), 0, st, d, bds, wk, nw, mt, jac, par, dfm);
+    hipLaunchKernelGGL((bulk_curv2d_multi<R, R, 3, TW, PL, WIDE>), grid, dim3(TW), 0, st, d, bds, wk, nw, mt, jac,
+                       par, dfm);
   else
-    hipLaunchKernelGGL((bulk_curv2d_multi<R, R, 4, TW, PL>), grid, dim3(TW), 0, st, d, bds, wk, nw, mt, jac, par, dfm);
+    hipLaunchKernelGGL((bulk_curv2d_multi<R, R, 4, TW, PL, WIDE>), grid, dim3(TW), 0, st, d, bds, wk, nw, mt, jac,
+                       par, dfm);
 }
 
-template <typename R, int TW>
+template <typename R, int TW, bool WIDE = false>
 static void launch_mcurv_tw(const Tiled2D* d, const BlockDev* bds, const int2* wk, int nw, const R* mt, const R* jac,
                             int s, bool plastic, int par, int dfm, dim3 grid, hipStream_t st) {
-  if (plastic) launch_mcurv_pl<R, TW, true>(d, bds, wk, nw, mt, jac, s, par, dfm, grid, st);
-  else launch_mcurv_pl<R, TW, false>(d, bds, wk, nw, mt, jac, s, par, dfm, grid, st);
+  if (plastic) launch_mcurv_pl<R, TW, true, WIDE>(d, bds, wk, nw, mt, jac, s, par, dfm, grid, st);
+  else launch_mcurv_pl<R, TW, false, WIDE>(d, bds, wk, nw, mt, jac, s, par, dfm, grid, st);
 }
 
 // stage descriptor of a 2-D rectilinear block: RK coefficients and arrays
@@ -1289,7 +1310,10 @@ template <typename R>
 void Solver<R>::launch_tiled2(Tiled2D t2, int tw, int par, hipStream_t st) {
   t2.nyt = (t2.hi[1] - t2.lo[1] + tw - 1) / tw;
   t2.ntiles = t2.nyt * t2.nxc;
-  if (tw == 64) launch_tiled2_tw<R, 64>(t2, S.sbporder, par, st);
+  if (t2.wide) {
+    if (tw != 64) throw Error{"2-D tiles with one descriptor per field: 64-wide tiles only"};
+    launch_tiled2_tw<R, 64, true>(t2, S.sbporder, par, st);
+  } else if (tw == 64) launch_tiled2_tw<R, 64>(t2, S.sbporder, par, st);
   else if (tw == 128) launch_tiled2_tw<R, 128>(t2, S.sbporder, par, st);
   else launch_tiled2_tw<R, 256>(t2, S.sbporder, par, st);
 }
@@ -1298,6 +1322,13 @@ template <typename R>
 void Solver<R>::launch_tiled(Tiled3D ta, int ty) {
   if (ta.rows_nht > 0) {
     launch_rows<R>(ta, S.sbporder, f[cur].p, f[cur ^ 1].p, df.p, hs.p, stream);
+    return;
+  }
+  if (ta.wide) {  // one descriptor per field: the default tile height only
+    constexpr int TYW = sizeof(R) == 8 ? 4 : 8;
+    ta.nyt = (ta.hi[1] - ta.lo[1] + TYW - 1) / TYW;
+    ta.ntiles = ta.nzt * ta.nyt * ta.nxc;
+    launch_tiled_ty<R, TYW, true>(ta, S.sbporder, f[cur].p, f[cur ^ 1].p, df.p, hs.p, stream);
     return;
   }
   ta.nyt = (ta.hi[1] - ta.lo[1] + ty - 1) / ty;
@@ -1317,7 +1348,12 @@ void Solver<R>::launch_bulk(int b, const StageDev& sd) {
     const dim3 grid((unsigned)(((mnwork + 7) / 8) * 8));
     const int tw = tiles2[0].tw, s = S.sbporder;
     const int dm = sd.dfmode;
-    if (merged == 1) {
+    if (tiles2[0].wide) {  // one descriptor per field (64-wide tiles)
+      if (tw != 64) throw Error{"2-D tiles with one descriptor per field: 64-wide tiles only"};
+      if (merged == 1) launch_merged_tw<R, 64, true>(dsc, mwork.p, mnwork, s, cur, dm, grid, stream);
+      else launch_mcurv_tw<R, 64, true>(dsc, mbd.p, mwork.p, mnwork, metric.p, jac.p, s, S.plastic, cur, dm, grid,
+                                        stream);
+    } else if (merged == 1) {
       if (tw == 64) launch_merged_tw<R, 64>(dsc, mwork.p, mnwork, s, cur, dm, grid, stream);
       else if (tw == 128) launch_merged_tw<R, 128>(dsc, mwork.p, mnwork, s, cur, dm, grid, stream);
       else launch_merged_tw<R, 256>(dsc, mwork.p, mnwork, s, cur, dm, grid, stream);
@@ -1341,7 +1377,10 @@ void Solver<R>::launch_bulk(int b, const StageDev& sd) {
     Tiled2D t2 = stage_tile2(b, sd);
     t2.nyt = (t2.hi[1] - t2.lo[1] + t2.tw - 1) / t2.tw;
     t2.ntiles = t2.nyt * t2.nxc;
-    if (t2.tw == 64) launch_curv2_tw<R, 64>(t2, D, metric.p, jac.p, S.sbporder, S.plastic, cur, stream);
+    if (t2.wide) {
+      if (t2.tw != 64) throw Error{"2-D tiles with one descriptor per field: 64-wide tiles only"};
+      launch_curv2_tw<R, 64, true>(t2, D, metric.p, jac.p, S.sbporder, S.plastic, cur, stream);
+    } else if (t2.tw == 64) launch_curv2_tw<R, 64>(t2, D, metric.p, jac.p, S.sbporder, S.plastic, cur, stream);
     else if (t2.tw == 128) launch_curv2_tw<R, 128>(t2, D, metric.p, jac.p, S.sbporder, S.plastic, cur, stream);
     else launch_curv2_tw<R, 256>(t2, D, metric.p, jac.p, S.sbporder, S.plastic, cur, stream);
     HIPCHK(hipGetLastError());
@@ -1875,10 +1914,14 @@ Batch<R>::Batch(const std::vector<std::string>& paths, const fdf_opts& o) : opts
     const SetupH& S = I.S;
     if (S.ndim != 2 || S.mode != 2 || S.plastic)
       throw Error{"fdf_open_batch: instance " + std::to_string(i) + " is not a 2-D mode II elastic problem"};
-    for (size_t b = 0; b < S.blocks.size(); b++)
+    for (size_t b = 0; b < S.blocks.size(); b++) {
       if (!I.tiled_ok[b])
         throw Error{"fdf_open_batch: instance " + std::to_string(i) +
                     " has a block that is not rectilinear and homogeneous (or uses dissipation)"};
+      if (I.tiles2[b].wide)
+        throw Error{"fdf_open_batch: instance " + std::to_string(i) +
+                    " has field arrays above 4 GiB (open it on its own with fdf_open)"};
+    }
     if (i == 0) {
       nst = S.rk.nstages;
       sbp = S.sbporder;
@@ -2546,6 +2589,7 @@ static void fill_info(const fdf::SolverBase* sv, fdf_info* info) {
   info->tiled_blocks = sv->ntiled;
   info->row_blocks = sv->nrows;
   info->curv_blocks = sv->ncurv;
+  info->wide_blocks = sv->nwide;
 }
 
 int fdf_get_info(fdf_t* h, fdf_info* info) {

@@ -48,6 +48,9 @@ typedef struct fdf_info {
   int tiled_blocks;  /* blocks updated by a rectilinear tiled kernel           */
   int row_blocks;    /* of which with row-shaped tiles (bulk_rows3d)          */
   int curv_blocks;   /* 2-D mode II blocks on the curvilinear tiled kernel      */
+  int wide_blocks;   /* of the tiled and curvilinear blocks, those whose kernel
+                        addresses each field through its own buffer
+                        descriptor (nfields N elements above 4 GiB)           */
 } fdf_info;
 
 /* Replaces problem::problem(const char*) (src/problem.cpp:16-84): parses the
