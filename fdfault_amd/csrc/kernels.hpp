@@ -2544,6 +2544,55 @@ __global__ void __launch_bounds__(256) fault_batch_kernel(const FaultBatchDev* _
   }
 }
 
+// A 2-D RK stage in one launch, fault solve included.  A y-interface between
+// two rectilinear blocks whose x-chunks line up is cut into the x-chunks of
+// the tiles: work item (-1 - k, 0) is chunk k, one workgroup that solves the
+// chunk's fault points (one lane each, fault_point as in fault_kernel), then
+// updates the two tiles that hold the interface rows, reading the SAT terms
+// it has just written.  Work items (desc, tile) are the other tiles.  No
+// other workgroup reads those SAT terms or this chunk's fault state, so the
+// stage needs no separate fault launch and no grid-wide ordering; every
+// fault point and bulk point is computed once, with the arithmetic of the
+// separate launches (domain.cpp:228-305: the interface solve before the
+// bulk update).
+struct FaultPair2D {
+  int fd;            // fault descriptor (FaultBatchDev) of the interface
+  int d0, d1;        // block descriptors of side 1 and side 2
+  int tile0, tile1;  // tile ids (yt + xc nyt) holding each side's interface row
+  int p0, np;        // first fault point of the chunk and its point count
+};
+
+// The fault code sets the kernel's register count (about 200 VGPRs, 2 waves
+// per SIMD).  Capping it at the tile kernel's count spills the fault part to
+// scratch, and a kernel with scratch had fewer waves in flight: measured
+// slower in both precisions than the uncapped kernel.
+template <typename R, typename C, int S, int TW, int LAW>
+__global__ void __launch_bounds__(TW) bulk_fault2d_batch(const Tiled2D* __restrict__ descs, const int2* __restrict__ work, int nwork, int par, int dfm,
+                   const FaultBatchDev* __restrict__ fdv, const FaultPair2D* __restrict__ pairs, FaultStageDev fs,
+                   int* __restrict__ err) {
+  const int first = (blockIdx.x % 8) * (gridDim.x / 8) + blockIdx.x / 8;
+  for (int w = first; w < nwork; w += gridDim.x) {
+    const int2 it = work[w];
+    int nt = 1, d0 = it.x, t0 = it.y, d1 = 0, t1 = 0;
+    if (it.x < 0) {
+      const FaultPair2D pr = pairs[-1 - it.x];
+      if ((int)threadIdx.x < pr.np) {
+        const FaultBatchDev& I = fdv[pr.fd];
+        FaultStageDev fsi = fs;
+        fsi.dt = I.dt;
+        fault_point<0, R, LAW>(I.F, I.g, fsi, reinterpret_cast<const R*>(I.f[par]),
+                               reinterpret_cast<const R*>(I.mat), I.hs, err, nullptr, pr.p0 + (int)threadIdx.x,
+                               nullptr);
+      }
+      __syncthreads();  // the SAT terms in global memory, visible to the workgroup
+      nt = 2;
+      d0 = pr.d0; t0 = pr.tile0;
+      d1 = pr.d1; t1 = pr.tile1;
+    }
+    for (int k = 0; k < nt; k++) tile2d<R, C, S, TW, false>(descs[k ? d1 : d0], k ? t1 : t0, par, dfm);
+  }
+}
+
 // Linear map of the external SAT on one face point: column q is ext_sat
 // applied to the unit state e_q (9 fields), so M . f reproduces the penalty
 // added to df for state f at that point.  Run once at set-up.

@@ -216,6 +216,7 @@ class SolverBase {
   int nrows = 0;   // of which on the row-tile kernel
   int ncurv = 0;   // 2-D blocks on the curvilinear tiled kernel
   int nwide = 0;   // of the tiled and curvilinear blocks, with one descriptor per field
+  int nfused = 0;  // interfaces solved inside the merged 2-D stage launch
 };
 
 template <typename R>
@@ -298,6 +299,16 @@ class Solver : public SolverBase {
   std::vector<DBuf<Tiled2D>> mdesc;  // [stage][roundtrip] per-block descriptors
   DBuf<BlockDev> mbd;
   DBuf<int2> mwork;                  // (block, tile)
+  // ... and, for one slip-weakening y-interface whose two blocks' tiles line
+  // up in x, the same launch with the fault solve in it (bulk_fault2d_batch):
+  // work items (block, tile) and (-1 - chunk), one fault descriptor, the
+  // chunks (FP64 by default; FDF_FUSE2D=1 forces it, 0 turns it off)
+  bool fused2d = false;
+  int mnfwork = 0;
+  DBuf<int2> mfwork;
+  DBuf<FaultBatchDev> mfdev;
+  DBuf<FaultPair2D> mpairs;
+  void launch_fused(const StageDev& sd, const FaultStageDev& fs);
   std::vector<Tiled3D> tiles;
   std::vector<Tiled2D> tiles2;
   int tile2_tw = 64, tile2_cx = 2;  // 2-D tile: y points per workgroup, x planes per chunk
@@ -353,6 +364,60 @@ static const void* fault_kernel_for(int mode, int law) {
   if (mode == 0) return fault_kernel_law<R, 0>(law);
   if (mode == 1) return fault_kernel_law<R, 1>(law);
   return fault_kernel_law<R, 2>(law);
+}
+
+// The x-chunks of a 2-D y-interface between the blocks of tile geometries A
+// (side 1, below) and B (side 2, above), as work items of bulk_fault2d_batch:
+// false when the interface cannot be solved inside the stage launch (not
+// slip weakening, not along x over both blocks' whole x range, or tiles that
+// do not line up).  fd: fault descriptor; dA, dB: block descriptors; yA, yB:
+// the y tile of each side's interface row (their tiles join the chunk item).
+static bool fault_chunks_2d(const IfaceH& I, const FaultDev& F, const Tiled2D& A, const Tiled2D& B, int tw, int fd,
+                            int dA, int dB, std::vector<FaultPair2D>& out, int& yA, int& yB) {
+  if (I.law != LAW_SLIPWEAK || I.dir != 1 || F.n1 != 1) return false;
+  if (A.lo[0] != B.lo[0] || A.hi[0] != B.hi[0] || A.cx != B.cx || A.nxc != B.nxc || A.cx > tw) return false;
+  if (I.org[0] != A.lo[0] || F.n != A.hi[0] - A.lo[0]) return false;
+  if (I.org[1] != A.hi[1] - 1 || I.org[1] + 1 != B.lo[1]) return false;
+  yA = (I.org[1] - A.lo[1]) / tw;
+  yB = (I.org[1] + 1 - B.lo[1]) / tw;
+  for (int xc = 0; xc < A.nxc; xc++) {
+    const int x0 = A.lo[0] + xc * A.cx;
+    out.push_back(FaultPair2D{fd, dA, dB, yA + xc * A.nyt, yB + xc * B.nyt, x0 - I.org[0],
+                              std::min(A.cx, A.hi[0] - x0)});
+  }
+  return true;
+}
+
+// Work list of bulk_fault2d_batch: the tile items tl, and npairs chunk
+// items (-1 - k, 0) placed first in every XCD's run of the list (workgroup
+// b of a grid of G takes item (b % 8)(G / 8) + b / 8 first), so that the
+// chunk workgroups, which do the most work, start first and are spread over
+// the XCDs; the tile items keep their order.  G <= 0: one workgroup per
+// item (G is set to the item count rounded up to 8).
+static std::vector<int2> fused_work_list(const std::vector<int2>& tl, int npairs, int& G) {
+  const int total = (int)tl.size() + npairs;
+  if (G <= 0) G = ((total + 7) / 8) * 8;
+  const int per = std::max(1, G / 8);
+  std::vector<int2> out((size_t)total);
+  std::vector<char> used((size_t)total, 0);
+  int k = 0;
+  for (int sl = 0; sl < per && k < npairs; sl++)
+    for (int x = 0; x < 8 && k < npairs; x++) {
+      const long pos = (long)x * per + sl;
+      if (pos < total && !used[pos]) {
+        out[pos] = make_int2(-1 - k++, 0);
+        used[pos] = 1;
+      }
+    }
+  for (int pos = 0; pos < total && k < npairs; pos++)
+    if (!used[pos]) {
+      out[pos] = make_int2(-1 - k++, 0);
+      used[pos] = 1;
+    }
+  size_t t = 0;
+  for (int pos = 0; pos < total; pos++)
+    if (!used[pos]) out[pos] = tl[t++];
+  return out;
 }
 
 // row-kernel variant: 1 = double-buffered LDS tile (FP32; FP64's does not
@@ -1024,6 +1089,42 @@ Solver<R>::Solver(const std::string& path, const fdf_opts& o) {
         }
       if (merged == 2) mbd.upload(bdev);
     }
+    // on by default in FP64 only: the fault code sets the fused kernel's
+    // registers (2 waves per SIMD), which the FP64 tiles (3 waves per SIMD on
+    // their own) lose less to than the separate launch costs, and the FP32
+    // tiles (6 waves per SIMD) more (FDF_FUSE2D=1 forces it, 0 turns it off)
+    const char* ef = getenv("FDF_FUSE2D");
+    const bool want = ef ? atoi(ef) != 0 : sizeof(R) == 8;
+    if (merged == 1 && !tiles2[0].wide && S.ifaces.size() == 1 && want) {
+      const IfaceH& I = S.ifaces[0];
+      std::vector<FaultPair2D> pairs;
+      int yA = 0, yB = 0;
+      if (fault_chunks_2d(I, fdev[0], tiles2[I.b1], tiles2[I.b2], tiles2[0].tw, 0, I.b1, I.b2, pairs, yA, yB)) {
+        std::vector<int2> tl;
+        for (size_t b = 0; b < S.blocks.size(); b++)
+          for (int t = 0; t < tiles2[b].ntiles; t++) {
+            const int yt = t % tiles2[b].nyt;
+            if (((int)b == I.b1 && yt == yA) || ((int)b == I.b2 && yt == yB)) continue;
+            tl.push_back(make_int2((int)b, t));
+          }
+        int G0 = 0;
+        const std::vector<int2> wlf = fused_work_list(tl, (int)pairs.size(), G0);
+        mnfwork = (int)wlf.size();
+        mfwork.upload(wlf);
+        FaultBatchDev fb{};
+        fb.F = fdev[0];
+        fb.g = G;
+        fb.f[0] = f[0].p;
+        fb.f[1] = f[1].p;
+        fb.mat = mat.p;
+        fb.hs = hs.p;
+        fb.dt = S.dt;
+        mfdev.upload(std::vector<FaultBatchDev>{fb});
+        mpairs.upload(pairs);
+        fused2d = true;
+        nfused = 1;
+      }
+    }
   }
   {
     // by default only for blocks of fewer than 8 rounds of one workgroup per
@@ -1268,6 +1369,31 @@ static void launch_merged_tw(const Tiled2D* d, const int2* wk, int nw, int s, in
   else hipLaunchKernelGGL((bulk_tiled2d_batch<R, R, 4, TW, WIDE>), grid, dim3(TW), 0, st, d, wk, nw, par, dfm);
 }
 
+// stage launch of a 2-D problem with its fault solve (slip weakening)
+template <typename R, int TW>
+static void launch_fused_tw(const Tiled2D* d, const int2* wk, int nw, int s, int par, int dfm, dim3 grid,
+                            const FaultBatchDev* fdv, const FaultPair2D* pairs, const FaultStageDev& fs, int* err,
+                            hipStream_t st) {
+  if (s == 2)
+    hipLaunchKernelGGL((bulk_fault2d_batch<R, R, 2, TW, LAW_SLIPWEAK>), grid, dim3(TW), 0, st, d, wk, nw, par, dfm,
+                       fdv, pairs, fs, err);
+  else if (s == 3)
+    hipLaunchKernelGGL((bulk_fault2d_batch<R, R, 3, TW, LAW_SLIPWEAK>), grid, dim3(TW), 0, st, d, wk, nw, par, dfm,
+                       fdv, pairs, fs, err);
+  else
+    hipLaunchKernelGGL((bulk_fault2d_batch<R, R, 4, TW, LAW_SLIPWEAK>), grid, dim3(TW), 0, st, d, wk, nw, par, dfm,
+                       fdv, pairs, fs, err);
+}
+
+template <typename R>
+static void launch_fused(int tw, const Tiled2D* d, const int2* wk, int nw, int s, int par, int dfm, dim3 grid,
+                         const FaultBatchDev* fdv, const FaultPair2D* pairs, const FaultStageDev& fs, int* err,
+                         hipStream_t st) {
+  if (tw == 64) launch_fused_tw<R, 64>(d, wk, nw, s, par, dfm, grid, fdv, pairs, fs, err, st);
+  else if (tw == 128) launch_fused_tw<R, 128>(d, wk, nw, s, par, dfm, grid, fdv, pairs, fs, err, st);
+  else launch_fused_tw<R, 256>(d, wk, nw, s, par, dfm, grid, fdv, pairs, fs, err, st);
+}
+
 template <typename R, int TW, bool PL, bool WIDE>
 static void launch_mcurv_pl(const Tiled2D* d, const BlockDev* bds, const int2* wk, int nw, const R* mt, const R* jac,
                             int s, int par, int dfm, dim3 grid, hipStream_t st) {
@@ -1407,6 +1533,16 @@ void Solver<R>::launch_bulk(int b, const StageDev& sd) {
   HIPCHK(hipGetLastError());
 }
 
+// the merged 2-D stage launch with the interface solve in it
+template <typename R>
+void Solver<R>::launch_fused(const StageDev& sd, const FaultStageDev& fs) {
+  const Tiled2D* dsc = mdesc[(size_t)cur_stage * 2 + (sd.roundtrip ? 1 : 0)].p;
+  const dim3 grid((unsigned)(((mnfwork + 7) / 8) * 8));
+  fdf::launch_fused<R>(tiles2[0].tw, dsc, mfwork.p, mnfwork, S.sbporder, cur, sd.dfmode, grid, mfdev.p, mpairs.p, fs,
+                       err.p, stream);
+  HIPCHK(hipGetLastError());
+}
+
 template <typename R>
 void Solver<R>::launch_plastic(int b) {
   const BlockDev& D = bdev[b];
@@ -1459,13 +1595,14 @@ void Solver<R>::do_stage(int istep, int st, bool full_step) {
   const double tfront = (double)(istep + 1) * S.dt;
   FaultStageDev fs{S.rk.A[st], S.rk.B[st], S.dt, tst, 0, (full_step && last && !S.plastic) ? tfront : -1.,
                    capturing ? graph_step_ptr : nullptr, S.rk.C[st], (full_step && last && !S.plastic) ? 1 : 0};
-  if (!capturing && !S.ifaces.empty()) {
+  if (!capturing && !S.ifaces.empty() && !fused2d) {
     if (nev == NEV) flush_events();
     HIPCHK(hipEventRecord(evs[2 * nev], stream));
     evkind[nev] = 1;
   }
-  for (size_t i = 0; i < S.ifaces.size(); i++) launch_fault((int)i, fs);
-  if (!capturing && !S.ifaces.empty()) {
+  if (!fused2d)
+    for (size_t i = 0; i < S.ifaces.size(); i++) launch_fault((int)i, fs);
+  if (!capturing && !S.ifaces.empty() && !fused2d) {
     HIPCHK(hipEventRecord(evs[2 * nev + 1], stream));
     nev++;
   }
@@ -1480,12 +1617,14 @@ void Solver<R>::do_stage(int istep, int st, bool full_step) {
     const hipStream_t main = stream;
     if (pair3 && b == 1) stream = side;  // launch_bulk issues on `stream`
     if (capturing) {
-      launch_bulk((int)b, sd);
+      if (fused2d) launch_fused(sd, fs);
+      else launch_bulk((int)b, sd);
     } else {
       if (nev == NEV) flush_events();
       HIPCHK(hipEventRecord(evs[2 * nev], stream));
       evkind[nev] = 0;
-      launch_bulk((int)b, sd);
+      if (fused2d) launch_fused(sd, fs);
+      else launch_bulk((int)b, sd);
       HIPCHK(hipEventRecord(evs[2 * nev + 1], stream));
       nev++;
     }
@@ -1861,6 +2000,7 @@ class BatchBase {
   virtual int size() const = 0;
   virtual void sync() = 0;
   virtual void finish() = 0;
+  virtual int fused_instances() const { return 0; }  // instances whose fault solve is in the stage launch
   int tw = 0, cx = 0;
   int persist = 1;  // 1: persistent grid over the work list, 0: one workgroup per item
   double tune_ms = 0.;
@@ -1898,6 +2038,19 @@ class Batch : public BatchBase {
   DBuf<FaultBatchDev> fdescs;
   DBuf<int2> fwork;
   int nfwork = 0;
+  std::vector<int> inst_fd;  // fault descriptor of each instance's first interface
+  // every instance with one slip-weakening y-interface whose tiles line up:
+  // one launch per stage (bulk_fault2d_batch) over the tiles and the fault
+  // chunks of all instances (off by default; FDF_FUSE2D=1 forces it)
+  bool fused = false;
+  DBuf<int2> ffwork;
+  DBuf<FaultPair2D> fpairs;
+  int nffwork = 0, ffgrid = 0;
+
+ public:
+  int fused_instances() const override { return fused ? (int)inst.size() : 0; }
+
+ private:
   DBuf<int> err, dstep;
   hipGraphExec_t graphs[2] = {nullptr, nullptr};
   bool dstep_valid = false;
@@ -1942,7 +2095,8 @@ Batch<R>::Batch(const std::vector<std::string>& paths, const fdf_opts& o) : opts
   // fault work list: (instance interface, first point of a chunk of 256)
   std::vector<FaultBatchDev> fd;
   std::vector<int2> fw;
-  for (auto& I : inst)
+  for (auto& I : inst) {
+    inst_fd.push_back(I->fdev.empty() ? -1 : (int)fd.size());
     for (size_t k = 0; k < I->fdev.size(); k++) {
       FaultBatchDev d{};
       d.F = I->fdev[k];
@@ -1955,6 +2109,7 @@ Batch<R>::Batch(const std::vector<std::string>& paths, const fdf_opts& o) : opts
       for (int p0 = 0; p0 < d.F.n; p0 += 256) fw.push_back(make_int2((int)fd.size(), p0));
       fd.push_back(d);
     }
+  }
   fdescs.upload(fd);
   fwork.upload(fw);
   nfwork = (int)fw.size();
@@ -1978,6 +2133,7 @@ void Batch<R>::build(int tw_, int cx_, int persist_) {
   cx = cx_;
   persist = persist_;
   std::vector<int2> wl;
+  std::vector<Tiled2D> geo;  // stage 0 descriptors: the tile geometry
   for (int st = 0; st < nst; st++) {
     std::vector<Tiled2D> d;
     for (auto& I : inst) {
@@ -1993,10 +2149,60 @@ void Batch<R>::build(int tw_, int cx_, int persist_) {
         d.push_back(t2);
       }
     }
+    if (st == 0) geo = d;
     descs[st].upload(d);
   }
   work.upload(wl);
   nwork = (int)wl.size();
+  // the fused list: every instance's fault chunks, and its tiles apart from
+  // the interface tiles that the chunks take
+  {
+    std::vector<FaultPair2D> pairs;
+    std::vector<std::pair<int, int>> skip;  // (descriptor, y tile)
+    // off by default: at 2 waves per SIMD the ensemble's large bulk stage
+    // loses more than the separate fault launch costs (FDF_FUSE2D=1 forces it)
+    const char* ef = getenv("FDF_FUSE2D");
+    bool ok = ef && atoi(ef) != 0;
+    int base = 0;
+    for (size_t i = 0; i < inst.size() && ok; i++) {
+      const Solver<R>& I = *inst[i];
+      int yA = 0, yB = 0;
+      ok = I.S.ifaces.size() == 1 && inst_fd[i] >= 0;
+      if (ok) {
+        const IfaceH& F = I.S.ifaces[0];
+        ok = fault_chunks_2d(F, I.fdev[0], geo[base + F.b1], geo[base + F.b2], tw, inst_fd[i], base + F.b1,
+                             base + F.b2, pairs, yA, yB);
+        skip.emplace_back(base + F.b1, yA);
+        skip.emplace_back(base + F.b2, yB);
+      }
+      base += (int)I.S.blocks.size();
+    }
+    fused = ok && !pairs.empty();
+    if (fused) {
+      std::vector<int2> tl;
+      for (const int2& it : wl) {
+        const int yt = it.y % geo[it.x].nyt;
+        bool drop = false;
+        for (const auto& sk : skip) drop = drop || (sk.first == it.x && sk.second == yt);
+        if (!drop) tl.push_back(it);
+      }
+      const int nitems = (int)(tl.size() + pairs.size());
+      fpairs.upload(pairs);
+      int dev0 = 0, ncu0 = 0, occ0 = 0;
+      HIPCHK(hipGetDevice(&dev0));
+      HIPCHK(hipDeviceGetAttribute(&ncu0, hipDeviceAttributeMultiprocessorCount, dev0));
+      if (tw == 64)
+        HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ0, bulk_fault2d_batch<R, R, 3, 64, LAW_SLIPWEAK>, 64, 0));
+      else if (tw == 128)
+        HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ0, bulk_fault2d_batch<R, R, 3, 128, LAW_SLIPWEAK>, 128, 0));
+      else
+        HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ0, bulk_fault2d_batch<R, R, 3, 256, LAW_SLIPWEAK>, 256, 0));
+      ffgrid = persist ? std::max(8, (std::min(nitems, ncu0 * std::max(occ0, 1)) / 8) * 8) : 0;
+      const std::vector<int2> wlf = fused_work_list(tl, (int)pairs.size(), ffgrid);
+      nffwork = (int)wlf.size();
+      ffwork.upload(wlf);
+    }
+  }
   // persistent grid: as many workgroups as the device keeps resident (a
   // multiple of 8 for the XCD mapping); otherwise one per work item
   int dev = 0, ncu = 0, occ = 0;
@@ -2042,6 +2248,13 @@ template <typename R>
 void Batch<R>::launch_stage(int st) {
   const bool last = st + 1 == nst;
   const FaultStageDev fs{rk.A[st], rk.B[st], 0., 0., 0, -1., dstep.p, rk.C[st], last ? 1 : 0};
+  if (fused) {
+    fdf::launch_fused<R>(tw, descs[st].p, ffwork.p, nffwork, sbp, cur, stage_dfmode(st), dim3(ffgrid), fdescs.p,
+                         fpairs.p, fs, err.p, stream);
+    HIPCHK(hipGetLastError());
+    cur ^= 1;
+    return;
+  }
   const int fgrid = std::max(1, std::min(nfwork, 4096));
   if (nfwork)
     hipLaunchKernelGGL((fault_batch_kernel<0, R>), dim3(fgrid), dim3(256), 0, stream, fdescs.p, fwork.p, nfwork, fs, cur,
@@ -2590,6 +2803,7 @@ static void fill_info(const fdf::SolverBase* sv, fdf_info* info) {
   info->row_blocks = sv->nrows;
   info->curv_blocks = sv->ncurv;
   info->wide_blocks = sv->nwide;
+  info->fused_faults = sv->nfused;
 }
 
 int fdf_get_info(fdf_t* h, fdf_info* info) {
@@ -2632,6 +2846,7 @@ int fdf_batch_step(fdf_batch_t* h, int nsteps) {
 int fdf_batch_get_info(fdf_batch_t* h, int inst, fdf_info* info) {
   if (!h || !info || inst < 0 || inst >= h->b->size()) return -2;
   fill_info(h->b->instance(inst), info);
+  info->fused_faults = h->b->fused_instances() ? 1 : 0;  // the batch's own launches
   return 0;
 }
 

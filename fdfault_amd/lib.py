@@ -39,7 +39,7 @@ class Info(C.Structure):
                 ("step", C.c_int), ("precision", C.c_int),
                 ("dt", C.c_double), ("ttot", C.c_double), ("cfl", C.c_double), ("t", C.c_double),
                 ("tiled_blocks", C.c_int), ("row_blocks", C.c_int), ("curv_blocks", C.c_int),
-                ("wide_blocks", C.c_int)]
+                ("wide_blocks", C.c_int), ("fused_faults", C.c_int)]
 
 
 EXPORTS = ["fdf_open", "fdf_step", "fdf_stage", "fdf_run", "fdf_read_field", "fdf_read_array",
