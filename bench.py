@@ -49,12 +49,17 @@ def workload_3d(d, nx=401, ny=201, nz=201, dx=0.1, sbporder=3, nt=1500, seed=123
     return p.write(os.path.join(d, "bench3d.in"))
 
 
-def workload_2d(d, nx=801, ny=401, dx=0.04, sbporder=3, nt=2000):
+def workload_2d(d, nx=801, ny=401, dx=0.04, sbporder=3, nt=2000, outputs=False):
     """2-D mode II planar fault: two blocks nx x ny, -120 MPa normal / 70 MPa
-    shear, 3 km nucleation patch (+11.6 MPa) at x = 16 km, slip weakening."""
+    shear, 3 km nucleation patch (+11.6 MPa) at x = 16 km, slip weakening.
+    outputs: slip and slip rate along the whole fault at every step (two
+    output units) besides the rupture front."""
+    from fdfault_amd.problems import Output
     loads = [Pert("constant", 0., 0., 0., 0., 0., (-120., 70., 0.)),
              Pert("boxcar", 0., 16.0, 1.5, 0., 0., (0., 11.6, 0.))]
-    p = two_block_fault(ndim=2, nx=nx, ny=ny, dx=dx, nt=nt, sbporder=sbporder, loads=loads,
+    outs = [Output("slip", "U", (0, nt, 1), (0, nx - 1, 1), (ny, ny, 1), (0, 0, 1)),
+            Output("sliprate", "V", (0, nt, 1), (0, nx - 1, 1), (ny, ny, 1), (0, 0, 1))] if outputs else []
+    p = two_block_fault(ndim=2, nx=nx, ny=ny, dx=dx, nt=nt, sbporder=sbporder, loads=loads, outputs=outs,
                         datadir=os.path.join(d, "data") + "/", name="planar2d", front=("V", 0.001))
     os.makedirs(os.path.join(d, "data"), exist_ok=True)
     return p.write(os.path.join(d, "planar2d.in"))
@@ -111,6 +116,26 @@ def single_workload(args):
         g.step(args.steps)
         t = time.perf_counter() - t0
         g.close()
+        with_out = None
+        if not rough:
+            # the same run writing its output units (slip and slip rate along
+            # the fault at every step) and front: steps, then the final drain
+            # of the staged output copies and the front file (finish)
+            od = os.path.join(d, "out")
+            os.makedirs(od)
+            po = workload_2d(od, outputs=True)
+            g = lib.Solver(po, precision=prec, graph=True, write_outputs=True)
+            g.step(args.warmup)
+            no = max(args.steps, 200)  # (finish's front file amortized over 200 steps or more)
+            t0 = time.perf_counter()
+            g.step(no)
+            g.finish()
+            to = time.perf_counter() - t0
+            g.close()
+            nbytes = sum(os.path.getsize(os.path.join(od, "data", fn)) for fn in os.listdir(os.path.join(od, "data")))
+            with_out = {"value": round(npts * nst * no / to / 1e6, 2), "ms_per_step": round(to / no * 1e3, 3),
+                        "steps": no, "outputs": "slip and slip rate along the whole fault at every step, rupture front",
+                        "bytes_written": nbytes}
         cpu = None
         if not args.no_cpu:
             threads, host = cpu_threads()
@@ -156,6 +181,8 @@ def single_workload(args):
                          npts * (4 * nf + (5 if rough else 0)) * w / 1e6)},
         "cpu_baseline": cpu,
     }
+    if with_out:
+        line["with_outputs"] = with_out
     print(json.dumps(line), flush=True)
 
 

@@ -1209,6 +1209,58 @@ __global__ void gather_box(const R* __restrict__ src, const double* __restrict__
   dst[t] = v;
 }
 
+// Every output unit due at a step in one launch (blockIdx.y: the unit in
+// the list): the unit's index box, C order, of a field (stress fields
+// absolute: f + s0 (+ s)) or of one component of a fault array
+// ([ncomp][n0][n1]; a0, a1 the grid directions along the interface, xa0, xa1
+// its first index in them), into the unit's staging slot
+// (outputunit.cpp:771-889).
+struct OutGatherDev {
+  int iface;        // 0: grid field, 1: fault array
+  int field;        // grid: index into f (< nf) or, from nf, the plastic fields
+  int add, hsf;     // grid: absolute stress (s0add, heterogeneous stress field hsf >= 0)
+  double s0add;
+  int lo[3], st[3], n[3];
+  const double* arr;  // fault array; n_if points per component, row length nn1
+  long long n_if;
+  int nn1, comp, a0, a1, xa0, xa1;
+  double* dst;      // slot 0 of the staging ring; slot k at dst + k cnt
+  long long cnt;
+};
+
+struct OutGatherList {
+  static constexpr int MAXU = 16;
+  int n;
+  int unit[MAXU], slot[MAXU];
+};
+
+template <typename R>
+__global__ void __launch_bounds__(256) gather_units(const OutGatherDev* __restrict__ units, OutGatherList L,
+                                                    const R* __restrict__ f, const R* __restrict__ fp, int nf,
+                                                    const double* __restrict__ hs, long long N, int Ny, int Nz) {
+  const OutGatherDev& U = units[L.unit[blockIdx.y]];
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= U.cnt) return;
+  const int c = (int)(t % U.n[2]);
+  const long long r = t / U.n[2];
+  const int b = (int)(r % U.n[1]);
+  const int a = (int)(r / U.n[1]);
+  const int cc[3] = {U.lo[0] + a * U.st[0], U.lo[1] + b * U.st[1], U.lo[2] + c * U.st[2]};
+  double v;
+  if (U.iface) {
+    v = U.arr[(long long)U.comp * U.n_if + (long long)(cc[U.a0] - U.xa0) * U.nn1 + (cc[U.a1] - U.xa1)];
+  } else {
+    const long long id = ((long long)cc[0] * Ny + cc[1]) * Nz + cc[2];
+    v = U.field < nf ? (double)f[(long long)U.field * N + id] : (double)fp[(long long)(U.field - nf) * N + id];
+    if (U.add) {  // absolute stress, the order of fields::set_stress (fields.cpp:184-220)
+      v += U.s0add;
+      if (U.hsf >= 0) v += hs[U.hsf * N + id];
+    }
+  }
+  U.dst[(long long)L.slot[blockIdx.y] * U.cnt + t] = v;
+}
+
+
 template <typename R>
 __global__ void convert_to_double(const R* __restrict__ src, double* __restrict__ dst, long long n) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;

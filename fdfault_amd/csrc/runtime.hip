@@ -9,6 +9,7 @@
 #include <cstring>
 #include <algorithm>
 #include <ctime>
+#include <deque>
 #include <memory>
 #include <mutex>
 #include <random>
@@ -260,7 +261,7 @@ class Solver : public SolverBase {
   std::vector<DBuf<R>> fcoefs;  // per block [6][16] external SAT entries
   int tile_ty = 4;
   void launch_plastic(int b);
-  void write_outputs(int tstep);
+  void write_outputs(int tstep, hipStream_t st = nullptr);
   // device-side set-up: the builder, and the FP64 grid arrays on the device
   // (the solver's own for FP64, the builder's for FP32)
   std::unique_ptr<DeviceGrid> dgrid;
@@ -337,6 +338,28 @@ class Solver : public SolverBase {
   DBuf<int> err;
   DBuf<double> stage_buf;
   std::vector<OutFile> outs;
+  // Output units without a blocking copy per unit (outputunit.cpp:771-889):
+  // on an output step each unit's values are gathered on the device into the
+  // next slot of its staging ring, copied to pinned host memory on the copy
+  // stream, and written to the unit's file once the copy has landed, in step
+  // order (a FIFO over all units).  A slot is reused only after its file
+  // write, so the host waits only when a ring is full.
+  struct OutStage {
+    DBuf<double> dev;        // [nslot][cnt]
+    double* host = nullptr;  // pinned, [nslot][cnt]
+    size_t cnt = 0;
+    int nslot = 0, next = 0;
+    std::vector<hipEvent_t> gathered, copied;
+    std::vector<char> busy;
+  };
+  std::vector<OutStage> ostage;
+  DBuf<OutGatherDev> ogather;  // per unit: what gather_units copies into its ring
+  std::deque<std::pair<int, int>> opend;  // (unit, slot) in step order
+  hipStream_t ostream = nullptr;
+  void setup_output_staging();
+  bool drain_output(bool wait);  // write the oldest pending slot; false if none (or not landed, !wait)
+  void drain_outputs();
+  DBuf<double>* iface_buf(int iface, int which);
   // graph replay state
   void graph_step();
   hipGraphExec_t graphs[2] = {nullptr, nullptr};
@@ -1214,6 +1237,7 @@ Solver<R>::Solver(const std::string& path, const fdf_opts& o) {
       fclose(py);
       outs.push_back(of);
     }
+    setup_output_staging();
     write_outputs(0);
   }
   // front at t = 0 (front.cpp:103-125) from V of the initial interface solve
@@ -1233,11 +1257,22 @@ Solver<R>::Solver(const std::string& path, const fdf_opts& o) {
 
 template <typename R>
 Solver<R>::~Solver() {
+  try {
+    drain_outputs();
+  } catch (...) {
+  }
   for (OutFile& o : outs) {
     if (o.data) fclose(o.data);
     if (o.time) fclose(o.time);
   }
   if (stream) (void)hipStreamSynchronize(stream);
+  if (ostream) (void)hipStreamSynchronize(ostream);
+  for (OutStage& O : ostage) {
+    for (hipEvent_t e : O.gathered) (void)hipEventDestroy(e);
+    for (hipEvent_t e : O.copied) (void)hipEventDestroy(e);
+    if (O.host) (void)hipHostFree(O.host);
+  }
+  if (ostream) (void)hipStreamDestroy(ostream);
   if (side) (void)hipStreamSynchronize(side);
   for (int e = 0; e < 2 * NEV; e++)
     if (evs[e]) (void)hipEventDestroy(evs[e]);
@@ -1713,9 +1748,148 @@ void Solver<R>::step(int n) {
   sync_check();
 }
 
-// outputunit::write_unit, outputunit.cpp:771-889 (stress fields absolute)
+// staging rings of the output units: as many slots of a unit as fit 64 MB,
+// 2 to 16
 template <typename R>
-void Solver<R>::write_outputs(int tstep) {
+void Solver<R>::setup_output_staging() {
+  HIPCHK(hipStreamCreateWithFlags(&ostream, hipStreamNonBlocking));
+  ostage.resize(S.outputs.size());
+  std::vector<OutGatherDev> gd(S.outputs.size());
+  for (size_t u = 0; u < S.outputs.size(); u++) {
+    const OutputH& U = S.outputs[u];
+    OutStage& O = ostage[u];
+    O.cnt = (size_t)U.nx[0] * U.nx[1] * U.nx[2];
+    if (!O.cnt) continue;
+    O.nslot = (int)std::max<size_t>(2, std::min<size_t>(16, ((size_t)64 << 20) / (O.cnt * sizeof(double))));
+    O.dev.alloc(O.cnt * O.nslot);
+    HIPCHK(hipHostMalloc((void**)&O.host, O.cnt * O.nslot * sizeof(double), hipHostMallocDefault));
+    O.gathered.resize(O.nslot);
+    O.copied.resize(O.nslot);
+    O.busy.assign(O.nslot, 0);
+    for (int k = 0; k < O.nslot; k++) {
+      HIPCHK(hipEventCreateWithFlags(&O.gathered[k], hipEventDisableTiming));
+      HIPCHK(hipEventCreateWithFlags(&O.copied[k], hipEventDisableTiming));
+    }
+    OutGatherDev& D = gd[u];
+    for (int d = 0; d < 3; d++) {
+      D.lo[d] = U.xm[d];
+      D.st[d] = U.xs[d];
+      D.n[d] = U.nx[d];
+    }
+    D.dst = O.dev.p;
+    D.cnt = (long long)O.cnt;
+    const int hi[3] = {U.xm[0] + (U.nx[0] - 1) * U.xs[0], U.xm[1] + (U.nx[1] - 1) * U.xs[1],
+                       U.xm[2] + (U.nx[2] - 1) * U.xs[2]};
+    if (U.iface >= 0) {  // every index of the box inside the interface arrays
+      const IfaceH& I = S.ifaces[U.iface];
+      if (I.law == LAW_LOCKED) throw Error{"read_iface: interface is not frictional"};
+      if ((U.fidx == 7 || U.fidx == 10) && I.law != LAW_STZ) throw Error{"read_iface: interface has no state"};
+      const int a0 = (I.dir == 0) ? 1 : 0, a1 = (I.dir == 2) ? 1 : 2;
+      const long n = (long)I.n0 * I.n1;
+      const long ncomp = (long)iface_buf(U.iface, U.fidx)->n / std::max(n, 1L);
+      if (U.xm[a0] < I.xm[a0] || hi[a0] - I.xm[a0] >= I.n0 || U.xm[a1] < I.xm[a1] || hi[a1] - I.xm[a1] >= I.n1 ||
+          U.comp < 0 || U.comp >= ncomp)
+        throw Error{"output unit " + U.name + ": index box outside the interface"};
+      D.iface = 1;
+      D.arr = iface_buf(U.iface, U.fidx)->p;
+      D.n_if = n;
+      D.nn1 = I.n1;
+      D.comp = U.comp;
+      D.a0 = a0;
+      D.a1 = a1;
+      D.xa0 = I.xm[a0];
+      D.xa1 = I.xm[a1];
+    } else {
+      for (int d = 0; d < 3; d++)
+        if (U.xm[d] < 0 || hi[d] >= S.nx[d]) throw Error{"output unit " + U.name + ": index box outside the grid"};
+      if (U.fidx < 0 || U.fidx >= S.nf + S.nfp) throw Error{"output unit " + U.name + ": bad field"};
+      D.field = U.fidx;
+      D.hsf = -1;
+      if (U.fidx >= S.nv && U.fidx < S.nv + S.ns) {
+        const int i = U.fidx - S.nv;
+        D.add = 1;
+        D.s0add = S.s0[S.sidx[i]];
+        if (S.hetstress) D.hsf = i;
+      }
+    }
+  }
+  ogather.upload(gd);
+}
+
+// write the oldest pending output slot once its copy has landed
+template <typename R>
+bool Solver<R>::drain_output(bool wait) {
+  if (opend.empty()) return false;
+  const int u = opend.front().first, k = opend.front().second;
+  OutStage& O = ostage[u];
+  if (!wait) {
+    const hipError_t q = hipEventQuery(O.copied[k]);
+    if (q == hipErrorNotReady) return false;
+    HIPCHK(q);
+  }
+  HIPCHK(hipEventSynchronize(O.copied[k]));
+  fwrite(O.host + (size_t)k * O.cnt, sizeof(double), O.cnt, outs[u].data);
+  O.busy[k] = 0;
+  opend.pop_front();
+  return true;
+}
+
+template <typename R>
+void Solver<R>::drain_outputs() {
+  while (drain_output(true)) {
+  }
+}
+
+template <typename R>
+DBuf<double>* Solver<R>::iface_buf(int iface, int which) {
+  if (iface < 0 || iface >= (int)S.ifaces.size()) throw Error{"read_iface: bad interface"};
+  IfBufs& ib = ibufs[iface];
+  switch (which) {
+    case 0: return &ib.u;
+    case 1: return &ib.v;
+    case 2: return &ib.s;
+    case 3: return &ib.sn;
+    case 4: return &ib.ux;
+    case 5: return &ib.vx;
+    case 6: return &ib.sx;
+    case 7: return &ib.state;
+    case 8: return &ib.du;
+    case 9: return &ib.dux;
+    case 10: return &ib.dstate;
+    default: throw Error{"read_iface: bad array id"};
+  }
+}
+
+// outputunit::write_unit, outputunit.cpp:771-889: stress fields absolute
+// (f + s0 + s, the order of fields::set_stress), interface arrays by
+// component; gathered on stream st (default: the solver's own), which must
+// hold the step's last launch
+template <typename R>
+void Solver<R>::write_outputs(int tstep, hipStream_t st) {
+  if (!st) st = stream;
+  OutGatherList L{};
+  long long most = 0;
+  auto launch = [&]() {
+    if (!L.n) return;
+    const dim3 grid((unsigned)((most + 255) / 256), (unsigned)L.n);
+    hipLaunchKernelGGL((gather_units<R>), grid, dim3(256), 0, st, ogather.p, L, f[cur].p, fp.p, S.nf, hs.p, G.N,
+                       G.Ny, G.Nz);
+    HIPCHK(hipGetLastError());
+    for (int q = 0; q < L.n; q++) {  // copy each unit's slot once the gather is done
+      OutStage& O = ostage[L.unit[q]];
+      const int k = L.slot[q];
+      if (q == 0) {
+        HIPCHK(hipEventRecord(O.gathered[k], st));
+        HIPCHK(hipStreamWaitEvent(ostream, O.gathered[k], 0));
+      }
+      HIPCHK(hipMemcpyAsync(O.host + (size_t)k * O.cnt, O.dev.p + (size_t)k * O.cnt, O.cnt * sizeof(double),
+                            hipMemcpyDeviceToHost, ostream));
+      HIPCHK(hipEventRecord(O.copied[k], ostream));
+      opend.emplace_back(L.unit[q], k);
+    }
+    L.n = 0;
+    most = 0;
+  };
   for (size_t u = 0; u < S.outputs.size(); u++) {
     const OutputH& U = S.outputs[u];
     if (tstep < U.tm || tstep > U.tp || (tstep - U.tm) % U.ts != 0) continue;
@@ -1723,40 +1897,23 @@ void Solver<R>::write_outputs(int tstep) {
       const double t = (double)tstep * S.dt;
       fwrite(&t, sizeof(double), 1, outs[u].time);
     }
-    const size_t cnt = (size_t)U.nx[0] * U.nx[1] * U.nx[2];
-    std::vector<double> host(cnt);
-    if (U.iface < 0) {
-      const int hi[3] = {U.xp[0], U.xp[1], U.xp[2]};
-      read_field(U.fidx, U.xm, hi, U.xs, host.data());
-      // absolute stress: the output runs between set_stress and remove_stress
-      if (U.fidx >= S.nv && U.fidx < S.nv + S.ns) {
-        // read_field returns f; add s0 (+ s) here in the same order
-        const int i = U.fidx - S.nv;
-        const double s0 = S.s0[S.sidx[i]];
-        size_t q = 0;
-        for (int a = U.xm[0]; a <= U.xp[0]; a += U.xs[0])
-          for (int b = U.xm[1]; b <= U.xp[1]; b += U.xs[1])
-            for (int c = U.xm[2]; c <= U.xp[2]; c += U.xs[2], q++) {
-              double v = host[q] + s0;
-              if (S.hetstress) v += S.s[(size_t)i * S.N + S.idx(a, b, c)];
-              host[q] = v;
-            }
-      }
-    } else {
-      const IfaceH& I = S.ifaces[U.iface];
-      const long n = (long)I.n0 * I.n1;
-      std::vector<double> arr((size_t)(S.ndim - 1) * n > (size_t)n ? (size_t)(S.ndim - 1) * n : (size_t)n);
-      read_iface(U.iface, U.fidx, arr.data(), (long)arr.size());
-      const int a0 = (I.dir == 0) ? 1 : 0, a1 = (I.dir == 2) ? 1 : 2;
-      size_t q = 0;
-      for (int a = U.xm[0]; a <= U.xp[0]; a += U.xs[0])
-        for (int b = U.xm[1]; b <= U.xp[1]; b += U.xs[1])
-          for (int c = U.xm[2]; c <= U.xp[2]; c += U.xs[2], q++) {
-            const int cc[3] = {a, b, c};
-            host[q] = arr[(size_t)U.comp * n + (size_t)(cc[a0] - I.xm[a0]) * I.n1 + (cc[a1] - I.xm[a1])];
-          }
+    OutStage& O = ostage[u];
+    if (!O.cnt) continue;
+    const int k = O.next;
+    O.next = (O.next + 1) % O.nslot;
+    while (O.busy[k]) {  // the ring is full: write the oldest pending slots
+      if (opend.empty()) launch();  // (this step's own earlier units hold it)
+      drain_output(true);
     }
-    fwrite(host.data(), sizeof(double), cnt, outs[u].data);
+    O.busy[k] = 1;
+    L.unit[L.n] = (int)u;
+    L.slot[L.n] = k;
+    L.n++;
+    most = std::max(most, (long long)O.cnt);
+    if (L.n == OutGatherList::MAXU) launch();
+  }
+  launch();
+  while (drain_output(false)) {
   }
 }
 
@@ -1878,23 +2035,7 @@ void Solver<R>::write_array(int which, const double* h, long n) {
 
 template <typename R>
 long Solver<R>::read_iface(int iface, int which, double* h, long n) {
-  if (iface < 0 || iface >= (int)S.ifaces.size()) throw Error{"read_iface: bad interface"};
-  IfBufs& ib = ibufs[iface];
-  DBuf<double>* b = nullptr;
-  switch (which) {
-    case 0: b = &ib.u; break;
-    case 1: b = &ib.v; break;
-    case 2: b = &ib.s; break;
-    case 3: b = &ib.sn; break;
-    case 4: b = &ib.ux; break;
-    case 5: b = &ib.vx; break;
-    case 6: b = &ib.sx; break;
-    case 7: b = &ib.state; break;
-    case 8: b = &ib.du; break;
-    case 9: b = &ib.dux; break;
-    case 10: b = &ib.dstate; break;
-    default: throw Error{"read_iface: bad array id"};
-  }
+  DBuf<double>* b = iface_buf(iface, which);
   if (S.ifaces[iface].law == LAW_LOCKED) throw Error{"read_iface: interface is not frictional"};
   if ((which == 7 || which == 10) && S.ifaces[iface].law != LAW_STZ) throw Error{"read_iface: interface has no state"};
   if (!h) return (long)b->n;
@@ -1918,6 +2059,7 @@ long Solver<R>::read_front(int iface, double* h, long n) {
 // outputlist::close_list + frontlist::write_list (front.cpp:127-359)
 template <typename R>
 void Solver<R>::finish() {
+  drain_outputs();
   for (OutFile& o : outs) {
     if (o.data) fclose(o.data);
     if (o.time) fclose(o.time);
@@ -2416,10 +2558,12 @@ void Batch<R>::step(int n) {
     cur_step++;
     const bool stale = !(stage_dfmode(nst - 1) & DF_STORE);
     for (auto& I : inst) I->df_stale = stale;
-    if (opts.write_outputs) {
-      sync();
-      for (auto& I : inst) I->write_outputs(cur_step);
-    }
+    if (opts.write_outputs)
+      for (auto& I : inst) {  // gathered on the batch stream, after this step's launches
+        I->cur = cur;
+        I->cur_step = cur_step;
+        I->write_outputs(cur_step, stream);
+      }
   }
   sync();
 }

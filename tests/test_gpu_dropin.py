@@ -64,3 +64,34 @@ def test_executable_error_message(tmp_path):
     r = subprocess.run([exe, str(bad)], capture_output=True, text=True, timeout=60)
     assert r.returncode != 0
     assert "Error in initializing RK method" in r.stderr
+
+
+def test_batch_output_files_match_single_runs(tmp_path):
+    """An ensemble writing its instances' output units (gathered on the batch
+    stream, copied and written asynchronously) gives the same files, byte for
+    byte, as each instance run on its own, over more output steps than an
+    output unit's staging ring holds."""
+    from fdfault_amd import lib
+    outs = [Output("U", "U", (0, 24, 1), (0, 80, 1), (31, 31, 1), (0, 0, 1)),
+            Output("vy", "vy", (0, 24, 3), (0, 80, 4), (0, 61, 2), (0, 0, 1)),
+            Output("sxy", "sxy", (2, 22, 2), (10, 70, 1), (28, 34, 1), (0, 0, 1))]
+    single, batch = [], []
+    for i, nx in enumerate((81, 97)):
+        for tag, lst in (("s", single), ("b", batch)):
+            lst.append(write_case(str(tmp_path), "%s%d" % (tag, i), nx=nx, ny=31, dx=0.1, nt=24,
+                                  outputs=outs, front=("V", 0.001)))
+    bt = lib.Batch(batch, precision=lib.FP32, graph=True, write_outputs=True)
+    bt.step(24)
+    bt.finish()
+    bt.close()
+    for ps, pb in zip(single, batch):
+        s = lib.Solver(ps, precision=lib.FP32, graph=True, write_outputs=True)
+        s.step(24)
+        s.finish()
+        s.close()
+        ds, db = os.path.join(os.path.dirname(ps), "data"), os.path.join(os.path.dirname(pb), "data")
+        fs = sorted(os.listdir(ds))
+        assert len(fs) >= 15
+        for fn in fs:
+            fb = fn.replace(os.path.basename(os.path.dirname(ps)), os.path.basename(os.path.dirname(pb)), 1)
+            assert filecmp.cmp(os.path.join(ds, fn), os.path.join(db, fb), shallow=False), fn
