@@ -108,6 +108,7 @@ def single_workload(args):
         npts, nst, nb, nt_ = g.N, g.nstages, g.info.nblocks, g.nt
         tiled = g.info.tiled_blocks == nb
         curv = g.info.curv_blocks == nb
+        fused = g.info.fused_faults > 0  # the stage's fault solve inside the bulk launch
         g.close()
         # throughput: whole time steps replayed as hipGraphs
         g = lib.Solver(path, precision=prec, graph=True)
@@ -153,7 +154,7 @@ def single_workload(args):
     # one launch per stage covers every block when the blocks are merged
     merged = nb > 1 and nl == nst * args.steps
     if tiled:
-        kernel = "bulk_tiled2d_batch" if merged else "bulk_tiled2d"
+        kernel = "bulk_fault2d_batch" if fused else ("bulk_tiled2d_batch" if merged else "bulk_tiled2d")
     elif curv:
         kernel = "bulk_curv2d_multi" if merged else "bulk_curv2d"
     else:
@@ -177,6 +178,7 @@ def single_workload(args):
                      "launches": nl, "bytes_per_point": B, "bytes_per_point_stage_with_plasticity": B_stage,
                      "points_per_launch": int(ppl),
                      "fault_ms_per_stage": round(fms, 4),
+                     "fault_solve_in_bulk_launch": fused,
                      "note": "working set %.0f MB: may be resident in the 256 MB Infinity Cache" % (
                          npts * (4 * nf + (5 if rough else 0)) * w / 1e6)},
         "cpu_baseline": cpu,
