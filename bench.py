@@ -8,7 +8,9 @@ the dominant kernel (the fused bulk stage kernel) and an in-run CPU baseline
 The default line reports the FP64 path as `value` and, in the same run, the
 FP32-bulk / FP64-fault path with its speed ratio and its error against FP64
 after --check-steps steps.  Other workloads: --workload planar2d, rough2d,
-ensemble, stz3d.
+ensemble, stz3d, tpv4 (the reference's own
+example problems/tpv4.in run whole through the drop-in executable, against
+the run time the reference quotes for it).
 
 Default workload: 3-D mixed-mode planar-fault rupture, two blocks of
 401x201x201 points (grid 401x402x201), dx = 100 m, slip-weakening friction,
@@ -435,6 +437,75 @@ def ensemble(args):
     print(json.dumps(line), flush=True)
 
 
+# The reference quotes one run time: problems/tpv4.in, "about 1 hour" on an
+# 8-core Mac Pro using all processors (docs/examples/tpv4.rst:20), that is
+# 4 100 802 points x 701 steps x 5 RK stages / 3600 s = 4.0 Mpt-updates/s.
+TPV4_REF_WALL_S = 3600.0
+TPV4_REF_MPTS = 4100802 * 701 * 5 / TPV4_REF_WALL_S / 1e6
+
+
+def reference_tpv4(args):
+    """The reference's own example problems/tpv4.in (kept verbatim as
+    tests/golden/tpv4.in: 3-D, two 201x101x101 blocks, sbporder 4, 701
+    steps, a fault output unit every 10 steps and the rupture front), run
+    whole through the drop-in executable: process start, open, every step,
+    the output and front files, files closed.  value: Mpt-updates/s per RK
+    stage of that whole run; vs_baseline: against the rate the reference's
+    quoted run time implies.  Also the stepping alone (library, graph
+    replay, no outputs) and the FP64 CPU oracle on a few steps."""
+    from fdfault_amd import lib
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from cases import reference_example
+    nt = args.tpv4_steps
+    exe = os.path.join(ROOT, "fdfault_amd", "_lib", "fdfault")
+    with tempfile.TemporaryDirectory() as d:
+        dd = os.path.join(d, "data") + "/"
+        os.makedirs(dd)
+        path = reference_example(d, "tpv4", nt, dd)
+        env = dict(os.environ, FDFAULT_PRECISION=args.precision)
+        t0 = time.perf_counter()
+        r = subprocess.run([exe, path], capture_output=True, text=True, env=env, timeout=1200)
+        wall = time.perf_counter() - t0
+        if r.returncode != 0 or "Problem finished" not in r.stdout:
+            raise RuntimeError("drop-in run failed: %s %s" % (r.stdout[-500:], r.stderr[-500:]))
+        files = sorted(os.listdir(dd))
+        nbytes = sum(os.path.getsize(os.path.join(dd, f)) for f in files)
+        print("drop-in run: %.2f s, %d files, %d bytes" % (wall, len(files), nbytes), flush=True)
+        g = lib.Solver(path, precision=lib.FP32 if args.precision == "fp32" else lib.FP64, graph=True)
+        npts, nst = g.N, g.nstages
+        g.step(args.warmup)
+        t1 = time.perf_counter()
+        g.step(args.steps)
+        ts = time.perf_counter() - t1
+        g.close()
+        cpu = None
+        if not args.no_cpu:
+            threads, host = cpu_threads()
+            cv, cn = cpu_baseline(path, args.cpu_steps, threads)
+            cpu = {"value": round(cv, 3), "unit": "Mpt-updates/s", "cores": threads, "kind": "port",
+                   "sample": "the same problem (%d pts), %d steps, FP64 CPU oracle, OpenMP on %s"
+                             % (cn, args.cpu_steps, host)}
+    value = npts * nst * nt / wall / 1e6
+    line = {
+        "metric": "Mpt-updates/s per RK stage, whole drop-in run of the reference's problems/tpv4.in",
+        "value": round(value, 2), "unit": "Mpt-updates/s", "n_gpus": 1, "steps": nt, "warmup": 0,
+        "ms_per_step": round(wall / nt * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": round(value / TPV4_REF_MPTS, 1) if nt == 701 else None,
+        "dtype": "f64" if args.precision == "fp64" else "f32", "data": "the reference's example input",
+        "config": {"workload": "tpv4_2x201x101x101_sbp4", "points": npts, "rk_stages": nst, "nt": nt,
+                   "precision": args.precision, "outputs": "fault slip rate every 10 steps, rupture front"},
+        "wall_s": round(wall, 3), "files_written": len(files), "bytes_written": nbytes,
+        "reference_quoted": {"wall_s": TPV4_REF_WALL_S, "mpt_updates_per_s": round(TPV4_REF_MPTS, 2),
+                             "hardware": "8 core Mac Pro desktop, all processors (CPU model and year unstated)",
+                             "source": "docs/examples/tpv4.rst:20"},
+        "stepping_only": {"value": round(npts * nst * args.steps / ts / 1e6, 2),
+                          "ms_per_step": round(ts / args.steps * 1e3, 3), "steps": args.steps,
+                          "note": "library, graph replay, outputs off"},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -447,7 +518,9 @@ def main():
     ap.add_argument("--nz", type=int, default=201)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=20)
-    ap.add_argument("--workload", choices=["planar3d", "ensemble", "stz3d", "planar2d", "rough2d", "drift", "setup"],
+    ap.add_argument("--tpv4-steps", type=int, default=701, help="tpv4: time steps of the drop-in run")
+    ap.add_argument("--workload", choices=["planar3d", "ensemble", "stz3d", "planar2d", "rough2d", "drift", "setup",
+                                           "tpv4"],
                     default="planar3d")
     ap.add_argument("--compare-precision", action="store_true",
                     help="planar3d: FP32 vs FP64 errors after --compare-steps steps")
@@ -476,6 +549,9 @@ def main():
         return
     if args.workload == "drift":
         fp64_drift(args)
+        return
+    if args.workload == "tpv4":
+        reference_tpv4(args)
         return
     if args.workload == "setup":
         setup_times(args)

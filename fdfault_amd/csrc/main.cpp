@@ -4,7 +4,8 @@
 // FDFAULT_PRECISION=fp32 selects the FP32-bulk / FP64-fault path;
 // FDFAULT_EXACT=1 keeps the reference's arithmetic order everywhere (FP64
 // output files byte-identical to the reference algorithm) instead of the
-// rectilinear fast kernels (within 1e-10).
+// rectilinear fast kernels (within 1e-10).  Each time step replays as one
+// hipGraph (FDFAULT_GRAPH=0: launches issued one by one; same results).
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -28,6 +29,8 @@ int main(int argc, char** argv) {
   if (ex && atoi(ex) != 0) o.exact_fp64 = 1;
   const char* dv = getenv("FDFAULT_DEVICE");
   if (dv) o.device = atoi(dv);
+  const char* gr = getenv("FDFAULT_GRAPH");
+  o.use_graph = (gr && atoi(gr) == 0) ? 0 : 1;
   fdf_t* h = nullptr;
   if (fdf_open(argv[1], &o, &h) != 0) {
     fprintf(stderr, "%s\n", fdf_last_error());

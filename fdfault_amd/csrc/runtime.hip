@@ -453,13 +453,15 @@ static const int rows_grp_y = 16, rows_grp_x = 2;
 // row-tile workgroup shapes: 0 = FP32 4 rows (<= 1024 threads, 4 waves per
 // SIMD) or FP64 2 rows (<= 512 threads, 2 waves per SIMD), LDS rows padded to
 // 256 points; 1 = FP32 6 rows of at most 160 points, LDS rows padded to 160;
-// 2 = FP32 5 rows of at most 204 points.  As many rows as fit 1024 threads,
-// so that a workgroup fills the 16 wave slots of a CU.
+// 2 = FP32 5 rows of at most 204 points; 3 = FP64 4 rows of at most 128
+// points, LDS rows padded to 128.  As many rows as fit the thread budget, so
+// that a workgroup fills the wave slots its registers allow (FP32 16, FP64 8
+// per CU).
 template <typename R, int SH>
 struct RowsCfg {
   static constexpr bool F32 = sizeof(R) == 4;
-  static constexpr int TY = F32 ? (SH == 1 ? 6 : (SH == 2 ? 5 : 4)) : 2;
-  static constexpr int ZS = SH == 1 ? 160 : 256;
+  static constexpr int TY = F32 ? (SH == 1 ? 6 : (SH == 2 ? 5 : 4)) : (SH == 3 ? 4 : 2);
+  static constexpr int ZS = SH == 1 ? 160 : (SH == 3 ? 128 : 256);
   static constexpr int NTMAX = F32 ? 1024 : 512;
   static constexpr int WPE = F32 ? 4 : 2;
   static constexpr int NHT0 = F32 ? 1 : 2;  // halo entries per thread: NHT0 or NHT0 + 1
@@ -467,11 +469,13 @@ struct RowsCfg {
 
 template <typename R>
 static int rows_ty(int shape) {
-  return shape == 1 ? RowsCfg<R, 1>::TY : (shape == 2 ? RowsCfg<R, 2>::TY : RowsCfg<R, 0>::TY);
+  return shape == 1 ? RowsCfg<R, 1>::TY
+                    : (shape == 2 ? RowsCfg<R, 2>::TY : (shape == 3 ? RowsCfg<R, 3>::TY : RowsCfg<R, 0>::TY));
 }
 template <typename R>
 static int rows_zs(int shape) {
-  return shape == 1 ? RowsCfg<R, 1>::ZS : (shape == 2 ? RowsCfg<R, 2>::ZS : RowsCfg<R, 0>::ZS);
+  return shape == 1 ? RowsCfg<R, 1>::ZS
+                    : (shape == 2 ? RowsCfg<R, 2>::ZS : (shape == 3 ? RowsCfg<R, 3>::ZS : RowsCfg<R, 0>::ZS));
 }
 
 template <typename R, int S, int NHT, int SH, bool WIDE = false>
@@ -491,6 +495,8 @@ static const void* rows_kernel(int var, int shape, bool wide) {
   if constexpr (sizeof(R) == 4) {
     if (shape == 1) return rows_kernel_sh<R, S, NHT, 1>(var);
     if (shape == 2) return rows_kernel_sh<R, S, NHT, 2>(var);
+  } else {
+    if (shape == 3) return rows_kernel_sh<R, S, NHT, 3>(var);
   }
   return rows_kernel_sh<R, S, NHT, 0>(var);
 }
@@ -779,13 +785,15 @@ Solver<R>::Solver(const std::string& path, const fdf_opts& o) {
       // row-shaped tiles when TY whole rows fit one workgroup and its halo
       // needs at most NHT0 + 1 entries per thread (FDF_ROWS=0 keeps 64-wide
       // tiles; FDF_ROWS=1 single-buffered, 2 double-buffered LDS tile;
-      // FDF_ROWS_SHAPE=0 keeps 4-row workgroups for short rows)
+      // FDF_ROWS_SHAPE=0 keeps the 4-row FP32 and 2-row FP64 workgroups
+      // for short rows)
       {
         const int nz = ta.hi[2] - ta.lo[2], r = S.sbporder - 1;
         const char* es = getenv("FDF_ROWS_SHAPE");
         int shape = 0;  // (per-field descriptors: 4-row workgroups only)
         if (sizeof(R) == 4 && !(es && atoi(es) == 0) && !wide)
           shape = nz <= RowsCfg<R, 1>::ZS ? 1 : (RowsCfg<R, 2>::TY * nz <= 1024 ? 2 : 0);
+        if (sizeof(R) == 8 && !(es && atoi(es) == 0) && !wide && nz <= RowsCfg<R, 3>::ZS) shape = 3;
         const int TYs = rows_ty<R>(shape), NTM = RowsCfg<R, 0>::NTMAX;
         const int N0 = RowsCfg<R, 0>::NHT0;
         const int nt = ((TYs * nz + 63) / 64) * 64;

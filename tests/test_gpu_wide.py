@@ -35,10 +35,10 @@ def _small_cases(tmp):
     return {
         # (path, precision, environment)
         "rows3d_fp64": (write_case(str(tmp), "r64", ndim=3, nx=33, ny=17, nz=21, dx=0.1, nt=6,
-                                   loads=_nuc(1.6, 1.0, ndim=3)), 0, {}),
-        # (per-field descriptors take 4-row workgroups; FP32 rounding in the
-        # one-sided y rows depends on the workgroup's row count, so the
-        # single-descriptor run takes 4 rows as well)
+                                   loads=_nuc(1.6, 1.0, ndim=3)), 0, {"FDF_ROWS_SHAPE": "0"}),
+        # (per-field descriptors take the 4-row FP32 / 2-row FP64 workgroups;
+        # rounding in the one-sided y rows depends on the workgroup's row
+        # count, so the single-descriptor runs take them as well)
         "rows3d_fp32": (write_case(str(tmp), "r32", ndim=3, nx=33, ny=17, nz=21, dx=0.1, nt=6, sbporder=3,
                                    loads=_nuc(1.6, 1.0, ndim=3)), 1, {"FDF_ROWS_SHAPE": "0"}),
         "tiled3d_fp64": (write_case(str(tmp), "t64", ndim=3, nx=33, ny=17, nz=21, dx=0.1, nt=6,
