@@ -454,28 +454,35 @@ static const int rows_grp_y = 16, rows_grp_x = 2;
 // SIMD) or FP64 2 rows (<= 512 threads, 2 waves per SIMD), LDS rows padded to
 // 256 points; 1 = FP32 6 rows of at most 160 points, LDS rows padded to 160;
 // 2 = FP32 5 rows of at most 204 points; 3 = FP64 4 rows of at most 128
-// points, LDS rows padded to 128.  As many rows as fit the thread budget, so
-// that a workgroup fills the wave slots its registers allow (FP32 16, FP64 8
-// per CU).
+// points, LDS rows padded to 128; 4 = FP32 sbporder 4 (more than 128
+// registers): 6 rows of at most 128 points, <= 768 threads at 3 waves per
+// SIMD.  As many rows as fit the thread budget, so that a workgroup fills
+// the wave slots its registers allow (FP32 16 or 12, FP64 8 per CU).
 template <typename R, int SH>
 struct RowsCfg {
   static constexpr bool F32 = sizeof(R) == 4;
-  static constexpr int TY = F32 ? (SH == 1 ? 6 : (SH == 2 ? 5 : 4)) : (SH == 3 ? 4 : 2);
-  static constexpr int ZS = SH == 1 ? 160 : (SH == 3 ? 128 : 256);
-  static constexpr int NTMAX = F32 ? 1024 : 512;
-  static constexpr int WPE = F32 ? 4 : 2;
+  static constexpr int TY = F32 ? (SH == 1 ? 6 : (SH == 2 ? 5 : (SH == 4 ? 6 : 4))) : (SH == 3 ? 4 : 2);
+  static constexpr int ZS = SH == 1 ? 160 : (SH == 3 || SH == 4 ? 128 : 256);
+  static constexpr int NTMAX = F32 ? (SH == 4 ? 768 : 1024) : 512;
+  static constexpr int WPE = F32 ? (SH == 4 ? 3 : 4) : 2;
   static constexpr int NHT0 = F32 ? 1 : 2;  // halo entries per thread: NHT0 or NHT0 + 1
 };
 
 template <typename R>
 static int rows_ty(int shape) {
   return shape == 1 ? RowsCfg<R, 1>::TY
-                    : (shape == 2 ? RowsCfg<R, 2>::TY : (shape == 3 ? RowsCfg<R, 3>::TY : RowsCfg<R, 0>::TY));
+                    : (shape == 2 ? RowsCfg<R, 2>::TY
+                                  : (shape == 3 ? RowsCfg<R, 3>::TY : (shape == 4 ? RowsCfg<R, 4>::TY : RowsCfg<R, 0>::TY)));
 }
 template <typename R>
 static int rows_zs(int shape) {
   return shape == 1 ? RowsCfg<R, 1>::ZS
-                    : (shape == 2 ? RowsCfg<R, 2>::ZS : (shape == 3 ? RowsCfg<R, 3>::ZS : RowsCfg<R, 0>::ZS));
+                    : (shape == 2 ? RowsCfg<R, 2>::ZS
+                                  : (shape == 3 ? RowsCfg<R, 3>::ZS : (shape == 4 ? RowsCfg<R, 4>::ZS : RowsCfg<R, 0>::ZS)));
+}
+template <typename R>
+static int rows_ntmax(int shape) {
+  return shape == 4 ? RowsCfg<R, 4>::NTMAX : RowsCfg<R, 0>::NTMAX;
 }
 
 template <typename R, int S, int NHT, int SH, bool WIDE = false>
@@ -493,8 +500,12 @@ static const void* rows_kernel(int var, int shape, bool wide) {
     else return rows_kernel_sh<R, S, NHT, 0, true>(var);
   }
   if constexpr (sizeof(R) == 4) {
-    if (shape == 1) return rows_kernel_sh<R, S, NHT, 1>(var);
-    if (shape == 2) return rows_kernel_sh<R, S, NHT, 2>(var);
+    if constexpr (S == 4) {
+      if (shape == 4) return rows_kernel_sh<R, S, NHT, 4>(var);
+    } else {
+      if (shape == 1) return rows_kernel_sh<R, S, NHT, 1>(var);
+      if (shape == 2) return rows_kernel_sh<R, S, NHT, 2>(var);
+    }
   } else {
     if (shape == 3) return rows_kernel_sh<R, S, NHT, 3>(var);
   }
@@ -794,7 +805,10 @@ Solver<R>::Solver(const std::string& path, const fdf_opts& o) {
         if (sizeof(R) == 4 && !(es && atoi(es) == 0) && !wide)
           shape = nz <= RowsCfg<R, 1>::ZS ? 1 : (RowsCfg<R, 2>::TY * nz <= 1024 ? 2 : 0);
         if (sizeof(R) == 8 && !(es && atoi(es) == 0) && !wide && nz <= RowsCfg<R, 3>::ZS) shape = 3;
-        const int TYs = rows_ty<R>(shape), NTM = RowsCfg<R, 0>::NTMAX;
+        // FP32 sbporder 4 takes row tiles only in the 3-wave shape (rows of <= 128)
+        const bool f32s4 = sizeof(R) == 4 && S.sbporder == 4;
+        if (f32s4) shape = (!wide && nz <= RowsCfg<R, 4>::ZS && RowsCfg<R, 4>::TY * nz <= RowsCfg<R, 4>::NTMAX) ? 4 : -1;
+        const int TYs = rows_ty<R>(shape), NTM = rows_ntmax<R>(shape);
         const int N0 = RowsCfg<R, 0>::NHT0;
         const int nt = ((TYs * nz + 63) / 64) * 64;
         const int nht = (2 * r * nz + nt - 1) / nt;
@@ -808,7 +822,7 @@ Solver<R>::Solver(const std::string& path, const fdf_opts& o) {
         if (rows_lds<R>(S.sbporder, var, shape) + 1024 > (size_t)maxlds) var &= ~1;
         const size_t lds = rows_lds<R>(S.sbporder, var, shape);
         // (FP32 with sbporder 4 exceeds the 128-register budget of 1024 threads)
-        const bool fits = !(sizeof(R) == 4 && S.sbporder == 4);
+        const bool fits = shape >= 0;
         if (want && fits && nz <= 256 && nt <= NTM && nht <= N0 + 1 && lds + 1024 <= (size_t)maxlds) {
           ta.rows_nht = std::max(nht, N0);
           ta.rows_var = var;

@@ -275,19 +275,23 @@ def test_mode2_curved_plastic(tmp_path):
     _check(p, 15)
 
 
-@pytest.mark.parametrize("precision,shape,nz", [("fp64", "auto", 71), ("fp32", "auto", 71), ("fp32", "0", 71),
-                                                ("fp32", "auto", 181)])
+@pytest.mark.parametrize("precision,shape,nz,sbp", [("fp64", "auto", 71, 3), ("fp64", "0", 71, 3),
+                                                    ("fp64", "auto", 181, 3), ("fp64", "auto", 71, 4),
+                                                    ("fp32", "auto", 71, 3), ("fp32", "0", 71, 3),
+                                                    ("fp32", "auto", 181, 3), ("fp32", "auto", 71, 4)])
 @pytest.mark.parametrize("var", [1, 2])
-def test_row_kernel_variants_random_state(tmp_path, monkeypatch, precision, shape, nz, var):
+def test_row_kernel_variants_random_state(tmp_path, monkeypatch, precision, shape, nz, sbp, var):
     """Every variant of the row-tile kernel (FDF_ROWS=1 single-buffered,
     2 double-buffered LDS tile; FP32 workgroups of 6 rows for rows of at
-    most 160 points, 5 rows up to 204, or with FDF_ROWS_SHAPE=0 always 4)
+    most 160 points, 5 rows up to 204, or with FDF_ROWS_SHAPE=0 always 4;
+    FP64 workgroups of 4 rows for rows of at most 128 points, else 2; FP32
+    sbporder 4 in 6-row workgroups of 768 threads for rows of at most 128)
     reproduces the oracle's RK stages from a random state."""
     from fdfault_amd import lib
     monkeypatch.setenv("FDF_ROWS", str(var))
     if shape != "auto":
         monkeypatch.setenv("FDF_ROWS_SHAPE", shape)
-    p = write_case(str(tmp_path), "rv", ndim=3, nx=37, ny=22, nz=nz, dx=0.1, nt=2, sbporder=3)
+    p = write_case(str(tmp_path), "rv", ndim=3, nx=37, ny=22, nz=nz, dx=0.1, nt=2, sbporder=sbp)
     o = O.Oracle(p)
     g = lib.Solver(p, precision=lib.FP32 if precision == "fp32" else lib.FP64)
     assert g.info.row_blocks == 2
@@ -304,7 +308,8 @@ def test_row_kernel_variants_random_state(tmp_path, monkeypatch, precision, shap
         o.stage(0, st)
         g.stage(0, st)
         edf, ef = _rel(g.array("df"), o.array("df")), _rel(g.array("f"), o.array("f"))
-        print(precision, "shape", shape, "FDF_ROWS", var, "stage", st, "df %.2e f %.2e" % (edf, ef), flush=True)
+        print(precision, "shape", shape, "sbporder", sbp, "FDF_ROWS", var, "stage", st, "df %.2e f %.2e" % (edf, ef),
+              flush=True)
         assert edf <= tol and ef <= tol
     g.close()
     o.close()
