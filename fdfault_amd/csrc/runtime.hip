@@ -1846,7 +1846,10 @@ bool Solver<R>::drain_output(bool wait) {
   OutStage& O = ostage[u];
   if (!wait) {
     const hipError_t q = hipEventQuery(O.copied[k]);
-    if (q == hipErrorNotReady) return false;
+    if (q == hipErrorNotReady) {
+      (void)hipGetLastError();  // "not ready" is no error: keep it out of the next launch check
+      return false;
+    }
     HIPCHK(q);
   }
   HIPCHK(hipEventSynchronize(O.copied[k]));
