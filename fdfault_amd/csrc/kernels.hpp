@@ -676,7 +676,12 @@ struct NoHook {
   __device__ double operator()(double, double, double, double) const { return NAN; }
 };
 
-template <int M, typename R, int LAW = -1, typename Hook = NoHook>
+// LEAN: the same arithmetic in fewer registers (about 115 VGPRs against 170
+// for slip weakening in 2-D), for a kernel that runs it beside the bulk
+// tiles: the perturbation shape values and the fault state the update needs
+// are loaded where they are used instead of up front, and the interface
+// frame is formed again before the SAT terms instead of being held.
+template <int M, typename R, int LAW = -1, typename Hook = NoHook, bool LEAN = false>
 __device__ __forceinline__ void fault_point(const FaultDev& F, const GridDev& g, const FaultStageDev& fs,
                                             const R* __restrict__ f, const R* __restrict__ mat,
                                             const double* __restrict__ hs, int* __restrict__ err,
@@ -688,32 +693,39 @@ __device__ __forceinline__ void fault_point(const FaultDev& F, const GridDev& g,
   // write of this function, which may not alias them)
   double u_p = 0., du_p = 0., ux_p[2] = {0., 0.}, dux_p[2] = {0., 0.}, state_p = 0., dstate_p = 0., front_p = 0.;
   double lxy[4] = {0., 0., 0., 0.}, pxy[4] = {0., 0., 0., 0.};
-  if (law != 0) {
-    const int nn_ = F.n;
-    u_p = F.u[p];
+  auto load_state = [&]() {  // what the slip/state update reads
     du_p = F.du[p];
 #pragma unroll
     for (int q = 0; q < ((M == 2) ? 2 : 1); q++) {
-      ux_p[q] = F.ux[(long long)q * nn_ + p];
-      dux_p[q] = F.dux[(long long)q * nn_ + p];
+      ux_p[q] = F.ux[(long long)q * F.n + p];
+      dux_p[q] = F.dux[(long long)q * F.n + p];
     }
+  };
+  if (law != 0) {
+    const int nn_ = F.n;
+    u_p = F.u[p];
+    if constexpr (!LEAN) load_state();
     if (law == 3) {
       state_p = F.state[p];
       dstate_p = F.dstate[p];
     }
-    if (F.front && F.front_field == 0) front_p = F.front[p];
+    if constexpr (!LEAN) {
+      if (F.front && F.front_field == 0) front_p = F.front[p];
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
-      if (k < F.nloads) lxy[k] = F.load_xy[(long long)k * nn_ + p];
-      if (k < F.nperts) pxy[k] = F.pert_xy[(long long)k * nn_ + p];
+      for (int k = 0; k < 4; k++) {
+        if (k < F.nloads) lxy[k] = F.load_xy[(long long)k * nn_ + p];
+        if (k < F.nperts) pxy[k] = F.pert_xy[(long long)k * nn_ + p];
+      }
     }
   }
   // shape value of load / perturbation k at this point (preloaded for k < 4)
   auto load_x = [&](int k) {
-    return k == 0 ? lxy[0] : k == 1 ? lxy[1] : k == 2 ? lxy[2] : k == 3 ? lxy[3] : F.load_xy[(long long)k * F.n + p];
+    if constexpr (LEAN) return F.load_xy[(long long)k * F.n + p];
+    else return k == 0 ? lxy[0] : k == 1 ? lxy[1] : k == 2 ? lxy[2] : k == 3 ? lxy[3] : F.load_xy[(long long)k * F.n + p];
   };
   auto pert_x = [&](int k) {
-    return k == 0 ? pxy[0] : k == 1 ? pxy[1] : k == 2 ? pxy[2] : k == 3 ? pxy[3] : F.pert_xy[(long long)k * F.n + p];
+    if constexpr (LEAN) return F.pert_xy[(long long)k * F.n + p];
+    else return k == 0 ? pxy[0] : k == 1 ? pxy[1] : k == 2 ? pxy[2] : k == 3 ? pxy[3] : F.pert_xy[(long long)k * F.n + p];
   };
   const int ii = p / F.n1, jj = p % F.n1;
   const long long N = g.N;
@@ -743,26 +755,32 @@ __device__ __forceinline__ void fault_point(const FaultDev& F, const GridDev& g,
       q = cs2 / cp2; gamma2 = 1. - 2. * (q * q);
     }
   }
-  double nn[3] = {0., 0., 0.}, t1[3], t2[3];
-  const int ndim = (M == 2) ? 3 : 2;
-  for (int k = 0; k < ndim; k++) nn[k] = F.nrm[(long long)k * F.n + p];
-  if (F.dir == 0) {
-    t1[2] = 0.;
-    t1[1] = nn[0] / sqrt(nn[0] * nn[0] + nn[1] * nn[1]);
-    t1[0] = -nn[1] / sqrt(nn[0] * nn[0] + nn[1] * nn[1]);
-  } else if (F.dir == 1) {
-    t1[2] = 0.;
-    t1[0] = nn[1] / sqrt(nn[0] * nn[0] + nn[1] * nn[1]);
-    t1[1] = -nn[0] / sqrt(nn[0] * nn[0] + nn[1] * nn[1]);
-  } else {
-    t1[1] = 0.;
-    t1[0] = nn[2] / sqrt(nn[0] * nn[0] + nn[2] * nn[2]);
-    t1[2] = -nn[0] / sqrt(nn[0] * nn[0] + nn[2] * nn[2]);
-  }
-  t2[0] = nn[1] * t1[2] - nn[2] * t1[1];
-  t2[1] = nn[2] * t1[0] - nn[0] * t1[2];
-  t2[2] = nn[0] * t1[1] - nn[1] * t1[0];
-  if (F.dir == 1) { t2[0] = -t2[0]; t2[1] = -t2[1]; t2[2] = -t2[2]; }
+  // the interface frame at this point: unit normal nn, tangents t1 and t2
+  // (interface.cpp:647-686)
+  auto frame = [&](double* nn, double* t1, double* t2) {
+    nn[0] = nn[1] = nn[2] = 0.;
+    const int ndim = (M == 2) ? 3 : 2;
+    for (int k = 0; k < ndim; k++) nn[k] = F.nrm[(long long)k * F.n + p];
+    if (F.dir == 0) {
+      t1[2] = 0.;
+      t1[1] = nn[0] / sqrt(nn[0] * nn[0] + nn[1] * nn[1]);
+      t1[0] = -nn[1] / sqrt(nn[0] * nn[0] + nn[1] * nn[1]);
+    } else if (F.dir == 1) {
+      t1[2] = 0.;
+      t1[0] = nn[1] / sqrt(nn[0] * nn[0] + nn[1] * nn[1]);
+      t1[1] = -nn[0] / sqrt(nn[0] * nn[0] + nn[1] * nn[1]);
+    } else {
+      t1[1] = 0.;
+      t1[0] = nn[2] / sqrt(nn[0] * nn[0] + nn[2] * nn[2]);
+      t1[2] = -nn[0] / sqrt(nn[0] * nn[0] + nn[2] * nn[2]);
+    }
+    t2[0] = nn[1] * t1[2] - nn[2] * t1[1];
+    t2[1] = nn[2] * t1[0] - nn[0] * t1[2];
+    t2[2] = nn[0] * t1[1] - nn[1] * t1[0];
+    if (F.dir == 1) { t2[0] = -t2[0]; t2[1] = -t2[1]; t2[2] = -t2[2]; }
+  };
+  double nn[3], t1[3], t2[3];
+  frame(nn, t1, t2);
   BFd b1 = gather_bf<M, R>(f, N, id1), b2 = gather_bf<M, R>(f, N, id2);
   const double* s0 = g.s0;
   if constexpr (M == 2) {
@@ -982,6 +1000,7 @@ __device__ __forceinline__ void fault_point(const FaultDev& F, const GridDev& g,
       hs12 -= l2; hs22 -= l2; hs13 -= l3; hs23 -= l3;
     }
     if (!fs.no_sat) {  // friction::scale_df + calc_df + update for this point
+      if constexpr (LEAN) load_state();
       double du = du_p * fs.A;
       du += fs.dt * V;
       F.du[p] = du;
@@ -1003,8 +1022,11 @@ __device__ __forceinline__ void fault_point(const FaultDev& F, const GridDev& g,
         F.state[p] = state_p + fs.B * dst;
       }
     }
-    if (F.front && tfront >= 0. && F.front_field == 0 && front_p < 0. && V >= F.front_value)
-      F.front[p] = tfront;
+    if constexpr (LEAN) {
+      if (F.front && tfront >= 0. && F.front_field == 0 && V >= F.front_value && F.front[p] < 0.) F.front[p] = tfront;
+    } else {
+      if (F.front && tfront >= 0. && F.front_field == 0 && front_p < 0. && V >= F.front_value) F.front[p] = tfront;
+    }
   }
   if (fs.no_sat) return;
   // SAT terms for both sides (interface.cpp:802-962)
@@ -1022,6 +1044,10 @@ __device__ __forceinline__ void fault_point(const FaultDev& F, const GridDev& g,
   r2.s22 = gamma2 * (r2.s11 - hs21);
   r2.s33 = gamma2 * (r2.s11 - hs21);
   r2.s11 -= hs21; r2.s12 = 0.; r2.s13 = 0.; r2.s23 = 0.;
+  if constexpr (LEAN) {  // formed again rather than held through the friction solve
+    __builtin_amdgcn_sched_barrier(0);
+    frame(nn, t1, t2);
+  }
   BFd o1 = rot_to_xy(r1, nn, t1, t2), o2 = rot_to_xy(r2, nn, t1, t2);
   if constexpr (M == 2) {
     const double* a1 = &o1.v1;
@@ -2614,10 +2640,9 @@ struct FaultPair2D {
   int p0, np;        // first fault point of the chunk and its point count
 };
 
-// The fault code sets the kernel's register count (about 200 VGPRs, 2 waves
-// per SIMD).  Capping it at the tile kernel's count spills the fault part to
-// scratch, and a kernel with scratch had fewer waves in flight: measured
-// slower in both precisions than the uncapped kernel.
+// The fault part (fault_point's lean form) sets the kernel's register count,
+// about 135 VGPRs (3 waves per SIMD).  Capping it lower spills to scratch,
+// and the kernels with scratch measured slower than the uncapped kernel.
 template <typename R, typename C, int S, int TW, int LAW>
 __global__ void __launch_bounds__(TW) bulk_fault2d_batch(const Tiled2D* __restrict__ descs, const int2* __restrict__ work, int nwork, int par, int dfm,
                    const FaultBatchDev* __restrict__ fdv, const FaultPair2D* __restrict__ pairs, FaultStageDev fs,
@@ -2632,9 +2657,9 @@ __global__ void __launch_bounds__(TW) bulk_fault2d_batch(const Tiled2D* __restri
         const FaultBatchDev& I = fdv[pr.fd];
         FaultStageDev fsi = fs;
         fsi.dt = I.dt;
-        fault_point<0, R, LAW>(I.F, I.g, fsi, reinterpret_cast<const R*>(I.f[par]),
-                               reinterpret_cast<const R*>(I.mat), I.hs, err, nullptr, pr.p0 + (int)threadIdx.x,
-                               nullptr);
+        fault_point<0, R, LAW, NoHook, true>(I.F, I.g, fsi, reinterpret_cast<const R*>(I.f[par]),
+                                             reinterpret_cast<const R*>(I.mat), I.hs, err, nullptr,
+                                             pr.p0 + (int)threadIdx.x, nullptr);
       }
       __syncthreads();  // the SAT terms in global memory, visible to the workgroup
       nt = 2;

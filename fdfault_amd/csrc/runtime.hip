@@ -303,7 +303,7 @@ class Solver : public SolverBase {
   // ... and, for one slip-weakening y-interface whose two blocks' tiles line
   // up in x, the same launch with the fault solve in it (bulk_fault2d_batch):
   // work items (block, tile) and (-1 - chunk), one fault descriptor, the
-  // chunks (FP64 by default; FDF_FUSE2D=1 forces it, 0 turns it off)
+  // chunks (on by default; FDF_FUSE2D=0 turns it off)
   bool fused2d = false;
   int mnfwork = 0;
   DBuf<int2> mfwork;
@@ -1134,12 +1134,12 @@ Solver<R>::Solver(const std::string& path, const fdf_opts& o) {
         }
       if (merged == 2) mbd.upload(bdev);
     }
-    // on by default in FP64 only: the fault code sets the fused kernel's
-    // registers (2 waves per SIMD), which the FP64 tiles (3 waves per SIMD on
-    // their own) lose less to than the separate launch costs, and the FP32
-    // tiles (6 waves per SIMD) more (FDF_FUSE2D=1 forces it, 0 turns it off)
+    // on by default (FDF_FUSE2D=0 turns it off): the fault code (its lean
+    // form, about 135 VGPRs) sets the fused kernel's registers, 3 waves per
+    // SIMD, which the tiles of a single problem lose less to than a separate
+    // fault launch costs
     const char* ef = getenv("FDF_FUSE2D");
-    const bool want = ef ? atoi(ef) != 0 : sizeof(R) == 8;
+    const bool want = ef ? atoi(ef) != 0 : true;
     if (merged == 1 && !tiles2[0].wide && S.ifaces.size() == 1 && want) {
       const IfaceH& I = S.ifaces[0];
       std::vector<FaultPair2D> pairs;
@@ -2326,8 +2326,9 @@ void Batch<R>::build(int tw_, int cx_, int persist_) {
   {
     std::vector<FaultPair2D> pairs;
     std::vector<std::pair<int, int>> skip;  // (descriptor, y tile)
-    // off by default: at 2 waves per SIMD the ensemble's large bulk stage
-    // loses more than the separate fault launch costs (FDF_FUSE2D=1 forces it)
+    // off by default: at 3 waves per SIMD the ensemble's large bulk stage
+    // (the FP32 tiles run 6 on their own) loses more than the separate fault
+    // launch costs (FDF_FUSE2D=1 forces it)
     const char* ef = getenv("FDF_FUSE2D");
     bool ok = ef && atoi(ef) != 0;
     int base = 0;

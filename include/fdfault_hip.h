@@ -52,9 +52,10 @@ typedef struct fdf_info {
                         addresses each field through its own buffer
                         descriptor (nfields N elements above 4 GiB)           */
   int fused_faults;  /* interfaces whose friction solve runs inside the stage's
-                        bulk launch (2-D, one slip-weakening y-interface
-                        between two tiled blocks); the others have a fault
-                        launch of their own before the bulk launch           */
+                        bulk launch (2-D single problems with one
+                        slip-weakening interface along x between two tiled
+                        blocks); the others have a fault launch of their own
+                        before the bulk launch                               */
 } fdf_info;
 
 /* Replaces problem::problem(const char*) (src/problem.cpp:16-84): parses the

@@ -6,7 +6,7 @@ then updates the chunk's two interface tiles, so a stage is one launch
 instead of a fault launch followed by a bulk launch.  Every fault point and
 every bulk point is still computed once with the same arithmetic, so the
 results are bit-identical to the separate launches (FDF_FUSE2D=0; the
-fused launch is the default for single FP64 problems and forced elsewhere
+fused launch is the default for single problems and forced for ensembles
 by FDF_FUSE2D=1): fields,
 df, slip, slip rate, tractions and rupture fronts, in FP32 and FP64, eager
 and graph-replayed, for every SBP order, for x-chunks that do not divide the
@@ -135,7 +135,7 @@ def test_fused_default_per_precision(tmp_path, monkeypatch):
     p = _case(tmp_path, "df", 61, 31, nt=2)
     a = lib.Solver(p, precision=lib.FP64)
     b = lib.Solver(p, precision=lib.FP32)
-    assert a.info.fused_faults == 1 and b.info.fused_faults == 0
+    assert a.info.fused_faults == 1 and b.info.fused_faults == 1
     a.close()
     b.close()
     c = lib.Batch([p, p], precision=lib.FP64)
