@@ -2606,7 +2606,10 @@ struct FaultBatchDev {
   double dt;  // this instance's time step (stage times come from the step counter)
 };
 
-template <int M, typename R>
+// LAW: the friction law every instance shares (-1: any, chosen per point);
+// slip weakening takes fault_point's lean form (fewer registers, so more
+// waves in flight over the ensemble's faults)
+template <int M, typename R, int LAW = -1>
 __global__ void __launch_bounds__(256) fault_batch_kernel(const FaultBatchDev* __restrict__ inst,
                                                           const int2* __restrict__ work, int nwork,
                                                           FaultStageDev fs, int par, int* __restrict__ err) {
@@ -2617,8 +2620,8 @@ __global__ void __launch_bounds__(256) fault_batch_kernel(const FaultBatchDev* _
     FaultStageDev fsi = fs;
     fsi.dt = I.dt;
     if (p < I.F.n)
-      fault_point<M, R>(I.F, I.g, fsi, reinterpret_cast<const R*>(I.f[par]), reinterpret_cast<const R*>(I.mat), I.hs,
-                        err, nullptr, p, nullptr);
+      fault_point<M, R, LAW, NoHook, LAW == 2>(I.F, I.g, fsi, reinterpret_cast<const R*>(I.f[par]),
+                                               reinterpret_cast<const R*>(I.mat), I.hs, err, nullptr, p, nullptr);
   }
 }
 

@@ -2206,6 +2206,7 @@ class Batch : public BatchBase {
   DBuf<int2> fwork;
   int nfwork = 0;
   std::vector<int> inst_fd;  // fault descriptor of each instance's first interface
+  int flaw = -1;             // the friction law every interface shares (-1: mixed)
   // every instance with one slip-weakening y-interface whose tiles line up:
   // one launch per stage (bulk_fault2d_batch) over the tiles and the fault
   // chunks of all instances (off by default; FDF_FUSE2D=1 forces it)
@@ -2277,6 +2278,8 @@ Batch<R>::Batch(const std::vector<std::string>& paths, const fdf_opts& o) : opts
       fd.push_back(d);
     }
   }
+  for (size_t k = 0; k < fd.size(); k++) flaw = (k == 0 || fd[k].F.law == flaw) ? fd[k].F.law : -2;
+  if (flaw < 0) flaw = -1;
   fdescs.upload(fd);
   fwork.upload(fw);
   nfwork = (int)fw.size();
@@ -2424,9 +2427,14 @@ void Batch<R>::launch_stage(int st) {
     return;
   }
   const int fgrid = std::max(1, std::min(nfwork, 4096));
-  if (nfwork)
-    hipLaunchKernelGGL((fault_batch_kernel<0, R>), dim3(fgrid), dim3(256), 0, stream, fdescs.p, fwork.p, nfwork, fs, cur,
-                       err.p);
+  if (nfwork) {
+    if (flaw == LAW_SLIPWEAK)
+      hipLaunchKernelGGL((fault_batch_kernel<0, R, LAW_SLIPWEAK>), dim3(fgrid), dim3(256), 0, stream, fdescs.p, fwork.p,
+                         nfwork, fs, cur, err.p);
+    else
+      hipLaunchKernelGGL((fault_batch_kernel<0, R>), dim3(fgrid), dim3(256), 0, stream, fdescs.p, fwork.p, nfwork, fs,
+                         cur, err.p);
+  }
   const int dfm = stage_dfmode(st);
   if (tw == 64) launch_bulk_tw<64>(st, dfm);
   else if (tw == 128) launch_bulk_tw<128>(st, dfm);
