@@ -1771,7 +1771,7 @@ void Solver<R>::step(int n) {
 }
 
 // staging rings of the output units: as many slots of a unit as fit 64 MB,
-// 2 to 16
+// 2 to 16 (FDF_OUTPUT_SLOTS=k caps them at k)
 template <typename R>
 void Solver<R>::setup_output_staging() {
   HIPCHK(hipStreamCreateWithFlags(&ostream, hipStreamNonBlocking));
@@ -1783,6 +1783,8 @@ void Solver<R>::setup_output_staging() {
     O.cnt = (size_t)U.nx[0] * U.nx[1] * U.nx[2];
     if (!O.cnt) continue;
     O.nslot = (int)std::max<size_t>(2, std::min<size_t>(16, ((size_t)64 << 20) / (O.cnt * sizeof(double))));
+    if (const char* e = getenv("FDF_OUTPUT_SLOTS"))  // fewer slots (tests of a full ring)
+      if (atoi(e) >= 1) O.nslot = std::min(O.nslot, atoi(e));
     O.dev.alloc(O.cnt * O.nslot);
     HIPCHK(hipHostMalloc((void**)&O.host, O.cnt * O.nslot * sizeof(double), hipHostMallocDefault));
     O.gathered.resize(O.nslot);

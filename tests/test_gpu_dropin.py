@@ -57,6 +57,37 @@ def test_executable_files_match_oracle(tmp_path, exact):
             assert np.abs(x - y).max() <= 1e-10 * scale, fn
 
 
+@pytest.mark.parametrize("slots", ["1", "2"])
+def test_output_files_with_small_staging_rings(tmp_path, monkeypatch, slots):
+    """Output units written through staging rings of one or two slots (the
+    host waits for each unit's oldest copy on almost every output step) give
+    the same files, byte for byte, as the default rings, eager and graph
+    replayed."""
+    from fdfault_amd import lib
+    outs = [Output("vx", "vx", (0, 30, 1), (0, 60, 2), (0, 61, 3), (0, 0, 1)),
+            Output("V", "V", (0, 30, 1), (0, 60, 1), (31, 31, 1), (0, 0, 1)),
+            Output("sxy", "sxy", (3, 27, 4), (10, 50, 1), (25, 36, 1), (0, 0, 1))]
+    dirs = []
+    for k, (env, graph) in enumerate(((None, False), (slots, False), (slots, True))):
+        if env is None:
+            monkeypatch.delenv("FDF_OUTPUT_SLOTS", raising=False)
+        else:
+            monkeypatch.setenv("FDF_OUTPUT_SLOTS", env)
+        p = write_case(str(tmp_path), "r%d" % k, nx=61, ny=31, dx=0.1, nt=30, outputs=outs, front=("V", 0.001))
+        s = lib.Solver(p, graph=graph, write_outputs=True)
+        s.step(30)
+        s.finish()
+        s.close()
+        dirs.append((os.path.join(os.path.dirname(p), "data"), "r%d" % k))
+    base, bname = dirs[0]
+    files = sorted(os.listdir(base))
+    assert len(files) >= 15
+    for d, name in dirs[1:]:
+        for fn in files:
+            other = os.path.join(d, name + fn[len(bname):])
+            assert filecmp.cmp(os.path.join(base, fn), other, shallow=False), (name, fn)
+
+
 def test_executable_error_message(tmp_path):
     bad = tmp_path / "bad.in"
     bad.write_text("[fdfault.problem]\nx\ndata/\n1\n0\n0\n0.3\n1\n7\n")
