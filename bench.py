@@ -544,8 +544,8 @@ def main():
     ap.add_argument("--err-steps", type=int, default=2000)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this build runs on one GPU only (the whole grid fits one MI355X); "
-              "multi-GPU slab decomposition is designed but not built", file=sys.stderr)
+        print("bench.py: fdfault_amd runs on one GPU only (the whole grid fits one MI355X); "
+              "a multi-GPU slab decomposition is designed but not built", file=sys.stderr)
         sys.exit(2)
     if args.workload in ("planar2d", "rough2d"):
         single_workload(args)

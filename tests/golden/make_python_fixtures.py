@@ -1,7 +1,7 @@
 """Writes the fixtures of tests/golden/python/ with the reference's
 unchanged Python setup and analysis module (python/fdfault of the reference
-checkout).  Runs in the build container only, by hand; the GPU box never
-sees the reference.
+checkout).  Runs by hand, on a machine that has the reference checkout;
+the tests read only the committed fixtures.
 
 1. Runs the reference's example scripts problems/tpv5.py (3-D, cohesion
    barriers, on- and off-fault stations, a rupture front) and
