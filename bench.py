@@ -153,6 +153,7 @@ def single_workload(args):
     # the bulk kernel's own algorithmic bytes (the plastic fields are read and
     # written once per step by the plasticity kernel, not by this kernel)
     B = bytes_per_point(nf, w, curvilinear=rough, ndim=2)
+    B_need = bytes_per_point(nf, w, curvilinear=rough, ndim=2, needed=True)
     B_stage = bytes_per_point(nf, w, curvilinear=rough, ndim=2, plastic_words=2 * (4 + 2) if rough else 0)
     # point updates per bulk launch (blocks may share one launch)
     ppl = npts * nst * args.steps / nl
@@ -181,6 +182,8 @@ def single_workload(args):
                      "traffic": tr["traffic"] if tr else None, "traffic_source": tr["summary"] if tr else None,
                      "kernel": kernel, "avg_launch_ms": round(kms, 4),
                      "launches": nl, "bytes_per_point": B, "bytes_per_point_stage_with_plasticity": B_stage,
+                     "bytes_per_point_needed": B_need,
+                     "frac_at_bytes_needed": round(B_need * ppl / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                      "points_per_launch": int(ppl),
                      "fault_ms_per_stage": round(fms, 4),
                      "fault_solve_in_bulk_launch": fused,
@@ -330,12 +333,16 @@ def stage_field_words(nstages=5):
     return (3 + 4 * (nstages - 2) + 3) / nstages
 
 
-def bytes_per_point(nf, w, curvilinear=False, ndim=3, plastic_words=0, nstages=5):
-    """Algorithmic bytes per point per RK stage (average over a step): the
-    f/df words of stage_field_words per field + geometry words (curvilinear)
-    + plastic words per step."""
+def bytes_per_point(nf, w, curvilinear=False, ndim=3, plastic_words=0, nstages=5, needed=False):
+    """Algorithmic bytes per point per RK stage: 4 words per field (read f
+    and df, write df and f: a fused low-storage stage with on-chip stencil
+    reuse) + geometry words (curvilinear) + plastic words per step spread
+    over the stages.  needed=True: the f/df words a stage must move averaged
+    over a step instead (stage_field_words: no df read where A = 0, no df
+    write on the last stage)."""
     G = (ndim * ndim + 1) if curvilinear else 0
-    return w * (stage_field_words(nstages) * nf + G) + w * plastic_words / nstages
+    fw = stage_field_words(nstages) if needed else 4.0
+    return w * (fw * nf + G) + w * plastic_words / nstages
 
 
 def cpu_threads():
@@ -389,6 +396,7 @@ def ensemble(args):
         nst = b.nstages
         stage_ms = t / args.steps / nst * 1e3
         B = bytes_per_point(5, w)
+        B_need = bytes_per_point(5, w, needed=True)
         achieved = B * npts / (stage_ms * 1e-3) / 1e9
         rph = b.n / (t / args.steps * args.ens_nt) * 3600.
         err = None
@@ -434,7 +442,8 @@ def ensemble(args):
                      "traffic_note": "HBM bytes of one bulk_tiled2d_batch launch over every instance "
                                      "(the stage's fault kernel adds about 1.6 B per point)" if tr else None,
                      "kernel": "bulk_tiled2d_batch + fault_batch_kernel (whole stage)",
-                     "stage_ms": round(stage_ms, 4), "bytes_per_point": B},
+                     "stage_ms": round(stage_ms, 4), "bytes_per_point": B, "bytes_per_point_needed": B_need,
+                     "frac_at_bytes_needed": round(B_need * npts / (stage_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
         "error_vs_fp64": err,
     }
     print(json.dumps(line), flush=True)
@@ -645,6 +654,7 @@ def run_planar(lib, path, pname, args, workload):
     ntiled, nrows = g.info.tiled_blocks, g.info.row_blocks
     g.close()
     B = bytes_per_point(9, w)
+    B_need = bytes_per_point(9, w, needed=True)
     # point updates per bulk launch (both blocks may share one launch)
     ppl = npts * nst * args.steps / nlaunch
     achieved = B * ppl / (kms * 1e-3) / 1e9
@@ -656,10 +666,11 @@ def run_planar(lib, path, pname, args, workload):
             "traffic_source": tr["summary"] if tr else None, "kernel": kernel,
             "avg_launch_ms": round(kms, 4), "launches": nlaunch, "bytes_per_point": B,
             "points_per_launch": int(ppl),
-            # the same launches counted at 4 nf words per point and stage (read
-            # f and df, write df and f), the per-stage bytes of a low-storage
-            # update that does not exploit A = 0
-            "frac_at_4nf_words": round(w * 4 * 9 * ppl / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+            # the same launches counted at the bytes a stage must move,
+            # averaged over a step (3.6 nf words: no df read where A = 0, no
+            # df write on the last stage)
+            "bytes_per_point_needed": B_need,
+            "frac_at_bytes_needed": round(B_need * ppl / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
     ref = stream_reference(int(ppl), w) if not args.no_stream else None
     if ref:
         roof.update(ref)
