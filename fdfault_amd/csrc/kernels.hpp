@@ -1286,13 +1286,6 @@ __global__ void __launch_bounds__(256) gather_units(const OutGatherDev* __restri
   U.dst[(long long)L.slot[blockIdx.y] * U.cnt + t] = v;
 }
 
-
-template <typename R>
-__global__ void convert_to_double(const R* __restrict__ src, double* __restrict__ dst, long long n) {
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < n) dst[t] = (double)src[t];
-}
-
 template <typename R>
 __global__ void convert_from_double(const double* __restrict__ src, R* __restrict__ dst, long long n) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1461,30 +1454,6 @@ __device__ __forceinline__ void edge_x(C* out, const FA& F, unsigned psb, unsign
   if (!left)
 #pragma unroll
     for (int q = 0; q < 6; q++) out[q] = -out[q];
-}
-
-// derivative along one direction at a one-sided boundary row, taps read from
-// global memory: left rows use c_fd[pos-lo+1][n] at lo+n, right rows
-// -c_fd[hi-pos][n] at hi-1-n (block.cpp:1057-1162 pattern)
-template <typename R, typename C, int S, int NQ>
-__device__ __forceinline__ void edge_deriv(C* out, const R* __restrict__ f, const int* fq, long long N, long long base,
-                                           long long stride, int pos, int lo, int hi) {
-  constexpr int NT = 3 * (S - 1);
-  int row, start, step;
-  C sg;
-  if (pos - lo < 2 * (S - 1)) { row = pos - lo + 1; start = lo; step = 1; sg = (C)1; }
-  else { row = hi - pos; start = hi - 1; step = -1; sg = (C)-1; }
-#pragma unroll
-  for (int q = 0; q < NQ; q++) out[q] = (C)0;
-#pragma unroll
-  for (int n = 0; n < NT; n++) {
-    const C c = (C)c_fd[row][n];
-    const long long id = base + (long long)(start + step * n) * stride;
-#pragma unroll
-    for (int q = 0; q < NQ; q++) out[q] += c * (C)f[fq[q] * N + id];
-  }
-#pragma unroll
-  for (int q = 0; q < NQ; q++) out[q] *= sg;
 }
 
 // interior first derivative from 2r+1 samples u[0..2r] centred on u[r]; the

@@ -916,7 +916,7 @@ Solver<R>::Solver(const std::string& path, const fdf_opts& o) {
     F.dl1 = ib.dl1.p;
     F.dl2 = ib.dl2.p;
     const size_t n = (size_t)F.n;
-    auto pack = [n](const std::vector<PertH>& P, DBuf<double>& xy, DBuf<double>& par) {
+    auto pack = [](const std::vector<PertH>& P, DBuf<double>& xy, DBuf<double>& par) {
       std::vector<double> hx, hp;
       for (const PertH& p : P) {
         hx.insert(hx.end(), p.xy.begin(), p.xy.end());
