@@ -1306,21 +1306,25 @@ Solver<R>::~Solver() {
   if (stream) (void)hipStreamDestroy(stream);
 }
 
+// the message of the fault kernels' error flags: the reference's text where
+// it stops (stz.cpp:341-344, utilities.cpp:135-136,173-176)
+static std::string fault_error_message(int e) {
+  if (e & 16) return "Slip velocity exceeds melting value in stz.cpp";
+  if (e & 1) return "zero derivative in Newton's method in utilities.cpp";
+  if (e & 2) return "Newton's method failed to converge in utilities.cpp";
+  if (e & 8) return "STZ effective temperature must be positive";
+  return "bracket repair in Newton's method did not terminate";
+}
+
 template <typename R>
 void Solver<R>::sync_check() {
   HIPCHK(hipStreamSynchronize(stream));
   int e = 0;
   HIPCHK(hipMemcpy(&e, err.p, sizeof(int), hipMemcpyDeviceToHost));
   if (e) {
-    std::string m;
-    if (e & 16) m = "Slip velocity exceeds melting value in stz.cpp";
-    else if (e & 1) m = "zero derivative in Newton's method in utilities.cpp";
-    else if (e & 2) m = "Newton's method failed to converge in utilities.cpp";
-    else if (e & 8) m = "STZ effective temperature must be positive";
-    else m = "bracket repair in Newton's method did not terminate";
     int z = 0;
     HIPCHK(hipMemcpy(err.p, &z, sizeof(int), hipMemcpyHostToDevice));
-    throw Error{m};
+    throw Error{fault_error_message(e)};
   }
 }
 
@@ -2618,7 +2622,7 @@ void Batch<R>::sync() {
   if (e) {
     int z = 0;
     HIPCHK(hipMemcpy(err.p, &z, sizeof(int), hipMemcpyHostToDevice));
-    throw Error{"fault solve error flag " + std::to_string(e)};
+    throw Error{fault_error_message(e)};
   }
 }
 

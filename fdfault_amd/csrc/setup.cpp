@@ -78,10 +78,14 @@ class Scanner {
   size_t pos_ = 0;
 };
 
-std::vector<double> load_f8(const std::string& path, size_t n, const char* what) {
+// a binary input file of n doubles; open_msg: the reference's message when
+// the file cannot be opened (fields.cpp:497,627, friction.cpp:555,627,
+// slipweak.cpp:358, stz.cpp:406, surface.cpp:44)
+std::vector<double> load_f8(const std::string& path, size_t n, const std::string& open_msg, const char* what) {
   std::vector<double> v(n);
   std::ifstream in(path, std::ios::binary);
-  if (!in || !in.read(reinterpret_cast<char*>(v.data()), (std::streamsize)(n * sizeof(double))))
+  if (!in) fail(open_msg);
+  if (!in.read(reinterpret_cast<char*>(v.data()), (std::streamsize)(n * sizeof(double))))
     fail(std::string("Error reading ") + what + " from file " + path);
   return v;
 }
@@ -264,7 +268,8 @@ std::vector<double> face_surface(const SetupH& S, const BlockH& B, int face, int
   n0 = B.nx[a];
   n1 = B.nx[b];
   const size_t n = (size_t)n0 * n1;
-  if (B.bfile[face] != "none") return load_f8(B.bfile[face], (size_t)S.ndim * n, "surface");
+  if (B.bfile[face] != "none")
+    return load_f8(B.bfile[face], (size_t)S.ndim * n, "Error reading surface from file " + B.bfile[face], "surface");
   double corner[3] = {B.x0[0], B.x0[1], B.x0[2]};
   if (face & 1) corner[d] = B.x0[d] + B.L[d];
   const double la = B.L[a], lb = B.L[b];
@@ -400,8 +405,9 @@ bool edges_match(const SetupH& S, const SurfView& s1, int e1, const SurfView& s2
   return true;
 }
 
-void read_stack(const std::string& path, int k, size_t n, std::vector<double>* out, const char* what) {
-  std::vector<double> all = load_f8(path, (size_t)k * n, what);
+void read_stack(const std::string& path, int k, size_t n, std::vector<double>* out, const std::string& open_msg,
+                const char* what) {
+  std::vector<double> all = load_f8(path, (size_t)k * n, open_msg, what);
   for (int q = 0; q < k; q++) out[q].assign(all.begin() + (size_t)q * n, all.begin() + (size_t)(q + 1) * n);
 }
 
@@ -536,8 +542,14 @@ void compile_problem(const std::string& path, SetupH& S, bool outputs, bool quie
     S.metric.resize((size_t)S.ndim * S.ndim * S.N);
     S.jac.resize(S.N);
   }
-  if (loadfile != "none") { S.hetstress = true; S.s = load_f8(loadfile, (size_t)S.ns * S.N, "heterogeneous stress"); }
-  if (matfile != "none") { S.hetmat = true; S.mat = load_f8(matfile, (size_t)S.nmat * S.N, "heterogeneous material"); }
+  if (loadfile != "none") {
+    S.hetstress = true;
+    S.s = load_f8(loadfile, (size_t)S.ns * S.N, "Error opening stress file in fields.cpp", "heterogeneous stress");
+  }
+  if (matfile != "none") {
+    S.hetmat = true;
+    S.mat = load_f8(matfile, (size_t)S.nmat * S.N, "Error opening material file in fields.cpp", "heterogeneous material");
+  }
   if (S.hetmat && S.ndim == 2 && S.mode == 3)
     fail("heterogeneous material in mode III is not supported (the reference reads past its material array there)");
   // [fdfault.operator] (optional), block.cpp:91-107
@@ -748,7 +760,7 @@ void compile_problem(const std::string& path, SetupH& S, bool outputs, bool quie
     const std::string lf = sc.word();
     if (lf != "none") {
       I.load_file = true;
-      read_stack(lf, 3, n, I.lf, "load");
+      read_stack(lf, 3, n, I.lf, "Error opening load file in friction.cpp", "load");
       if (S.ndim == 2 && S.mode == 2) std::fill(I.lf[2].begin(), I.lf[2].end(), 0.);
       else if (S.ndim == 2 && S.mode == 3) std::fill(I.lf[1].begin(), I.lf[1].end(), 0.);
     }
@@ -758,7 +770,7 @@ void compile_problem(const std::string& path, SetupH& S, bool outputs, bool quie
       const int np = sc.integer();
       for (int k = 0; k < np; k++) I.perts.push_back(make_pert(sc, 6, I, x2, l2));
       const std::string pf = sc.word();
-      if (pf != "none") { I.param_file = true; read_stack(pf, 6, n, I.pf, "slip weakening parameters"); }
+      if (pf != "none") { I.param_file = true; read_stack(pf, 6, n, I.pf, "Error opening file in friction.cpp", "slip weakening parameters"); }
     } else if (I.law == LAW_STZ) {
       sc.seek(hdr);
       if (!sc.seek("[fdfault.stz]", false)) fail("Error reading stz from input file");
@@ -768,9 +780,9 @@ void compile_problem(const std::string& path, SetupH& S, bool outputs, bool quie
       for (int k = 0; k < np; k++) I.perts.push_back(make_pert(sc, 9, I, x2, l2));
       const std::string pf = sc.word();
       I.state0.assign(n, 0.);
-      if (sfile != "none") I.state0 = load_f8(sfile, n, "state");
+      if (sfile != "none") I.state0 = load_f8(sfile, n, "Error opening state file in friction.cpp", "state");
       for (double& v : I.state0) v += chi0;
-      if (pf != "none") { I.param_file = true; read_stack(pf, 9, n, I.pf, "stz parameters"); }
+      if (pf != "none") { I.param_file = true; read_stack(pf, 9, n, I.pf, "Error opening file in stz.cpp", "stz parameters"); }
     }
   }
   // time step, problem.cpp:94-134

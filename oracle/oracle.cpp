@@ -89,10 +89,15 @@ class InText {
   bool eof_ = false;
 };
 
-static std::vector<double> read_binary(const std::string& path, size_t n, const char* what) {
+// open_msg: the reference's message for a file it cannot open
+// (fields.cpp:497,627, friction.cpp:555,627, slipweak.cpp:358, stz.cpp:406,
+// surface.cpp:44)
+static std::vector<double> read_binary(const std::string& path, size_t n, const std::string& open_msg,
+                                       const char* what) {
   std::vector<double> v(n);
   std::ifstream in(path, std::ios::in | std::ios::binary);
-  if (!in.is_open() || !in.read((char*)v.data(), sizeof(double) * n))
+  if (!in.is_open()) throw Err(open_msg);
+  if (!in.read((char*)v.data(), sizeof(double) * n))
     throw Err(std::string("Error reading ") + what + " from file " + path);
   return v;
 }
@@ -641,11 +646,13 @@ static void read_fields(Problem& P, InText& in) {  // fields.cpp:13-164
   P.jac.assign(P.N, 0.);
   if (P.loadfile != "none") {
     P.hetstress = true;
-    P.s = read_binary(P.loadfile, (size_t)P.ns * P.N, "heterogeneous stress");
+    P.s = read_binary(P.loadfile, (size_t)P.ns * P.N, "Error opening stress file in fields.cpp",
+                      "heterogeneous stress");
   }
   if (P.matfile != "none") {
     P.hetmat = true;
-    P.mat = read_binary(P.matfile, (size_t)P.nmat * P.N, "heterogeneous material");
+    P.mat = read_binary(P.matfile, (size_t)P.nmat * P.N, "Error opening material file in fields.cpp",
+                        "heterogeneous material");
   }
 }
 
@@ -657,7 +664,9 @@ static std::vector<double> make_surface(const Problem& P, const Block& B, int fa
   n0 = B.nx[id0];
   n1 = B.nx[id1];
   size_t n = (size_t)n0 * n1;
-  if (B.bfile[face] != "none") return read_binary(B.bfile[face], (size_t)P.ndim * n, "surface");
+  if (B.bfile[face] != "none")
+    return read_binary(B.bfile[face], (size_t)P.ndim * n, "Error reading surface from file " + B.bfile[face],
+                       "surface");
   double xin[3], lin[2];
   for (int i = 0; i < 3; i++) xin[i] = B.xb[i];
   if (face % 2 == 1) xin[dir] = B.xb[dir] + B.lb[dir];
@@ -903,8 +912,9 @@ static void read_blocks(Problem& P, InText& in) {  // block.cpp:22-264, domain.c
 }
 
 // reads [3]/[6]/[9] x [n0][n1] parameter stacks (friction.cpp:507-567 and friends)
-static void read_stack(const std::string& path, int nstack, size_t n, std::vector<double>* dst, const char* what) {
-  std::vector<double> all = read_binary(path, (size_t)nstack * n, what);
+static void read_stack(const std::string& path, int nstack, size_t n, std::vector<double>* dst,
+                       const std::string& open_msg, const char* what) {
+  std::vector<double> all = read_binary(path, (size_t)nstack * n, open_msg, what);
   for (int k = 0; k < nstack; k++) dst[k].assign(all.begin() + k * n, all.begin() + (k + 1) * n);
 }
 
@@ -1026,7 +1036,7 @@ static void read_interfaces(Problem& P, InText& in) {  // interface.cpp:17-403, 
     if (lf != "none") {
       I.load_file = true;
       std::vector<double> st[3];
-      read_stack(lf, 3, n, st, "load");
+      read_stack(lf, 3, n, st, "Error opening load file in friction.cpp", "load");
       I.ls1 = st[0]; I.ls2 = st[1]; I.ls3 = st[2];
       if (P.ndim == 2 && P.mode == 2) std::fill(I.ls3.begin(), I.ls3.end(), 0.);
       else if (P.ndim == 2 && P.mode == 3) std::fill(I.ls2.begin(), I.ls2.end(), 0.);
@@ -1045,7 +1055,7 @@ static void read_interfaces(Problem& P, InText& in) {  // interface.cpp:17-403, 
         I.perts.push_back(pt);
       }
       std::string pf = in.tok();
-      if (pf != "none") { I.param_file = true; read_stack(pf, 6, n, I.prm, "slip weakening parameters"); }
+      if (pf != "none") { I.param_file = true; read_stack(pf, 6, n, I.prm, "Error opening file in friction.cpp", "slip weakening parameters"); }
     } else if (I.type == 3) {  // stz.cpp:1-230
       I.has_state = true;
       in.rewind();
@@ -1066,9 +1076,9 @@ static void read_interfaces(Problem& P, InText& in) {  // interface.cpp:17-403, 
       I.state.assign(n, 0.);
       I.dstate.assign(n, 0.);
       I.dstatedt.assign(n, 0.);
-      if (sfile != "none") I.state = read_binary(sfile, n, "state");
+      if (sfile != "none") I.state = read_binary(sfile, n, "Error opening state file in friction.cpp", "state");
       for (size_t q = 0; q < n; q++) I.state[q] += I.chi0;
-      if (pf != "none") { I.param_file = true; read_stack(pf, 9, n, I.prm, "stz parameters"); }
+      if (pf != "none") { I.param_file = true; read_stack(pf, 9, n, I.prm, "Error opening file in stz.cpp", "stz parameters"); }
     }
   }
 }

@@ -1,0 +1,70 @@
+"""Error paths of the input readers, on the CPU: a binary input file that
+cannot be opened stops the set-up with the reference's message, in the
+library's host problem compiler (fdf_host_setup) and in the oracle alike.
+The reference prints these lines before MPI_Abort: fields.cpp:497 (stress
+file), fields.cpp:627 (material file), friction.cpp:555 (load file),
+friction.cpp:627 (state file), slipweak.cpp:358 and stz.cpp:406 (parameter
+files), surface.cpp:44 (surface file).  A file that opens but is too short
+is an error too (the reference would read past its end)."""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from cases import STZ_PARAMS, write_case
+from fdfault_amd import lib
+
+MISSING = "/nonexistent/fdfault_input.bin"
+
+
+def _host_setup_error(path):
+    L = lib.load()
+    info = lib.Info()
+    rc = L.fdf_host_setup(path.encode(), ctypes.byref(info), None, None, None)
+    assert rc < 0
+    return L.fdf_last_error().decode()
+
+
+def _oracle_error(path):
+    with pytest.raises(O.OracleError) as e:
+        O.Oracle(path)
+    return str(e.value)
+
+
+CASES = {
+    "stress": (dict(hetstressfile=MISSING), "Error opening stress file in fields.cpp"),
+    "material": (dict(matfile=MISSING), "Error opening material file in fields.cpp"),
+    "load": (dict(loadfile=MISSING), "Error opening load file in friction.cpp"),
+    "sw_param": (dict(paramfile=MISSING), "Error opening file in friction.cpp"),
+    "stz_param": (dict(law="stz", stz=dict(STZ_PARAMS, chi0=0.07, paramfile=MISSING)), "Error opening file in stz.cpp"),
+    "stz_state": (dict(law="stz", stz=dict(STZ_PARAMS, chi0=0., statefile=MISSING)),
+                  "Error opening state file in friction.cpp"),
+}
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_missing_input_file_message(tmp_path, case):
+    kw, msg = CASES[case]
+    p = write_case(str(tmp_path), case, nx=41, ny=21, dx=0.1, nt=1, **kw)
+    assert _host_setup_error(p) == msg
+    assert _oracle_error(p) == msg
+
+
+def test_missing_surface_file_message(tmp_path):
+    surf = [[MISSING, "none", "none", "none"], ["none", "none", "none", "none"]]
+    p = write_case(str(tmp_path), "surf", nx=41, ny=21, dx=0.1, nt=1, surf=surf)
+    msg = "Error reading surface from file " + MISSING
+    assert _host_setup_error(p) == msg
+    assert _oracle_error(p) == msg
+
+
+def test_short_input_file(tmp_path):
+    short = str(tmp_path / "short.bin")
+    np.zeros(5).tofile(short)  # a load file needs 3 x 41 values
+    p = write_case(str(tmp_path), "short", nx=41, ny=21, dx=0.1, nt=1, loadfile=short)
+    msg = "Error reading load from file " + short
+    assert _host_setup_error(p) == msg
+    assert _oracle_error(p) == msg
+    assert os.path.getsize(short) == 40
