@@ -65,7 +65,8 @@ MUTANTS = [
      [LAW + "::test_fault_input_file_equals_perturbation"]),
     ("stz_file_order", "prm[4] += I.prm[0][index];\n        prm[5] += I.prm[1][index];",
      "prm[4] += I.prm[1][index];\n        prm[5] += I.prm[0][index];", [LAW + "::test_fault_input_file_equals_perturbation"]),
-    ("state_file_ignored", "if (sfile != \"none\") I.state = read_binary(sfile, n, \"state\");", "",
+    ("state_file_ignored",
+     "if (sfile != \"none\") I.state = read_binary(sfile, n, \"Error opening state file in friction.cpp\", \"state\");", "",
      [LAW + "::test_stz_state_file_layout"]),
     ("sbp_interior_order", "row(fd.c[0], {-0.5, 0., 0.5});", "row(fd.c[0], {-0.5, 0.01, 0.49});",
      [PINS + "::test_sbp_rows_exact_on_polynomials"]),
