@@ -5,7 +5,9 @@ The reference prints these lines before MPI_Abort: fields.cpp:497 (stress
 file), fields.cpp:627 (material file), friction.cpp:555 (load file),
 friction.cpp:627 (state file), slipweak.cpp:358 and stz.cpp:406 (parameter
 files), surface.cpp:44 (surface file).  A file that opens but is too short
-is an error too (the reference would read past its end)."""
+is an error too (the reference would read past its end).  A folded block
+grid (a non-positive Jacobian), which the reference leaves to an assert in
+its metric code, stops the set-up with a message."""
 import ctypes
 import os
 
@@ -13,7 +15,7 @@ import numpy as np
 import pytest
 
 import oracle_lib as O
-from cases import STZ_PARAMS, write_case
+from cases import STZ_PARAMS, curved_surfaces, write_case
 from fdfault_amd import lib
 
 MISSING = "/nonexistent/fdfault_input.bin"
@@ -68,3 +70,20 @@ def test_short_input_file(tmp_path):
     assert _host_setup_error(p) == msg
     assert _oracle_error(p) == msg
     assert os.path.getsize(short) == 40
+
+
+JAC = "non-positive Jacobian in block grid (grid is not smooth enough)"
+
+
+def folded_case(tmp):
+    """A fault curve that dips below the lower block's bottom face
+    (amplitude 1.5 block heights): that block's grid folds over."""
+    nx, ny, dx = 41, 21, 0.1
+    surf = curved_surfaces(str(tmp), nx, ny, dx, amp=1.5 * (ny - 1) * dx)
+    return write_case(str(tmp), "fold", nx=nx, ny=ny, dx=dx, nt=1, surf=surf)
+
+
+def test_folded_grid_message(tmp_path):
+    p = folded_case(tmp_path)
+    assert _host_setup_error(p) == JAC
+    assert _oracle_error(p) == JAC

@@ -78,3 +78,14 @@ def test_melting_velocity_in_a_batch(tmp_path):
         b.step(10)
     assert MELT in str(e.value)
     b.close()
+
+
+def test_folded_grid_on_the_device(tmp_path):
+    """The grid built on the device stops on a folded block as the host
+    compiler and the oracle do (tests/test_error_paths.py)."""
+    from fdfault_amd import lib
+    from test_error_paths import JAC, folded_case
+    p = folded_case(tmp_path)
+    with pytest.raises(lib.FdfaultError) as e:
+        lib.Solver(p)
+    assert str(e.value) == JAC
