@@ -1,0 +1,138 @@
+"""GPU parity on block layouts other than two blocks split by one y-normal
+fault: faults normal to x and to z, several interfaces per problem and per
+block, and a 2 x 2 block grid whose four interfaces meet at one point, with
+slip-weakening, locked and frictionless interfaces (domain.cpp:33-140,
+interface.cpp:17-403: blocks on a grid, interfaces between neighbours in any
+direction).  The FP64 path, on the tiled kernels and bit-faithful, against
+the oracle at the parity bar (1e-10), and the FP32 path within the FP32 bar
+on the slip."""
+import itertools
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from fdfault_amd.problems import Pert, Problem
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10
+SW = [Pert("constant", 0., 0., 0., 0., 0., (0.4, 0.677, 0.525, 0., 0., 0.))]
+
+
+def layout(tmp, name, ndim, nb, npts, ifaces, mode=2, sbporder=3, dx=0.1, nt=30):
+    """Blocks on an nb[0] x nb[1] x nb[2] grid, each npts points per
+    direction; ifaces: (dir, b1, b2, law, nucleate) with dir in "xyz".
+    Outer faces absorbing, inner faces joined by the interfaces."""
+    nbz = nb[2] if ndim == 3 else 1
+    L = [(npts[d] - 1) * dx for d in range(3)]
+    blocks = []
+    for i, j, k in itertools.product(range(nb[0]), range(nb[1]), range(nbz)):
+        c = (i, j, k)
+        bc = []
+        for d in range(ndim):
+            bc += ["absorbing" if c[d] == 0 else "none", "absorbing" if c[d] == nb[d] - 1 else "none"]
+        blocks.append(dict(coords=c, x0=[i * L[0], j * L[1], k * L[2]], l=L, bc=bc))
+    itfs, laws = [], []
+    for (dr, b1, b2, law, nuc) in ifaces:
+        d = "xyz".index(dr)
+        a, b = [q for q in range(3) if q != d][:2]  # the interface's in-plane coordinates
+        ca, cb = 0.5 * L[a], 0.5 * L[b] if ndim == 3 else 0.
+        loads = []
+        if law != "locked":
+            # shear along the in-plane direction that slips: the second
+            # component, or the third (z) in a 2-D mode III problem
+            m3 = ndim == 2 and mode == 3
+            loads = [Pert("constant", 0., 0., 0., 0., 0., (-120., 0., 70.) if m3 else (-120., 70., 0.))]
+            if nuc:
+                loads.append(Pert("boxcar", 0., b1[a] * L[a] + ca, 0.6, cb, 0.6 if ndim == 3 else 0.,
+                                  (0., 0., 11.6) if m3 else (0., 11.6, 0.)))
+        itfs.append(dict(dir=dr, b1=b1, b2=b2, loads=loads, sw=SW, loadfile="none", paramfile="none"))
+        laws.append(law)
+    nxb = [tuple([npts[d]] * nb[d]) for d in range(2)] + [tuple([npts[2]] * nbz) if ndim == 3 else (1,)]
+    p = Problem(name=name, datadir=str(tmp / name) + "/", nt=nt, ndim=ndim, mode=mode, nx_block=nxb,
+                sbporder=sbporder, iftypes=tuple(laws), blocks=blocks, interfaces=itfs, front=("V", 0.001))
+    (tmp / name).mkdir(exist_ok=True)
+    return p.write(str(tmp / name / (name + ".in")))
+
+
+def _rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+LAYOUTS = {
+    # 2-D mode II, one fault normal to x
+    "x_fault_2d": dict(ndim=2, nb=(2, 1, 1), npts=(31, 41, 1), ifaces=[("x", (0, 0, 0), (1, 0, 0), "slipweak", True)]),
+    # 2-D mode III, one fault normal to x
+    "x_fault_mode3": dict(ndim=2, mode=3, nb=(2, 1, 1), npts=(31, 41, 1),
+                          ifaces=[("x", (0, 0, 0), (1, 0, 0), "slipweak", True)]),
+    # three blocks in y: a slip-weakening fault below a locked interface
+    "stack_3": dict(ndim=2, nb=(1, 3, 1), npts=(51, 21, 1),
+                    ifaces=[("y", (0, 0, 0), (0, 1, 0), "slipweak", True), ("y", (0, 1, 0), (0, 2, 0), "locked", False)]),
+    # 2 x 2 blocks, four interfaces meeting at the centre
+    "grid_2x2": dict(ndim=2, nb=(2, 2, 1), npts=(31, 31, 1),
+                     ifaces=[("x", (0, 0, 0), (1, 0, 0), "slipweak", True), ("x", (0, 1, 0), (1, 1, 0), "slipweak", False),
+                             ("y", (0, 0, 0), (0, 1, 0), "locked", False), ("y", (1, 0, 0), (1, 1, 0), "frictionless", False)]),
+    # 3-D, one fault normal to z, and one normal to x
+    "z_fault_3d": dict(ndim=3, nb=(1, 1, 2), npts=(31, 27, 21), ifaces=[("z", (0, 0, 0), (0, 0, 1), "slipweak", True)]),
+    "x_fault_3d": dict(ndim=3, nb=(2, 1, 1), npts=(21, 31, 27), ifaces=[("x", (0, 0, 0), (1, 0, 0), "slipweak", True)]),
+}
+
+
+@pytest.mark.parametrize("exact", [False, True])
+@pytest.mark.parametrize("name", sorted(LAYOUTS))
+def test_layout_against_oracle(tmp_path, name, exact):
+    from fdfault_amd import lib
+    kw = LAYOUTS[name]
+    nt = 20 if kw["ndim"] == 3 else 30
+    p = layout(tmp_path, name, nt=nt, **kw)
+    print("layout", name, "exact", exact, flush=True)
+    o = O.Oracle(p)
+    g = lib.Solver(p, exact=exact)
+    i = g.info
+    print("blocks", i.nblocks, "tiled", i.tiled_blocks, "rows", i.row_blocks, "fused", i.fused_faults, flush=True)
+    if not exact and kw.get("mode", 2) == 2:
+        assert i.tiled_blocks == i.nblocks  # the fast Cartesian kernels take every block
+    o.step(nt)
+    g.step(nt)
+    fo, fg = o.array("f"), g.array("f")
+    for q in range(fo.shape[0]):
+        assert _rel(fg[q], fo[q]) <= TOL, ("field", q, _rel(fg[q], fo[q]))
+    moved = False
+    for i in range(o.nifaces):
+        if o.iface_shape(i)[3] == 0:
+            continue  # locked
+        sn = np.abs(o.iface(i, "Sn")).max()
+        for w in ("U", "V", "S", "Sn"):
+            a, b = g.iface(i, w), o.iface(i, w)
+            r = np.abs(a - b).max() / max(np.abs(b).max(), sn if w == "S" else 0., 1e-300)
+            assert r <= TOL, (i, w, r)
+        moved = moved or np.abs(o.iface(i, "V")).max() > 1e-3
+        fo_, fg_ = o.front(i), g.front(i)
+        if fo_ is not None:
+            assert np.abs(fo_ - fg_).max() <= TOL * max(1.0, nt * o.dt)
+    assert moved  # a rupture runs on the nucleated fault
+    g.close()
+    o.close()
+
+
+@pytest.mark.parametrize("name", ["x_fault_2d", "grid_2x2", "z_fault_3d"])
+def test_layout_fp32(tmp_path, name):
+    """FP32 bulk / FP64 fault on the same layouts: slip within 1e-4 of the
+    FP64 oracle's, relative to its largest value."""
+    from fdfault_amd import lib
+    kw = LAYOUTS[name]
+    nt = 20 if kw["ndim"] == 3 else 30
+    p = layout(tmp_path, name, nt=nt, **kw)
+    o = O.Oracle(p)
+    g = lib.Solver(p, precision=lib.FP32)
+    o.step(nt)
+    g.step(nt)
+    for i in range(o.nifaces):
+        if o.iface_shape(i)[3] == 0:
+            continue
+        a, b = g.iface(i, "U"), o.iface(i, "U")
+        if np.abs(b).max() > 0:
+            assert _rel(a, b) <= 1e-4, (i, _rel(a, b))
+    g.close()
+    o.close()
