@@ -208,6 +208,8 @@ class SolverBase {
   virtual long read_iface(int iface, int which, double* h, long n) = 0;
   virtual long read_front(int iface, double* h, long n) = 0;
   virtual void finish() = 0;
+  virtual void save_state(const char* path) = 0;
+  virtual void load_state(const char* path) = 0;
   virtual double bulk_ms(int* n) = 0;
   virtual double fault_ms(int* n) = 0;
   SetupH S;
@@ -238,6 +240,8 @@ class Solver : public SolverBase {
   long read_iface(int iface, int which, double* h, long n) override;
   long read_front(int iface, double* h, long n) override;
   void finish() override;
+  void save_state(const char* path) override;
+  void load_state(const char* path) override;
   double bulk_ms(int* n) override;
   double fault_ms(int* n) override;
 
@@ -2087,6 +2091,113 @@ long Solver<R>::read_front(int iface, double* h, long n) {
   return (long)b.n;
 }
 
+// State snapshot for a later resume (the reference has no restart): a
+// header (tag, version, bulk word size, N, nf, nfp, interface count, step),
+// then f, df and the plastic fields in the bulk precision, then per
+// interface its slip/state arrays and front in FP64.  The interface SAT
+// terms are not state: every stage's fault solve writes them before the bulk
+// update reads them.
+static const char kStateTag[8] = {'F', 'D', 'F', 'S', 'T', 'A', 'T', 'E'};
+static const int kStateVersion = 1;
+static const int kStateIface[] = {0, 8, 1, 2, 3, 4, 9, 5, 6, 7, 10};  // u du v s sn ux dux vx sx state dstate
+
+template <typename R>
+void Solver<R>::save_state(const char* path) {
+  materialize_df();
+  HIPCHK(hipStreamSynchronize(stream));
+  FILE* fh = fopen(path, "wb");
+  if (!fh) throw Error{std::string("save_state: cannot open ") + path};
+  std::unique_ptr<FILE, int (*)(FILE*)> guard(fh, fclose);
+  auto put = [&](const void* p, size_t bytes) {
+    if (fwrite(p, 1, bytes, fh) != bytes) throw Error{std::string("save_state: cannot write ") + path};
+  };
+  auto put_dev = [&](const void* d, size_t bytes) {
+    std::vector<char> h(bytes);
+    if (bytes) HIPCHK(hipMemcpy(h.data(), d, bytes, hipMemcpyDeviceToHost));
+    put(h.data(), bytes);
+  };
+  const long long hdr[7] = {kStateVersion, (long long)sizeof(R), (long long)S.N, S.nf, S.nfp,
+                            (long long)S.ifaces.size(), cur_step};
+  put(kStateTag, sizeof(kStateTag));
+  put(hdr, sizeof(hdr));
+  put_dev(f[cur].p, (size_t)S.nf * S.N * sizeof(R));
+  put_dev(df.p, (size_t)S.nf * S.N * sizeof(R));
+  if (S.nfp) put_dev(fp.p, (size_t)S.nfp * S.N * sizeof(R));
+  for (size_t i = 0; i < ibufs.size(); i++) {
+    IfBufs& ib = ibufs[i];
+    for (int w : kStateIface) {
+      DBuf<double>* b = iface_buf((int)i, w);
+      const long long n = (long long)b->n;
+      put(&n, sizeof(n));
+      put_dev(b->p, b->n * sizeof(double));
+    }
+    const long long nf_ = (long long)ib.front.n;
+    put(&nf_, sizeof(nf_));
+    put_dev(ib.front.p, ib.front.n * sizeof(double));
+  }
+  guard.reset();
+}
+
+template <typename R>
+void Solver<R>::load_state(const char* path) {
+  FILE* fh = fopen(path, "rb");
+  if (!fh) throw Error{std::string("load_state: cannot open ") + path};
+  std::unique_ptr<FILE, int (*)(FILE*)> guard(fh, fclose);
+  auto get = [&](void* p, size_t bytes) {
+    if (fread(p, 1, bytes, fh) != bytes) throw Error{std::string("load_state: truncated snapshot ") + path};
+  };
+  char tag[8];
+  long long hdr[7];
+  get(tag, sizeof(tag));
+  get(hdr, sizeof(hdr));
+  if (memcmp(tag, kStateTag, sizeof(tag)) != 0 || hdr[0] != kStateVersion)
+    throw Error{std::string("load_state: not a state snapshot: ") + path};
+  if (hdr[1] != (long long)sizeof(R) || hdr[2] != (long long)S.N || hdr[3] != S.nf || hdr[4] != S.nfp ||
+      hdr[5] != (long long)S.ifaces.size())
+    throw Error{"load_state: the snapshot is of another problem or precision"};
+  // read everything into host memory first, so a bad file leaves the state as it was
+  std::vector<char> hf((size_t)S.nf * S.N * sizeof(R)), hdf(hf.size()), hfp((size_t)S.nfp * S.N * sizeof(R));
+  get(hf.data(), hf.size());
+  get(hdf.data(), hdf.size());
+  if (S.nfp) get(hfp.data(), hfp.size());
+  std::vector<std::vector<double>> hif;
+  for (size_t i = 0; i < ibufs.size(); i++) {
+    for (int w : kStateIface) {
+      long long n = 0;
+      get(&n, sizeof(n));
+      if (n != (long long)iface_buf((int)i, w)->n) throw Error{"load_state: the snapshot is of another problem"};
+      hif.emplace_back((size_t)n);
+      get(hif.back().data(), (size_t)n * sizeof(double));
+    }
+    long long n = 0;
+    get(&n, sizeof(n));
+    if (n != (long long)ibufs[i].front.n) throw Error{"load_state: the snapshot is of another problem"};
+    hif.emplace_back((size_t)n);
+    get(hif.back().data(), (size_t)n * sizeof(double));
+  }
+  char extra;
+  if (fread(&extra, 1, 1, fh) != 0) throw Error{std::string("load_state: trailing data in ") + path};
+  drain_outputs();
+  HIPCHK(hipStreamSynchronize(stream));
+  HIPCHK(hipMemcpy(f[cur].p, hf.data(), hf.size(), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(df.p, hdf.data(), hdf.size(), hipMemcpyHostToDevice));
+  if (S.nfp) HIPCHK(hipMemcpy(fp.p, hfp.data(), hfp.size(), hipMemcpyHostToDevice));
+  size_t k = 0;
+  for (size_t i = 0; i < ibufs.size(); i++) {
+    for (int w : kStateIface) {
+      DBuf<double>* b = iface_buf((int)i, w);
+      if (b->n) HIPCHK(hipMemcpy(b->p, hif[k].data(), b->n * sizeof(double), hipMemcpyHostToDevice));
+      k++;
+    }
+    if (ibufs[i].front.n)
+      HIPCHK(hipMemcpy(ibufs[i].front.p, hif[k].data(), ibufs[i].front.n * sizeof(double), hipMemcpyHostToDevice));
+    k++;
+  }
+  cur_step = (int)hdr[6];
+  dstep_valid = false;  // the device step counter follows at the next graph replay
+  df_stale = false;
+}
+
 // outputlist::close_list + frontlist::write_list (front.cpp:127-359)
 template <typename R>
 void Solver<R>::finish() {
@@ -2946,6 +3057,14 @@ int fdf_read_array(fdf_t* h, int which, double* host, long n) {
 
 int fdf_write_array(fdf_t* h, int which, const double* host, long n) {
   return guarded([&] { h->solver->write_array(which, host, n); });
+}
+
+int fdf_save_state(fdf_t* h, const char* path) {
+  return guarded([&] { h->solver->save_state(path); });
+}
+
+int fdf_load_state(fdf_t* h, const char* path) {
+  return guarded([&] { h->solver->load_state(path); });
 }
 
 long fdf_read_iface(fdf_t* h, int iface, int which, double* host, long n) {

@@ -43,7 +43,7 @@ class Info(C.Structure):
 
 
 EXPORTS = ["fdf_open", "fdf_step", "fdf_stage", "fdf_run", "fdf_read_field", "fdf_read_array",
-           "fdf_write_array", "fdf_read_iface", "fdf_read_front", "fdf_finish", "fdf_get_info",
+           "fdf_write_array", "fdf_save_state", "fdf_load_state", "fdf_read_iface", "fdf_read_front", "fdf_finish", "fdf_get_info",
            "fdf_host_setup", "fdf_last_error", "fdf_close", "fdf_bulk_kernel_ms", "fdf_fault_kernel_ms",
            "fdf_open_batch", "fdf_batch_step", "fdf_batch_size", "fdf_batch_get_info", "fdf_batch_read_iface",
            "fdf_batch_read_front", "fdf_batch_read_array", "fdf_batch_tuning", "fdf_batch_persistent", "fdf_batch_finish",
@@ -70,6 +70,8 @@ def load():
     L.fdf_read_field.argtypes = [C.c_void_p, C.c_int, ip, ip, ip, dp]
     L.fdf_read_array.argtypes = [C.c_void_p, C.c_int, dp, C.c_long]
     L.fdf_write_array.argtypes = [C.c_void_p, C.c_int, dp, C.c_long]
+    L.fdf_save_state.argtypes = [C.c_void_p, C.c_char_p]
+    L.fdf_load_state.argtypes = [C.c_void_p, C.c_char_p]
     L.fdf_read_iface.restype = C.c_long
     L.fdf_read_iface.argtypes = [C.c_void_p, C.c_int, C.c_int, dp, C.c_long]
     L.fdf_read_front.restype = C.c_long
@@ -194,6 +196,18 @@ class Solver:
     def set_array(self, which: str, arr):
         a = np.ascontiguousarray(arr, dtype=np.float64).ravel()
         if load().fdf_write_array(self.h, ARRAYS[which], _dp(a), a.size) != 0:
+            raise _err()
+
+    def save_state(self, path: str):
+        """Snapshot of the evolving state after the last completed step
+        (fdf_save_state)."""
+        if load().fdf_save_state(self.h, path.encode()) != 0:
+            raise _err()
+
+    def load_state(self, path: str):
+        """Resume from a snapshot of the same problem and precision
+        (fdf_load_state)."""
+        if load().fdf_load_state(self.h, path.encode()) != 0:
             raise _err()
 
     def field(self, q: int, lo, hi, st=(1, 1, 1)):

@@ -96,6 +96,18 @@ long fdf_read_iface(fdf_t* h, int iface, int which, double* host, long n);
  * src/front.cpp:103-125); returns the element count or a negative code. */
 long fdf_read_front(fdf_t* h, int iface, double* host, long n);
 
+/* State snapshot and resume: no counterpart in the reference, which has no
+ * restart (its output files are truncated at open, src/outputunit.cpp:580).
+ * fdf_save_state writes the evolving state after the last completed step --
+ * f, df and the plastic fields in the bulk precision, every interface's
+ * slip, slip rate, tractions, state, their RK increments and the rupture
+ * front, and the step count -- to a binary file; fdf_load_state reads one
+ * into a handle opened on the same problem with the same bulk precision, so
+ * that stepping on gives bit-identical results to the run that wrote it.
+ * Output units and the STZ proposal network are not part of the state. */
+int fdf_save_state(fdf_t* h, const char* path);
+int fdf_load_state(fdf_t* h, const char* path);
+
 /* Flushes and closes output files and writes front files
  * (outputlist::close_list, frontlist::write_list; src/problem.cpp:182-187). */
 int fdf_finish(fdf_t* h);
