@@ -7,7 +7,8 @@
   tile width / x-chunk.
 * A batch of instances of different sizes gives, per instance, bit-identical
   results to single runs (same kernels, same arithmetic per point), in FP32
-  and FP64, eager and graph-replayed; the autotuner's cache is reused.
+  and FP64, eager and graph-replayed, also with instances of different
+  friction laws in one batch; the autotuner's cache is reused.
 """
 import os
 
@@ -185,3 +186,35 @@ def test_batch_rejects_unsupported_instance(tmp_path):
     diss = _case(tmp_path, "d", 61, 31, nt=5, cdiss=0.1)
     with pytest.raises(lib.FdfaultError, match="not rectilinear"):
         lib.Batch([good, diss], precision=lib.FP32)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_batch_mixed_friction_laws(tmp_path, monkeypatch, precision):
+    """Instances with different friction laws (slip weakening, STZ,
+    frictionless) in one batch take the fault launch that dispatches on each
+    interface's law; every instance is bit-identical to its single run,
+    STZ state included."""
+    from cases import STZ_PARAMS
+    from fdfault_amd import lib
+    prec = lib.FP32 if precision == "fp32" else lib.FP64
+    monkeypatch.setenv("FDF_BATCH_TW", "64")
+    monkeypatch.setenv("FDF_BATCH_CX", "2")
+    monkeypatch.setenv("FDF_BATCH_PERSIST", "0")
+    cases = [("sw", 61, 31, {}), ("stz", 61, 31, dict(law="stz", stz=dict(STZ_PARAMS, chi0=0.07))),
+             ("fl", 45, 27, dict(law="frictionless")), ("sw2", 97, 45, {})]
+    paths = [_case(tmp_path, n, nx, ny, nt=25, **kw) for n, nx, ny, kw in cases]
+    b = lib.Batch(paths, precision=prec, graph=True)
+    b.step(25)
+    monkeypatch.setenv("FDF_TILE2_TW", "64")
+    monkeypatch.setenv("FDF_TILE2_CX", "2")
+    monkeypatch.setenv("FDF_FUSE2D", "0")  # single runs with the separate fault launch, as the batch
+    for i, p in enumerate(paths):
+        s = lib.Solver(p, precision=prec)
+        s.step(25)
+        assert np.array_equal(b.array(i, "f"), s.array("f")), i
+        ws = ("U", "V", "S", "Sn") + (("state",) if cases[i][0] == "stz" else ())
+        for w in ws:
+            assert np.array_equal(b.iface(i, 0, w), s.iface(0, w)), (i, w)
+        assert np.array_equal(b.front(i, 0), s.front(0)), i
+        s.close()
+    b.close()
