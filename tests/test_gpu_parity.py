@@ -143,6 +143,43 @@ def test_mode2_het_stress_material(tmp_path):
 
 
 @pytest.mark.parametrize("exact", [True, False])
+def test_3d_het_stress_material(tmp_path, exact):
+    """3-D heterogeneous prestress (six components) and material (rho, lambda,
+    G per point, fields.cpp:452-711): the generic kernels' per-point moduli
+    and the fault solve's per-point impedances in 3-D."""
+    nx, ny, nz = 25, 14, 17
+    rng = np.random.default_rng(3)
+    shape = (nx, 2 * ny, nz)
+    s = rng.normal(0., 2., size=(6,) + shape)
+    sp = str(tmp_path / "hs3.bin")
+    s.tofile(sp)
+    mat = np.empty((3,) + shape)
+    mat[0] = 2.67 * (1 + 0.05 * rng.random(shape))
+    mat[1] = 32.04 * (1 + 0.05 * rng.random(shape))
+    mat[2] = 32.04 * (1 + 0.05 * rng.random(shape))
+    mp = str(tmp_path / "mat3.bin")
+    mat.tofile(mp)
+    loads = [Pert("constant", 0., 0., 0., 0., 0., (-120., 70., 0.)),
+             Pert("boxcar", 0., 1.2, 0.6, 0.8, 0.6, (0., 11.6, 0.))]
+    p = write_case(str(tmp_path), "het3", ndim=3, nx=nx, ny=ny, nz=nz, dx=0.1, nt=8, loads=loads, sbporder=3,
+                   hetstressfile=sp, matfile=mp)
+    _check(p, 8, exact=exact)
+
+
+def test_mode3_het_stress(tmp_path):
+    """Mode III with heterogeneous prestress (sxz, syz per point)."""
+    nx, ny = 61, 31
+    rng = np.random.default_rng(5)
+    s = rng.normal(0., 2., size=(2, nx, 2 * ny, 1))
+    sp = str(tmp_path / "hs_m3.bin")
+    s.tofile(sp)
+    loads = [Pert("constant", 0., 0., 0., 0., 0., (-120., 0., 70.)),
+             Pert("boxcar", 0., 3.0, 0.6, 0., 0., (0., 0., 11.6))]
+    p = write_case(str(tmp_path), "hm3", mode=3, nx=nx, ny=ny, dx=0.1, nt=15, loads=loads, hetstressfile=sp)
+    _check(p, 15)
+
+
+@pytest.mark.parametrize("exact", [True, False])
 @pytest.mark.parametrize("sbp", [2, 3, 4])
 def test_3d_slipweak(tmp_path, sbp, exact):
     """exact=False exercises the rectilinear interior kernel (x-marching
