@@ -166,6 +166,31 @@ def test_3d_het_stress_material(tmp_path, exact):
     _check(p, 8, exact=exact)
 
 
+@pytest.mark.parametrize("rk", [1, 2, 3])
+def test_3d_rk_orders(tmp_path, rk):
+    """RK orders 1-3 on the 3-D row kernel (1 to 3 stages: which stages read
+    and write df differs from RK4's five)."""
+    loads = [Pert("constant", 0., 0., 0., 0., 0., (-120., 70., 0.)),
+             Pert("boxcar", 0., 1.5, 0.6, 0.8, 0.6, (0., 11.6, 0.))]
+    p = write_case(str(tmp_path), "rk3d%d" % rk, ndim=3, nx=31, ny=14, nz=17, dx=0.1, nt=8, loads=loads,
+                   sbporder=3, rkorder=rk, cfl=0.2)
+    from fdfault_amd import lib
+    g = lib.Solver(p)
+    assert g.info.row_blocks == 2
+    g.close()
+    _check(p, 8)
+
+
+def test_3d_dissipation_rigid(tmp_path):
+    """3-D artificial dissipation (the cdiss terms of block.cpp:1588-1598) with rigid
+    outer faces."""
+    loads = [Pert("constant", 0., 0., 0., 0., 0., (-120., 70., 0.)),
+             Pert("boxcar", 0., 1.2, 0.6, 0.8, 0.6, (0., 11.6, 0.))]
+    p = write_case(str(tmp_path), "diss3", ndim=3, nx=25, ny=14, nz=17, dx=0.1, nt=8, loads=loads, sbporder=3,
+                   cdiss=0.1, bc="rigid")
+    _check(p, 8)
+
+
 def test_mode3_het_stress(tmp_path):
     """Mode III with heterogeneous prestress (sxz, syz per point)."""
     nx, ny = 61, 31
