@@ -57,6 +57,32 @@ def test_executable_files_match_oracle(tmp_path, exact):
             assert np.abs(x - y).max() <= 1e-10 * scale, fn
 
 
+def test_executable_graph_replay_and_eager_write_the_same_files(tmp_path):
+    """FDFAULT_GRAPH=0 issues every launch of a step one by one instead of
+    replaying the step's hipGraph: the output and front files are
+    byte-identical (2-D with the fault solve in the stage launch, and 3-D on
+    the row kernel)."""
+    exe = os.path.join(ROOT, "fdfault_amd", "_lib", "fdfault")
+    outs = [Output("V", "V", (0, 20, 1), (0, 60, 1), (31, 31, 1), (0, 0, 1)),
+            Output("vx", "vx", (0, 20, 5), (0, 60, 2), (0, 61, 3), (0, 0, 1))]
+    outs3 = [Output("V", "V", (0, 10, 1), (0, 30, 1), (21, 21, 1), (0, 24, 1))]
+    for name, kw in (("g2", dict(nx=61, ny=31, nt=20, outputs=outs)),
+                     ("g3", dict(ndim=3, nx=31, ny=21, nz=25, nt=10, sbporder=3, outputs=outs3))):
+        dirs = []
+        for graph in ("1", "0"):
+            p = write_case(str(tmp_path / graph), name, dx=0.1, front=("V", 0.001), **kw)
+            r = subprocess.run([exe, p], capture_output=True, text=True, timeout=120,
+                               env=dict(os.environ, FDFAULT_GRAPH=graph))
+            assert r.returncode == 0, r.stderr[-400:]
+            dirs.append(os.path.join(os.path.dirname(p), "data"))
+        fa, fb = sorted(os.listdir(dirs[0])), sorted(os.listdir(dirs[1]))
+        assert fa == fb and len(fa) >= 8
+        for fn in fa:
+            a, b = os.path.join(dirs[0], fn), os.path.join(dirs[1], fn)
+            if fn.endswith(".dat"):
+                assert filecmp.cmp(a, b, shallow=False), (name, fn)
+
+
 @pytest.mark.parametrize("slots", ["1", "2"])
 def test_output_files_with_small_staging_rings(tmp_path, monkeypatch, slots):
     """Output units written through staging rings of one or two slots (the
