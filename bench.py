@@ -197,12 +197,30 @@ def single_workload(args):
 
 
 def compare_precision(args):
-    """FP32 bulk / FP64 fault against FP64 on the 3-D planar workload: per-field
-    relative error of f, final slip and slip-rate error, front error in steps."""
+    """FP32 bulk / FP64 fault against FP64: per-field relative error of f,
+    final slip and slip-rate error, front error in steps.  The 3-D planar
+    workload, or (--workload planar2d / rough2d) the 2-D planar and the 2-D
+    rough-fault plastic ones over their full run lengths by default."""
     from fdfault_amd import lib
     res = {}
+    w2 = args.workload in ("planar2d", "rough2d")
     with tempfile.TemporaryDirectory() as d:
-        path = workload_3d(d, args.nx, args.ny, args.nz, sbporder=args.sbporder)
+        if args.workload == "planar2d":
+            path, wname = workload_2d(d), "planar2d_2x801x401"
+            names = ["vx", "vy", "sxx", "sxy", "syy"]
+        elif args.workload == "rough2d":
+            path, wname = workload_rough2d(d), "rough2d_plastic_2x1601x801"
+            names = ["vx", "vy", "sxx", "sxy", "syy", "szz", "lambda", "gammap"]
+        else:
+            path = workload_3d(d, args.nx, args.ny, args.nz, sbporder=args.sbporder)
+            wname = "3d_planar_fault_slipweak_2x%dx%dx%d" % (args.nx, args.ny, args.nz)
+            names = ["vx", "vy", "vz", "sxx", "sxy", "sxz", "syy", "syz", "szz"]
+        if args.compare_steps <= 0:
+            args.compare_steps = 1500
+            if w2:
+                s0 = lib.Solver(path)
+                args.compare_steps = s0.nt
+                s0.close()
         for prec in (lib.FP64, lib.FP32):
             g = lib.Solver(path, precision=prec, graph=True)
             done = 0
@@ -215,13 +233,11 @@ def compare_precision(args):
                          "dt": g.dt}
             g.close()
     a, b = res[lib.FP32], res[lib.FP64]
-    names = ["vx", "vy", "vz", "sxx", "sxy", "sxz", "syy", "syz", "szz"]
     per = {n: float(np.abs(a["f"][q] - b["f"][q]).max() / max(np.abs(b["f"][q]).max(), 1e-300))
-           for q, n in enumerate(names)}
+           for q, n in enumerate(names[:a["f"].shape[0]])}
     both = (a["front"] >= 0) & (b["front"] >= 0)
-    line = {"metric": "FP32 bulk / FP64 fault vs FP64, 3-D planar fault",
-            "config": {"workload": "3d_planar_fault_slipweak_2x%dx%dx%d" % (args.nx, args.ny, args.nz),
-                       "steps": args.compare_steps},
+    line = {"metric": "FP32 bulk / FP64 fault vs FP64, " + wname,
+            "config": {"workload": wname, "steps": args.compare_steps},
             "per_field_rel_err": per,
             "final_slip_rel_err": float(np.abs(a["U"] - b["U"]).max() / max(np.abs(b["U"]).max(), 1e-300)),
             "slip_rate_rel_err": float(np.abs(a["V"] - b["V"]).max() / max(np.abs(b["V"]).max(), 1e-300)),
@@ -535,8 +551,9 @@ def main():
                                            "tpv4"],
                     default="planar3d")
     ap.add_argument("--compare-precision", action="store_true",
-                    help="planar3d: FP32 vs FP64 errors after --compare-steps steps")
-    ap.add_argument("--compare-steps", type=int, default=1500)
+                    help="planar3d, planar2d, rough2d: FP32 vs FP64 errors after --compare-steps steps")
+    ap.add_argument("--compare-steps", type=int, default=0,
+                    help="0: the workload's run length (1500 steps for planar3d, nt of the 2-D workloads)")
     ap.add_argument("--no-stream", action="store_true",
                     help="planar3d: skip the copy / same-shape stream bandwidth references")
     ap.add_argument("--only", action="store_true",
@@ -556,6 +573,9 @@ def main():
         print("bench.py: fdfault_amd runs on one GPU only (the whole grid fits one MI355X); "
               "a multi-GPU slab decomposition is designed but not built", file=sys.stderr)
         sys.exit(2)
+    if args.compare_precision and args.workload in ("planar3d", "planar2d", "rough2d"):
+        compare_precision(args)
+        return
     if args.workload in ("planar2d", "rough2d"):
         single_workload(args)
         return
@@ -567,9 +587,6 @@ def main():
         return
     if args.workload == "setup":
         setup_times(args)
-        return
-    if args.workload == "planar3d" and args.compare_precision:
-        compare_precision(args)
         return
     if args.workload == "stz3d":
         if "--precision" not in sys.argv:
