@@ -3,6 +3,7 @@
 // the device so each stage reads f and writes the other buffer.
 #include <hip/hip_runtime.h>
 
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -2135,7 +2136,8 @@ void Solver<R>::save_state(const char* path) {
     put(&nf_, sizeof(nf_));
     put_dev(ib.front.p, ib.front.n * sizeof(double));
   }
-  guard.reset();
+  // a write that fails only when the buffered data is flushed shows at fclose
+  if (fclose(guard.release()) != 0) throw Error{std::string("save_state: cannot write ") + path};
 }
 
 template <typename R>
@@ -2155,6 +2157,7 @@ void Solver<R>::load_state(const char* path) {
   if (hdr[1] != (long long)sizeof(R) || hdr[2] != (long long)S.N || hdr[3] != S.nf || hdr[4] != S.nfp ||
       hdr[5] != (long long)S.ifaces.size())
     throw Error{"load_state: the snapshot is of another problem or precision"};
+  if (hdr[6] < 0 || hdr[6] > INT_MAX) throw Error{std::string("load_state: not a state snapshot: ") + path};
   // read everything into host memory first, so a bad file leaves the state as it was
   std::vector<char> hf((size_t)S.nf * S.N * sizeof(R)), hdf(hf.size()), hfp((size_t)S.nfp * S.N * sizeof(R));
   get(hf.data(), hf.size());

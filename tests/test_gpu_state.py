@@ -87,5 +87,11 @@ def test_snapshot_refusals(tmp_path):
     np.zeros(16).tofile(other)
     with pytest.raises(lib.FdfaultError, match="not a state snapshot"):
         a.load_state(other)
+    bad = str(tmp_path / "bad_step.bin")
+    raw = bytearray(open(snap, "rb").read())
+    raw[56:64] = np.int64(-1).tobytes()  # the step count, after the tag and six header words
+    open(bad, "wb").write(bytes(raw))
+    with pytest.raises(lib.FdfaultError, match="not a state snapshot"):
+        a.load_state(bad)
     assert np.array_equal(a.array("f"), before)  # refused loads left the state alone
     a.close()
