@@ -3002,6 +3002,22 @@ static int guarded(Fn&& fn) {
   return -1;
 }
 
+// Entry points turn a null handle or a null buffer into an error return
+// rather than a crash.
+static fdf::SolverBase* solver_of(fdf_t* h, const char* fn) {
+  if (!h || !h->solver) throw Error{std::string(fn) + ": null handle"};
+  return h->solver.get();
+}
+
+static fdf::BatchBase* batch_of(fdf_batch_t* h, const char* fn) {
+  if (!h || !h->b) throw Error{std::string(fn) + ": null handle"};
+  return h->b.get();
+}
+
+static void need(const void* p, const char* fn) {
+  if (!p) throw Error{std::string(fn) + ": null argument"};
+}
+
 extern "C" {
 
 const char* fdf_last_error(void) { return fdf::g_last_error.c_str(); }
@@ -3027,13 +3043,14 @@ int fdf_open(const char* in_path, const fdf_opts* opts, fdf_t** out) {
 void fdf_close(fdf_t* h) { delete h; }
 
 int fdf_step(fdf_t* h, int nsteps) {
-  return guarded([&] { h->solver->step(nsteps); });
+  return guarded([&] { solver_of(h, "fdf_step")->step(nsteps); });
 }
 
 int fdf_stage(fdf_t* h, int istep, int stage) {
   return guarded([&] {
-    if (stage < 0 || stage >= h->solver->S.rk.nstages) throw Error{"fdf_stage: bad stage"};
-    h->solver->stage(istep, stage);
+    fdf::SolverBase* sv = solver_of(h, "fdf_stage");
+    if (stage < 0 || stage >= sv->S.rk.nstages) throw Error{"fdf_stage: bad stage"};
+    sv->stage(istep, stage);
   });
 }
 
@@ -3051,39 +3068,55 @@ int fdf_run(const char* in_path, const fdf_opts* opts) {
 }
 
 int fdf_read_field(fdf_t* h, int field, const int lo[3], const int hi[3], const int st[3], double* host) {
-  return guarded([&] { h->solver->read_field(field, lo, hi, st, host); });
+  return guarded([&] {
+    fdf::SolverBase* sv = solver_of(h, "fdf_read_field");
+    for (const void* p : {(const void*)lo, (const void*)hi, (const void*)st, (const void*)host}) need(p, "fdf_read_field");
+    sv->read_field(field, lo, hi, st, host);
+  });
 }
 
 int fdf_read_array(fdf_t* h, int which, double* host, long n) {
-  return guarded([&] { h->solver->read_array(which, host, n); });
+  return guarded([&] {
+    need(host, "fdf_read_array");
+    solver_of(h, "fdf_read_array")->read_array(which, host, n);
+  });
 }
 
 int fdf_write_array(fdf_t* h, int which, const double* host, long n) {
-  return guarded([&] { h->solver->write_array(which, host, n); });
+  return guarded([&] {
+    need(host, "fdf_write_array");
+    solver_of(h, "fdf_write_array")->write_array(which, host, n);
+  });
 }
 
 int fdf_save_state(fdf_t* h, const char* path) {
-  return guarded([&] { h->solver->save_state(path); });
+  return guarded([&] {
+    need(path, "fdf_save_state");
+    solver_of(h, "fdf_save_state")->save_state(path);
+  });
 }
 
 int fdf_load_state(fdf_t* h, const char* path) {
-  return guarded([&] { h->solver->load_state(path); });
+  return guarded([&] {
+    need(path, "fdf_load_state");
+    solver_of(h, "fdf_load_state")->load_state(path);
+  });
 }
 
 long fdf_read_iface(fdf_t* h, int iface, int which, double* host, long n) {
   long r = -1;
-  int rc = guarded([&] { r = h->solver->read_iface(iface, which, host, n); });
+  int rc = guarded([&] { r = solver_of(h, "fdf_read_iface")->read_iface(iface, which, host, n); });
   return rc ? -1 : r;
 }
 
 long fdf_read_front(fdf_t* h, int iface, double* host, long n) {
   long r = -1;
-  int rc = guarded([&] { r = h->solver->read_front(iface, host, n); });
+  int rc = guarded([&] { r = solver_of(h, "fdf_read_front")->read_front(iface, host, n); });
   return rc ? -1 : r;
 }
 
 int fdf_finish(fdf_t* h) {
-  return guarded([&] { h->solver->finish(); });
+  return guarded([&] { solver_of(h, "fdf_finish")->finish(); });
 }
 
 static void fill_info(const fdf::SolverBase* sv, fdf_info* info) {
@@ -3146,7 +3179,7 @@ void fdf_batch_close(fdf_batch_t* h) { delete h; }
 int fdf_batch_size(fdf_batch_t* h) { return h ? h->b->size() : -2; }
 
 int fdf_batch_step(fdf_batch_t* h, int nsteps) {
-  return guarded([&] { h->b->step(nsteps); });
+  return guarded([&] { batch_of(h, "fdf_batch_step")->step(nsteps); });
 }
 
 int fdf_batch_get_info(fdf_batch_t* h, int inst, fdf_info* info) {
@@ -3159,8 +3192,9 @@ int fdf_batch_get_info(fdf_batch_t* h, int inst, fdf_info* info) {
 long fdf_batch_read_iface(fdf_batch_t* h, int inst, int iface, int which, double* host, long n) {
   long r = -1;
   int rc = guarded([&] {
-    if (inst < 0 || inst >= h->b->size()) throw Error{"fdf_batch_read_iface: bad instance"};
-    r = h->b->instance(inst)->read_iface(iface, which, host, n);
+    fdf::BatchBase* b = batch_of(h, "fdf_batch_read_iface");
+    if (inst < 0 || inst >= b->size()) throw Error{"fdf_batch_read_iface: bad instance"};
+    r = b->instance(inst)->read_iface(iface, which, host, n);
   });
   return rc ? -1 : r;
 }
@@ -3168,16 +3202,19 @@ long fdf_batch_read_iface(fdf_batch_t* h, int inst, int iface, int which, double
 long fdf_batch_read_front(fdf_batch_t* h, int inst, int iface, double* host, long n) {
   long r = -1;
   int rc = guarded([&] {
-    if (inst < 0 || inst >= h->b->size()) throw Error{"fdf_batch_read_front: bad instance"};
-    r = h->b->instance(inst)->read_front(iface, host, n);
+    fdf::BatchBase* b = batch_of(h, "fdf_batch_read_front");
+    if (inst < 0 || inst >= b->size()) throw Error{"fdf_batch_read_front: bad instance"};
+    r = b->instance(inst)->read_front(iface, host, n);
   });
   return rc ? -1 : r;
 }
 
 int fdf_batch_read_array(fdf_batch_t* h, int inst, int which, double* host, long n) {
   return guarded([&] {
-    if (inst < 0 || inst >= h->b->size()) throw Error{"fdf_batch_read_array: bad instance"};
-    h->b->instance(inst)->read_array(which, host, n);
+    fdf::BatchBase* b = batch_of(h, "fdf_batch_read_array");
+    need(host, "fdf_batch_read_array");
+    if (inst < 0 || inst >= b->size()) throw Error{"fdf_batch_read_array: bad instance"};
+    b->instance(inst)->read_array(which, host, n);
   });
 }
 
@@ -3193,7 +3230,7 @@ int fdf_batch_tuning(fdf_batch_t* h, int* tile_width, int* chunk, double* stage_
 int fdf_batch_persistent(fdf_batch_t* h) { return h ? h->b->persist : -2; }
 
 int fdf_batch_finish(fdf_batch_t* h) {
-  return guarded([&] { h->b->finish(); });
+  return guarded([&] { batch_of(h, "fdf_batch_finish")->finish(); });
 }
 
 int fdf_mlp_grad(const float* params, const float* X, const float* T, long n, float* grad, float* loss) {
@@ -3287,23 +3324,23 @@ int fdf_mlp_param_count(void) { return fdf::MLP_NP; }
 
 int fdf_stz_set_model(fdf_t* h, const float* params, int n) {
   return guarded([&] {
-    if (auto* s = dynamic_cast<fdf::Solver<double>*>(h->solver.get())) s->stz_set_model(params, n);
-    else if (auto* s32 = dynamic_cast<fdf::Solver<float>*>(h->solver.get())) s32->stz_set_model(params, n);
+    if (auto* s = dynamic_cast<fdf::Solver<double>*>(solver_of(h, "fdf_stz"))) s->stz_set_model(params, n);
+    else if (auto* s32 = dynamic_cast<fdf::Solver<float>*>(solver_of(h, "fdf_stz"))) s32->stz_set_model(params, n);
   });
 }
 
 int fdf_stz_log(fdf_t* h, long capacity) {
   return guarded([&] {
-    if (auto* s = dynamic_cast<fdf::Solver<double>*>(h->solver.get())) s->stz_enable_log(capacity);
-    else if (auto* s32 = dynamic_cast<fdf::Solver<float>*>(h->solver.get())) s32->stz_enable_log(capacity);
+    if (auto* s = dynamic_cast<fdf::Solver<double>*>(solver_of(h, "fdf_stz"))) s->stz_enable_log(capacity);
+    else if (auto* s32 = dynamic_cast<fdf::Solver<float>*>(solver_of(h, "fdf_stz"))) s32->stz_enable_log(capacity);
   });
 }
 
 long fdf_stz_log_read(fdf_t* h, double* out, long cap) {
   long r = -1;
   int rc = guarded([&] {
-    if (auto* s = dynamic_cast<fdf::Solver<double>*>(h->solver.get())) r = s->stz_read_log(out, cap);
-    else if (auto* s32 = dynamic_cast<fdf::Solver<float>*>(h->solver.get())) r = s32->stz_read_log(out, cap);
+    if (auto* s = dynamic_cast<fdf::Solver<double>*>(solver_of(h, "fdf_stz"))) r = s->stz_read_log(out, cap);
+    else if (auto* s32 = dynamic_cast<fdf::Solver<float>*>(solver_of(h, "fdf_stz"))) r = s32->stz_read_log(out, cap);
   });
   return rc ? -1 : r;
 }
@@ -3311,14 +3348,14 @@ long fdf_stz_log_read(fdf_t* h, double* out, long cap) {
 int fdf_stz_stats(fdf_t* h, unsigned long long* out4) {
   return guarded([&] {
     if (!out4) throw Error{"fdf_stz_stats: null argument"};
-    if (auto* s = dynamic_cast<fdf::Solver<double>*>(h->solver.get())) s->stz_stats(out4);
-    else if (auto* s32 = dynamic_cast<fdf::Solver<float>*>(h->solver.get())) s32->stz_stats(out4);
+    if (auto* s = dynamic_cast<fdf::Solver<double>*>(solver_of(h, "fdf_stz"))) s->stz_stats(out4);
+    else if (auto* s32 = dynamic_cast<fdf::Solver<float>*>(solver_of(h, "fdf_stz"))) s32->stz_stats(out4);
   });
 }
 
 
-double fdf_bulk_kernel_ms(fdf_t* h, int* nlaunch) { return h->solver->bulk_ms(nlaunch); }
-double fdf_fault_kernel_ms(fdf_t* h, int* nstages) { return h->solver->fault_ms(nstages); }
+double fdf_bulk_kernel_ms(fdf_t* h, int* nlaunch) { return h && h->solver ? h->solver->bulk_ms(nlaunch) : -1.; }
+double fdf_fault_kernel_ms(fdf_t* h, int* nstages) { return h && h->solver ? h->solver->fault_ms(nstages) : -1.; }
 
 int fdf_host_setup(const char* in_path, fdf_info* info, double* x, double* metric, double* jac) {
   return guarded([&] {

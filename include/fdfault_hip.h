@@ -8,8 +8,9 @@
  * Each entry point below states which reference interface it replaces.  All
  * entry points return 0 on success and a negative code on error; the message
  * the reference would have printed before MPI_Abort is available from
- * fdf_last_error().  No C++ exception crosses this boundary.  Handles are
- * independent; one host thread per handle.
+ * fdf_last_error().  A null handle, or a null pointer where a buffer, box or
+ * path is required, is such an error, not a crash.  No C++ exception crosses
+ * this boundary.  Handles are independent; one host thread per handle.
  */
 #ifndef FDFAULT_HIP_H
 #define FDFAULT_HIP_H
