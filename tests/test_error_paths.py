@@ -87,3 +87,46 @@ def test_folded_grid_message(tmp_path):
     p = folded_case(tmp_path)
     assert _host_setup_error(p) == JAC
     assert _oracle_error(p) == JAC
+
+
+# Malformed input files: one edit of a valid two-block problem each, and the
+# message the set-up stops with.  Missing sections print the reference's
+# messages (problem.cpp:33, domain.cpp:43, fields.cpp:32, block.cpp:61,
+# interface.cpp:38, friction.cpp:93, slipweak.cpp:59, frontlist.cpp:36), and
+# so do bad FD and RK orders (fd.cpp:150, rk.cpp:31,93).  Where the reference
+# stops on an assert (boundary type boundary.cpp:22, interface type
+# domain.cpp:364, block size block.cpp:118) the message is this library's.
+INPUT_ERRORS = {
+    "no_problem": (("[fdfault.problem]", "[fdfault.problemX]"), "Error reading problem from input file"),
+    "no_domain": (("[fdfault.domain]", "[fdfault.domainX]"), "Error reading domain from input file"),
+    "no_fields": (("[fdfault.fields]", "[fdfault.fieldsX]"), "Error reading fields from input file"),
+    "no_block": (("[fdfault.block010]", "[fdfault.block010X]"), "Error reading block 010 from input file"),
+    "no_interface": (("[fdfault.interface0]", "[fdfault.interface0X]"), "Error reading interface 0 from input file"),
+    "no_friction": (("[fdfault.friction]", "[fdfault.frictionX]"), "Error reading interface 0 from input file"),
+    "no_slipweak": (("[fdfault.slipweak]", "[fdfault.slipweakX]"), "Error reading slipweak from input file"),
+    "no_frontlist": (("[fdfault.frontlist]", "[fdfault.frontlistX]"), "Error reading frontlist from input file"),
+    "boundary": (("absorbing\nabsorbing\nabsorbing\nnone", "sticky\nabsorbing\nabsorbing\nnone"),
+                 "unknown boundary type sticky"),
+    "iface_type": (("slipweak\n4\nelastic", "glue\n4\nelastic"), "unknown interface type glue"),
+    "sbporder": (("slipweak\n4\nelastic", "slipweak\n5\nelastic"), "Error in initializing FD method"),
+    "rkorder": (("1000\n4\n", "1000\n7\n"), "Error in initializing RK method"),
+    "block_size": (("41 42 1\n1 2 1\n41\n21 21", "41 8 1\n1 2 1\n41\n4 4"),
+                   "block too small for the finite difference method"),
+    "material": (("slipweak\n4\nelastic", "slipweak\n4\nrubber"), "material must be elastic or plastic"),
+}
+
+
+def edited_case(tmp, name, edit):
+    p = write_case(str(tmp), name, nx=41, ny=21, dx=0.1, nt=1)
+    text = open(p).read()
+    assert edit[0] in text
+    open(p, "w").write(text.replace(edit[0], edit[1], 1))
+    return p
+
+
+@pytest.mark.parametrize("case", sorted(INPUT_ERRORS))
+def test_malformed_input_message(tmp_path, case):
+    edit, msg = INPUT_ERRORS[case]
+    p = edited_case(tmp_path, case, edit)
+    assert _host_setup_error(p) == msg
+    assert _oracle_error(p) == msg
