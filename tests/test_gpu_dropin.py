@@ -57,6 +57,34 @@ def test_executable_files_match_oracle(tmp_path, exact):
             assert np.abs(x - y).max() <= 1e-10 * scale, fn
 
 
+@pytest.mark.parametrize("nt", [0, 1, 3])
+def test_short_runs_match_oracle(tmp_path, nt):
+    """Runs of 0, 1 and 3 steps: units whose last time index lies past nt are
+    clipped to nt and then to a whole number of strides (outputunit.cpp:301-308),
+    a unit with one time index writes no _t.dat file (outputunit.cpp:683), and
+    a run of no steps writes the t = 0 outputs and the front file only."""
+    outs = [Output("vx", "vx", (0, 20, 2), (0, 60, 4), (0, 61, 6), (0, 0, 1)),
+            Output("V", "V", (0, 0, 1), (0, 60, 1), (31, 31, 1), (0, 0, 1)),
+            Output("U", "U", (1, 9, 3), (0, 60, 2), (30, 30, 1), (0, 0, 1))]
+    p = write_case(str(tmp_path), "short", nx=61, ny=31, dx=0.1, nt=nt, outputs=outs, front=("V", 0.001))
+    d = os.path.dirname(p)
+    gdir, odir = os.path.join(d, "data"), os.path.join(d, "odata")
+    os.makedirs(odir)
+    po = os.path.join(d, "short_oracle.in")
+    open(po, "w").write(open(p).read().replace(gdir + "/", odir + "/"))
+    assert O.run(po) == 0
+    exe = os.path.join(ROOT, "fdfault_amd", "_lib", "fdfault")
+    print("nt", nt, flush=True)
+    r = subprocess.run([exe, p], capture_output=True, text=True, timeout=120, env=dict(os.environ, FDFAULT_EXACT="1"))
+    assert r.returncode == 0, r.stderr
+    gfiles, ofiles = sorted(os.listdir(gdir)), sorted(os.listdir(odir))
+    assert gfiles == ofiles, (gfiles, ofiles)
+    # vx: time indices 0, 2, ... up to nt, so more than one only from nt = 2
+    assert "short_V_t.dat" not in gfiles and ("short_vx_t.dat" in gfiles) == (nt >= 2)
+    for fn in gfiles:
+        assert filecmp.cmp(os.path.join(gdir, fn), os.path.join(odir, fn), shallow=False), fn
+
+
 def test_executable_graph_replay_and_eager_write_the_same_files(tmp_path):
     """FDFAULT_GRAPH=0 issues every launch of a step one by one instead of
     replaying the step's hipGraph: the output and front files are
