@@ -434,26 +434,34 @@ def test_paired_block_streams_do_not_change_results(tmp_path, monkeypatch, preci
             assert np.array_equal(fa, out[k][3]) and np.array_equal(dfa, out[k][4]), (pair, graph, kk)
 
 
-@pytest.mark.parametrize("plastic", [False, True])
-def test_graph_replay_matches_eager(tmp_path, plastic):
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("case", ["plain2d", "plastic2d", "stz2d", "planar3d", "stz3d"])
+def test_graph_replay_matches_eager(tmp_path, case, precision):
     """hipGraph replay of whole steps (device step counter) gives the same
     fields, slip and fronts as eager launches, over several replays with
-    host reads in between."""
+    host reads in between, in both bulk precisions."""
     from fdfault_amd import lib
     kw = dict(nx=61, ny=31, dx=0.1, nt=12)
-    if plastic:
+    if case == "plastic2d":
         kw.update(material="plastic", plastic=dict(mu=0.68, c=0., beta=0.35, eta=0.05 * 32.04),
                   s0=(-100., 52., 0., -120., 0., -55.), loads=NUC2)
+    elif case.startswith("stz"):
+        kw.update(law="stz", stz=dict(STZ_PARAMS, chi0=0.07))
+    if case.endswith("3d"):
+        kw.update(ndim=3, nx=31, ny=16, nz=25)
+        if case == "planar3d":
+            kw.update(loads=[Pert("constant", 0., 0., 0., 0., 0., (-120., 70., 0.))] + NUC3)
     p = write_case(str(tmp_path), "gr", **kw)
-    a = lib.Solver(p)
-    b = lib.Solver(p, graph=True)
+    a = lib.Solver(p, precision=precision)
+    b = lib.Solver(p, precision=precision, graph=True)
     for k in range(4):
         a.step(3)
         b.step(3)
         assert np.array_equal(a.array("f"), b.array("f"))
-        for w in ("U", "V", "S", "Sn"):
-            assert np.array_equal(a.iface(0, w), b.iface(0, w))
+        for w in ("U", "V", "S", "Sn") + (("state",) if case.startswith("stz") else ()):
+            assert np.array_equal(a.iface(0, w), b.iface(0, w)), w
         assert np.array_equal(a.front(0), b.front(0))
+    assert np.abs(a.iface(0, "V")).max() > 0
     a.close()
     b.close()
 
