@@ -162,3 +162,56 @@ def test_stz_proposal_gain_and_parity(tmp_path):
     o.close()
     plain.close()
     prop.close()
+
+
+def _stz3d_case(tmp, name, seed, nx=41, ny=16, nz=25, nt=20):
+    d = os.path.join(str(tmp), name)
+    os.makedirs(d, exist_ok=True)
+    chi = np.random.default_rng(seed).uniform(0.06, 0.08, size=(nx, nz))
+    sp = os.path.join(d, "state.bin")
+    chi.tofile(sp)
+    loads = [Pert("constant", 0., 0., 0., 0., 0., (-120., 70., 0.))]
+    return write_case(str(tmp), name, ndim=3, nx=nx, ny=ny, nz=nz, dx=0.1, nt=nt, law="stz",
+                      stz=dict(STZ_PARAMS, chi0=0., statefile=sp), loads=loads)
+
+
+@pytest.mark.parametrize("ndim,precision", [(3, 0), (2, 1)])
+def test_stz_proposal_other_dimension_and_precision(tmp_path, ndim, precision):
+    """The proposal in the two fault-kernel forms the other tests leave out:
+    3-D with FP64 bulk and 2-D with FP32 bulk.  Trained on plain solves of
+    one seed, used on another: fewer Newton iterations, (almost) no
+    fallbacks, and the plain solver's results (FP64 bulk: within 1e-10;
+    FP32 bulk: within 1e-5, the FP32 fields' rounding), every solve to the
+    plain solver's tolerance."""
+    from fdfault_amd import lib
+    nt = 20
+    mk = (lambda n, s: _stz3d_case(tmp_path, n, s, nt=nt)) if ndim == 3 else \
+         (lambda n, s: _stz_case(tmp_path, n, seed=s, nx=121, ny=31, nt=nt))
+    a = lib.Solver(mk("train", 1), precision=precision)
+    a.stz_log(400000)
+    a.step(nt)
+    rows = a.stz_log_read()
+    a.close()
+    print("ndim", ndim, "precision", precision, "logged", rows.shape, flush=True)
+    P, losses = lib.mlp_train(rows, epochs=40, batch=512, lr=3e-3, seed=0)
+    case = mk("use", 2)
+    plain = lib.Solver(case, precision=precision)
+    plain.step(nt)
+    sp = plain.stz_stats()
+    prop = lib.Solver(case, precision=precision)
+    prop.stz_set_model(P)
+    prop.step(nt)
+    sm = prop.stz_stats()
+    mp, mm = sp[1] / sp[0], sm[1] / sm[0]
+    print("plain %.2f iterations per solve; proposal %.2f, used %d of %d, fallbacks %d"
+          % (mp, mm, sm[2], sm[0], sm[3]), flush=True)
+    assert sp[0] == sm[0] and sp[2] == 0 and sp[3] == 0
+    assert sm[2] >= 0.95 * sm[0] and sm[3] <= 0.05 * sm[0]
+    assert mm < 0.7 * mp
+    tol = 1e-10 if precision == 0 else 1e-5
+    for w in ("U", "V", "S", "state"):
+        r = _rel(prop.iface(0, w), plain.iface(0, w))
+        print(w, "proposal vs plain", r, flush=True)
+        assert r <= tol, w
+    plain.close()
+    prop.close()
