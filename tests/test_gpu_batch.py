@@ -186,6 +186,12 @@ def test_batch_rejects_unsupported_instance(tmp_path):
     diss = _case(tmp_path, "d", 61, 31, nt=5, cdiss=0.1)
     with pytest.raises(lib.FdfaultError, match="not rectilinear"):
         lib.Batch([good, diss], precision=lib.FP32)
+    s2 = _case(tmp_path, "s2", 61, 31, nt=5, sbporder=2)
+    with pytest.raises(lib.FdfaultError, match="share sbporder and rkorder"):
+        lib.Batch([good, s2], precision=lib.FP32)
+    d3 = write_case(str(tmp_path), "d3", ndim=3, nx=25, ny=14, nz=17, dx=0.1, nt=5)
+    with pytest.raises(lib.FdfaultError, match="instance 1 is not a 2-D mode II elastic problem"):
+        lib.Batch([good, d3], precision=lib.FP32)
 
 
 @pytest.mark.parametrize("precision", ["fp32", "fp64"])
