@@ -116,7 +116,7 @@ def test_layout_against_oracle(tmp_path, name, exact):
     o.close()
 
 
-@pytest.mark.parametrize("name", ["x_fault_2d", "grid_2x2", "z_fault_3d"])
+@pytest.mark.parametrize("name", sorted(LAYOUTS))
 def test_layout_fp32(tmp_path, name):
     """FP32 bulk / FP64 fault on the same layouts: slip within 1e-4 of the
     FP64 oracle's, relative to its largest value."""
