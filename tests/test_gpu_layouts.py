@@ -161,3 +161,28 @@ def test_curved_fault_3d_against_oracle(tmp_path, exact):
     assert np.abs(o.iface(0, "V")).max() > 1e-3
     g.close()
     o.close()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("name", ["grid_2x2", "stack_3", "x_fault_3d"])
+def test_layout_graph_replay(tmp_path, name, precision):
+    """Several interfaces (and laws) per problem under hipGraph replay of
+    whole steps: bit-identical to eager launches, with reads in between."""
+    from fdfault_amd import lib
+    kw = LAYOUTS[name]
+    p = layout(tmp_path, name, nt=12, **kw)
+    a = lib.Solver(p, precision=precision)
+    b = lib.Solver(p, precision=precision, graph=True)
+    for k in range(3):
+        a.step(4)
+        b.step(4)
+        assert np.array_equal(a.array("f"), b.array("f"))
+        for i in range(a.info.nifaces):
+            try:
+                ua = a.iface(i, "U")
+            except lib.FdfaultError:
+                continue  # locked: no slip arrays
+            assert np.array_equal(ua, b.iface(i, "U")), i
+            assert np.array_equal(a.iface(i, "V"), b.iface(i, "V")), i
+    a.close()
+    b.close()
