@@ -95,3 +95,39 @@ def test_snapshot_refusals(tmp_path):
         a.load_state(bad)
     assert np.array_equal(a.array("f"), before)  # refused loads left the state alone
     a.close()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_resume_multi_interface_layout(tmp_path, precision):
+    """Four interfaces of three laws (slip weakening, locked, frictionless)
+    on a 2 x 2 block grid: the snapshot carries every frictional
+    interface's arrays, and the resumed run is bit-identical."""
+    from fdfault_amd import lib
+    from test_gpu_layouts import LAYOUTS, layout
+    p = layout(tmp_path, "grid", nt=30, **LAYOUTS["grid_2x2"])
+    snap = str(tmp_path / "snap.bin")
+
+    def state(s):
+        out = {"f": s.array("f"), "df": s.array("df")}
+        for i in range(s.info.nifaces):
+            for w in ("U", "V", "S", "Sn", "du"):
+                try:
+                    out["%d_%s" % (i, w)] = s.iface(i, w)
+                except lib.FdfaultError:
+                    pass  # the locked interface has no slip arrays
+        return out
+
+    a = lib.Solver(p, precision=precision)
+    a.step(10)
+    a.save_state(snap)
+    a.step(8)
+    want = state(a)
+    a.close()
+    b = lib.Solver(p, precision=precision)
+    b.load_state(snap)
+    b.step(8)
+    got = state(b)
+    b.close()
+    assert len(want) >= 2 + 3 * 5
+    for k in want:
+        assert np.array_equal(want[k], got[k]), k
